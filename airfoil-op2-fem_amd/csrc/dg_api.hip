@@ -1,0 +1,807 @@
+// C ABI of the DG Euler solver (declared in include/dg_euler.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/dg_euler.h"
+#include "dg_internal.h"
+#include "dg_mesh.h"
+#include "dg_ops.h"
+
+namespace {
+thread_local std::string g_err;
+}
+
+struct dg_ctx {
+  int N = 0, Np = 0, Nfp = 0, NF = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // host copies of the constants (op_decl_const names)
+  std::vector<double> r, s, Dr, Ds, Drw, Dsw, LIFT, ones;
+  std::vector<int> FMASK;
+  double gam = 1.4, bc_mach = 0.4, bc_alpha = 0.0, bc_p = 1.0, bc_r = 1.0, bc_u = 0.0,
+         bc_v = 0.0, bc_e = 0.0;
+  double *d_ops = nullptr;
+  int *d_fmask = nullptr;
+  int max_blocks = 0;
+  // declared mesh
+  int ncells = -1, nedges = 0, nbedges = 0;
+  int *d_fmap = nullptr, *d_b2c = nullptr, *d_bnum = nullptr, *d_btype = nullptr;
+  unsigned long long *d_red = nullptr;
+};
+
+struct dg_mesh {
+  dg::Mesh m;
+};
+
+namespace {
+
+int fail(dg_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  g_err = msg;
+  return code;
+}
+
+int hipfail(dg_ctx *c, hipError_t e, const char *what) {
+  return fail(c, DG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHECK(ctx, call)                                   \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return hipfail((ctx), e_, #call);   \
+  } while (0)
+
+dg::Launch launch_of(const dg_ctx *c) {
+  dg::Launch L;
+  L.N = c->N; L.Np = c->Np; L.Nfp = c->Nfp; L.NF = c->NF;
+  L.stream = c->stream;
+  L.ops = c->d_ops;
+  L.fmask = c->d_fmask;
+  L.ph = {c->gam, c->bc_r, c->bc_u, c->bc_v, c->bc_e, c->bc_p};
+  uint64_t mask = 0;
+  for (int f : c->FMASK) mask |= (1ull << f);
+  L.facenodes = mask;
+  L.max_blocks = c->max_blocks;
+  return L;
+}
+
+int upload_ops(dg_ctx *c) {
+  const int Np = c->Np, NF = c->NF;
+  std::vector<double> buf(dg::ops_size(Np, NF));
+  double *p = buf.data();
+  std::memcpy(p, c->Drw.data(), sizeof(double) * Np * Np); p += Np * Np;
+  std::memcpy(p, c->Dsw.data(), sizeof(double) * Np * Np); p += Np * Np;
+  std::memcpy(p, c->LIFT.data(), sizeof(double) * Np * NF); p += Np * NF;
+  std::memcpy(p, c->Dr.data(), sizeof(double) * Np * Np); p += Np * Np;
+  std::memcpy(p, c->Ds.data(), sizeof(double) * Np * Np); p += Np * Np;
+  std::memcpy(p, c->r.data(), sizeof(double) * Np); p += Np;
+  std::memcpy(p, c->s.data(), sizeof(double) * Np);
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(c->d_ops, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(c->d_fmask, c->FMASK.data(), sizeof(int) * NF, hipMemcpyHostToDevice));
+  return DG_OK;
+}
+
+bool mesh_declared(const dg_ctx *c) { return c->ncells >= 0; }
+
+// State records are moved as 16-byte vectors.
+bool al16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+const char *dg_last_error(const dg_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int dg_generate_operators(int order, double *r, double *s, double *Dr, double *Ds, double *Drw,
+                          double *Dsw, double *LIFT, int *FMASK, double *mass, double *face_mass) {
+  dg::Operators o;
+  const char *e = nullptr;
+  if (order < 1 || order > 8 || !dg::build_operators(order, o, &e))
+    return fail(nullptr, DG_ERR_ARG, std::string("dg_generate_operators: ") + (e ? e : "bad order"));
+  auto put = [](double *dst, const std::vector<double> &v) {
+    if (dst) std::memcpy(dst, v.data(), sizeof(double) * v.size());
+  };
+  put(r, o.r); put(s, o.s); put(Dr, o.Dr); put(Ds, o.Ds); put(Drw, o.Drw); put(Dsw, o.Dsw);
+  put(LIFT, o.LIFT); put(mass, o.mass); put(face_mass, o.face_mass);
+  if (FMASK) std::memcpy(FMASK, o.FMASK.data(), sizeof(int) * o.FMASK.size());
+  return DG_OK;
+}
+
+int dg_ctx_create(int order, dg_ctx **out) {
+  if (!out) return fail(nullptr, DG_ERR_ARG, "dg_ctx_create: null output pointer");
+  *out = nullptr;
+  if (order < 1 || order > 4)
+    return fail(nullptr, DG_ERR_ARG, "dg_ctx_create: order must be in [1, 4] (the calc_dt "
+                                     "face-scale indexing of the reference needs Np <= 3*Nfp)");
+  dg::Operators ops;
+  const char *e = nullptr;
+  if (!dg::build_operators(order, ops, &e)) return fail(nullptr, DG_ERR_ARG, e ? e : "operators");
+  dg_ctx *c = new dg_ctx();
+  c->N = order; c->Np = ops.Np; c->Nfp = ops.Nfp; c->NF = 3 * ops.Nfp;
+  c->r = ops.r; c->s = ops.s; c->Dr = ops.Dr; c->Ds = ops.Ds; c->Drw = ops.Drw;
+  c->Dsw = ops.Dsw; c->LIFT = ops.LIFT; c->FMASK = ops.FMASK;
+  c->ones.assign(c->Np, 1.0);
+  hipError_t he = hipGetDevice(&c->device);
+  if (he == hipSuccess) {
+    hipDeviceProp_t prop;
+    he = hipGetDeviceProperties(&prop, c->device);
+    if (he == hipSuccess) c->max_blocks = prop.multiProcessorCount * 2;
+  }
+  if (he == hipSuccess) he = hipMalloc(&c->d_ops, sizeof(double) * dg::ops_size(c->Np, c->NF));
+  if (he == hipSuccess) he = hipMalloc(&c->d_fmask, sizeof(int) * c->NF);
+  if (he == hipSuccess) he = hipMalloc(&c->d_red, sizeof(unsigned long long) * 4);
+  if (he != hipSuccess) {
+    std::string msg = std::string("dg_ctx_create: ") + hipGetErrorString(he);
+    dg_ctx_destroy(c);
+    return fail(nullptr, DG_ERR_HIP, msg);
+  }
+  // free stream of src/airfoil.cpp:92-99 (alpha = 0)
+  c->bc_u = std::sqrt(c->gam * c->bc_p / c->bc_r) * c->bc_mach;
+  c->bc_v = 0.0;
+  c->bc_e = c->bc_p / (c->gam - 1.0) + 0.5 * c->bc_r * (c->bc_u * c->bc_u);
+  int rc = upload_ops(c);
+  if (rc != DG_OK) {
+    std::string msg = c->err;
+    dg_ctx_destroy(c);
+    return fail(nullptr, rc, msg);
+  }
+  *out = c;
+  return DG_OK;
+}
+
+void dg_ctx_destroy(dg_ctx *c) {
+  if (!c) return;
+  (void)hipFree(c->d_ops); (void)hipFree(c->d_fmask); (void)hipFree(c->d_red);
+  (void)hipFree(c->d_fmap); (void)hipFree(c->d_b2c); (void)hipFree(c->d_bnum);
+  (void)hipFree(c->d_btype);
+  delete c;
+}
+
+int dg_order(const dg_ctx *c, int *N, int *Np, int *Nfp) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_order: null context");
+  if (N) *N = c->N;
+  if (Np) *Np = c->Np;
+  if (Nfp) *Nfp = c->Nfp;
+  return DG_OK;
+}
+
+int dg_set_stream(dg_ctx *c, void *s) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_set_stream: null context");
+  c->stream = (hipStream_t)s;
+  return DG_OK;
+}
+
+namespace {
+struct ConstSlot {
+  std::vector<double> *vec;
+  double *scalar;
+  int dim;
+};
+
+bool find_const(dg_ctx *c, const char *name, ConstSlot &s) {
+  const int Np = c->Np, NF = c->NF;
+  struct { const char *n; std::vector<double> *v; double *sc; int dim; } tab[] = {
+      {"gam", nullptr, &c->gam, 1},         {"bc_mach", nullptr, &c->bc_mach, 1},
+      {"bc_alpha", nullptr, &c->bc_alpha, 1}, {"bc_p", nullptr, &c->bc_p, 1},
+      {"bc_r", nullptr, &c->bc_r, 1},       {"bc_u", nullptr, &c->bc_u, 1},
+      {"bc_v", nullptr, &c->bc_v, 1},       {"bc_e", nullptr, &c->bc_e, 1},
+      {"ones", &c->ones, nullptr, Np},      {"r", &c->r, nullptr, Np},
+      {"s", &c->s, nullptr, Np},            {"Dr", &c->Dr, nullptr, Np * Np},
+      {"Ds", &c->Ds, nullptr, Np * Np},     {"Drw", &c->Drw, nullptr, Np * Np},
+      {"Dsw", &c->Dsw, nullptr, Np * Np},   {"LIFT", &c->LIFT, nullptr, Np * NF},
+  };
+  for (auto &t : tab)
+    if (std::strcmp(t.n, name) == 0) {
+      s = {t.v, t.sc, t.dim};
+      return true;
+    }
+  return false;
+}
+}  // namespace
+
+int dg_set_const(dg_ctx *c, const char *name, int dim, const void *data) {
+  if (!c || !name || !data) return fail(c, DG_ERR_ARG, "dg_set_const: null argument");
+  if (std::strcmp(name, "FMASK") == 0) {
+    if (dim != c->NF) return fail(c, DG_ERR_ARG, "dg_set_const: FMASK needs dim 3*Nfp");
+    const int *f = (const int *)data;
+    for (int i = 0; i < dim; i++)
+      if (f[i] < 0 || f[i] >= c->Np) return fail(c, DG_ERR_ARG, "dg_set_const: FMASK entry out of range");
+    c->FMASK.assign(f, f + dim);
+    return upload_ops(c);
+  }
+  ConstSlot s;
+  if (!find_const(c, name, s))
+    return fail(c, DG_ERR_NAME, std::string("dg_set_const: unknown constant name '") + name + "'");
+  if (dim != s.dim)
+    return fail(c, DG_ERR_ARG, std::string("dg_set_const: wrong dim for '") + name + "'");
+  const double *d = (const double *)data;
+  if (s.scalar) {
+    *s.scalar = d[0];
+    return DG_OK;
+  }
+  s.vec->assign(d, d + dim);
+  return upload_ops(c);
+}
+
+int dg_get_const(const dg_ctx *cc, const char *name, int dim, void *out) {
+  dg_ctx *c = const_cast<dg_ctx *>(cc);
+  if (!c || !name || !out) return fail(c, DG_ERR_ARG, "dg_get_const: null argument");
+  if (std::strcmp(name, "FMASK") == 0) {
+    if (dim != c->NF) return fail(c, DG_ERR_ARG, "dg_get_const: FMASK needs dim 3*Nfp");
+    std::memcpy(out, c->FMASK.data(), sizeof(int) * dim);
+    return DG_OK;
+  }
+  ConstSlot s;
+  if (!find_const(c, name, s))
+    return fail(c, DG_ERR_NAME, std::string("dg_get_const: unknown constant name '") + name + "'");
+  if (dim != s.dim) return fail(c, DG_ERR_ARG, std::string("dg_get_const: wrong dim for '") + name + "'");
+  if (s.scalar)
+    *(double *)out = *s.scalar;
+  else
+    std::memcpy(out, s.vec->data(), sizeof(double) * dim);
+  return DG_OK;
+}
+
+int dg_set_freestream(dg_ctx *c, double gam, double mach, double alpha_deg, double p, double rho) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_set_freestream: null context");
+  if (!(gam > 1.0) || !(p > 0.0) || !(rho > 0.0) || !(mach >= 0.0))
+    return fail(c, DG_ERR_ARG, "dg_set_freestream: need gam > 1, p > 0, rho > 0, mach >= 0");
+  c->gam = gam; c->bc_mach = mach; c->bc_alpha = alpha_deg; c->bc_p = p; c->bc_r = rho;
+  const double a = std::sqrt(gam * p / rho) * mach;
+  if (alpha_deg != 0.0) {
+    c->bc_u = std::sin((M_PI / 2.0) - (alpha_deg * M_PI / 180.0)) * a;
+    c->bc_v = std::cos((M_PI / 2.0) - (alpha_deg * M_PI / 180.0)) * a;
+    c->bc_e = p / (gam - 1.0) + 0.5 * rho * (c->bc_u * c->bc_u + c->bc_v * c->bc_v);
+  } else {
+    c->bc_u = a;
+    c->bc_v = 0.0;
+    c->bc_e = p / (gam - 1.0) + 0.5 * rho * (c->bc_u * c->bc_u);
+  }
+  return DG_OK;
+}
+
+int dg_decl_mesh(dg_ctx *c, int ncells, const int *cell2nodes, int nedges, const int *edge2cells,
+                 const int *edgeNum, int nbedges, const int *bedge2cells, const int *bedgeNum,
+                 const int *bedge_type) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_decl_mesh: null context");
+  if (ncells < 0 || nedges < 0 || nbedges < 0 || (nedges > 0 && (!edge2cells || !edgeNum)) ||
+      (nbedges > 0 && (!bedge2cells || !bedgeNum || !bedge_type)))
+    return fail(c, DG_ERR_ARG, "dg_decl_mesh: bad sizes or null arrays");
+  if ((long long)ncells * 4 >= (1ll << 31))
+    return fail(c, DG_ERR_ARG, "dg_decl_mesh: too many cells for the 32-bit face map");
+  std::vector<int> fmap;
+  std::string err;
+  if (!dg::build_face_map(ncells, nedges, edge2cells, edgeNum, nbedges, bedge2cells, bedgeNum,
+                          bedge_type, cell2nodes, fmap, err))
+    return fail(c, DG_ERR_MESH, "dg_decl_mesh: " + err);
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(c->d_fmap); (void)hipFree(c->d_b2c); (void)hipFree(c->d_bnum);
+  (void)hipFree(c->d_btype);
+  c->d_fmap = c->d_b2c = c->d_bnum = c->d_btype = nullptr;
+  c->ncells = -1;
+  if (ncells > 0) {
+    HIPCHECK(c, hipMalloc(&c->d_fmap, sizeof(int) * fmap.size()));
+    HIPCHECK(c, hipMemcpy(c->d_fmap, fmap.data(), sizeof(int) * fmap.size(), hipMemcpyHostToDevice));
+  }
+  if (nbedges > 0) {
+    HIPCHECK(c, hipMalloc(&c->d_b2c, sizeof(int) * nbedges));
+    HIPCHECK(c, hipMalloc(&c->d_bnum, sizeof(int) * nbedges));
+    HIPCHECK(c, hipMalloc(&c->d_btype, sizeof(int) * nbedges));
+    HIPCHECK(c, hipMemcpy(c->d_b2c, bedge2cells, sizeof(int) * nbedges, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_bnum, bedgeNum, sizeof(int) * nbedges, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_btype, bedge_type, sizeof(int) * nbedges, hipMemcpyHostToDevice));
+  }
+  c->ncells = ncells; c->nedges = nedges; c->nbedges = nbedges;
+  return DG_OK;
+}
+
+#define NEED_MESH(c, fn)                                                              \
+  do {                                                                                \
+    if (!(c)) return fail(nullptr, DG_ERR_ARG, fn ": null context");                  \
+    if (!mesh_declared(c)) return fail((c), DG_ERR_STATE, fn ": no mesh declared");   \
+  } while (0)
+
+int dg_init_grid(dg_ctx *c, const double *cell_xy, double *nodeX, double *nodeY, double *x,
+                 double *y, double *xr, double *yr, double *xs, double *ys, double *rx, double *ry,
+                 double *sx, double *sy, double *nx, double *ny, double *fscale) {
+  NEED_MESH(c, "dg_init_grid");
+  if (c->ncells > 0 && (!cell_xy || !nodeX || !nodeY || !x || !y || !xr || !yr || !xs || !ys ||
+                        !rx || !ry || !sx || !sy || !nx || !ny || !fscale))
+    return fail(c, DG_ERR_ARG, "dg_init_grid: null array");
+  HIPCHECK(c, dg::launch_init_grid(launch_of(c), c->ncells, cell_xy, nodeX, nodeY, x, y, xr, yr,
+                                   xs, ys, rx, ry, sx, sy, nx, ny, fscale, nullptr));
+  return DG_OK;
+}
+
+int dg_set_ic(dg_ctx *c, double *q, double *w) {
+  NEED_MESH(c, "dg_set_ic");
+  if (c->ncells > 0 && (!q || !w)) return fail(c, DG_ERR_ARG, "dg_set_ic: null array");
+  HIPCHECK(c, dg::launch_set_ic(launch_of(c), c->ncells, q, w));
+  return DG_OK;
+}
+
+int dg_neighbour_zero(dg_ctx *c, double *ext) {
+  NEED_MESH(c, "dg_neighbour_zero");
+  if (c->ncells > 0 && !ext) return fail(c, DG_ERR_ARG, "dg_neighbour_zero: null array");
+  HIPCHECK(c, dg::launch_zero(launch_of(c), (size_t)c->ncells * c->NF * 4, ext));
+  return DG_OK;
+}
+
+int dg_get_neighbour_q(dg_ctx *c, const double *nodeX, const double *nodeY, const double *w,
+                       double *ext) {
+  (void)nodeX; (void)nodeY;
+  NEED_MESH(c, "dg_get_neighbour_q");
+  if (c->ncells > 0 && (!w || !ext)) return fail(c, DG_ERR_ARG, "dg_get_neighbour_q: null array");
+  HIPCHECK(c, dg::launch_get_neighbour_q(launch_of(c), c->ncells, c->d_fmap, w, ext));
+  return DG_OK;
+}
+
+int dg_get_bedge_q(dg_ctx *c, const double *nx, const double *ny, const double *w, double *ext) {
+  NEED_MESH(c, "dg_get_bedge_q");
+  if (c->nbedges > 0 && (!nx || !ny || !w || !ext))
+    return fail(c, DG_ERR_ARG, "dg_get_bedge_q: null array");
+  HIPCHECK(c, dg::launch_get_bedge_q(launch_of(c), c->nbedges, c->d_b2c, c->d_bnum, c->d_btype, nx,
+                                     ny, w, ext));
+  return DG_OK;
+}
+
+int dg_euler_rhs(dg_ctx *c, const double *w, double *ext, const double *rx, const double *ry,
+                 const double *sx, const double *sy, const double *fscale, const double *nx,
+                 const double *ny, double *qRHS) {
+  NEED_MESH(c, "dg_euler_rhs");
+  if (c->ncells > 0 && (!w || !ext || !rx || !ry || !sx || !sy || !fscale || !nx || !ny || !qRHS))
+    return fail(c, DG_ERR_ARG, "dg_euler_rhs: null array");
+  if (!al16(w) || !al16(ext) || !al16(qRHS))
+    return fail(c, DG_ERR_ARG, "dg_euler_rhs: state arrays must be 16-byte aligned");
+  dg::StageArgs a{};
+  a.w = w; a.ext = ext; a.rx = rx; a.ry = ry; a.sx = sx; a.sy = sy;
+  a.fscale = fscale; a.nx = nx; a.ny = ny; a.out = qRHS; a.ncells = c->ncells;
+  HIPCHECK(c, dg::launch_rhs(launch_of(c), a, false, dg::EPI_RHS));
+  return DG_OK;
+}
+
+int dg_rhs_fused(dg_ctx *c, const double *w, const double *rx, const double *ry, const double *sx,
+                 const double *sy, const double *fscale, const double *nx, const double *ny,
+                 double *qRHS) {
+  NEED_MESH(c, "dg_rhs_fused");
+  if (c->ncells > 0 && (!w || !rx || !ry || !sx || !sy || !fscale || !nx || !ny || !qRHS))
+    return fail(c, DG_ERR_ARG, "dg_rhs_fused: null array");
+  if (!al16(w) || !al16(qRHS)) return fail(c, DG_ERR_ARG, "dg_rhs_fused: state arrays must be 16-byte aligned");
+  dg::StageArgs a{};
+  a.w = w; a.rx = rx; a.ry = ry; a.sx = sx; a.sy = sy; a.fscale = fscale; a.nx = nx; a.ny = ny;
+  a.fmap = c->d_fmap; a.out = qRHS; a.ncells = c->ncells;
+  HIPCHECK(c, dg::launch_rhs(launch_of(c), a, true, dg::EPI_RHS));
+  return DG_OK;
+}
+
+int dg_set_workingQ(dg_ctx *c, double dt, int stage, const double *q, const double *k1,
+                    const double *k2, double *w) {
+  NEED_MESH(c, "dg_set_workingQ");
+  if (c->ncells > 0 && (!q || !k1 || !w || (stage != 0 && !k2)))
+    return fail(c, DG_ERR_ARG, "dg_set_workingQ: null array");
+  HIPCHECK(c, dg::launch_set_workingQ(launch_of(c), c->ncells, dt, stage, q, k1, k2 ? k2 : k1, w));
+  return DG_OK;
+}
+
+int dg_update_Q(dg_ctx *c, double dt, double *q, const double *k1, const double *k2,
+                const double *k3, double *w) {
+  NEED_MESH(c, "dg_update_Q");
+  if (c->ncells > 0 && (!q || !k1 || !k2 || !k3 || !w))
+    return fail(c, DG_ERR_ARG, "dg_update_Q: null array");
+  HIPCHECK(c, dg::launch_update_Q(launch_of(c), c->ncells, dt, q, k1, k2, k3, w));
+  return DG_OK;
+}
+
+int dg_calc_dt(dg_ctx *c, const double *q, const double *fscale, double *dt1) {
+  NEED_MESH(c, "dg_calc_dt");
+  if (!dt1 || (c->ncells > 0 && (!q || !fscale))) return fail(c, DG_ERR_ARG, "dg_calc_dt: null argument");
+  HIPCHECK(c, hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream));
+  HIPCHECK(c, dg::launch_calc_dt(launch_of(c), c->ncells, q, fscale, c->d_red));
+  unsigned long long bits = 0;
+  HIPCHECK(c, hipMemcpyAsync(&bits, c->d_red, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  double v;
+  std::memcpy(&v, &bits, sizeof(v));
+  if (v > *dt1 || std::isnan(v)) *dt1 = v;
+  return DG_OK;
+}
+
+/* ---------------------------- meshes ---------------------------- */
+
+int dg_mesh_periodic(int nx, int ny, double x0, double y0, double Lx, double Ly, uint64_t seed,
+                     double jitter, dg_mesh **out) {
+  if (!out) return fail(nullptr, DG_ERR_ARG, "dg_mesh_periodic: null output");
+  *out = nullptr;
+  if (nx < 3 || ny < 3 || !(Lx > 0) || !(Ly > 0) || !(jitter >= 0.0) || jitter >= 0.5)
+    return fail(nullptr, DG_ERR_ARG, "dg_mesh_periodic: need nx, ny >= 3, L > 0, 0 <= jitter < 0.5");
+  dg_mesh *m = new dg_mesh();
+  std::string err;
+  if (!dg::gen_periodic(nx, ny, x0, y0, Lx, Ly, seed, jitter, m->m, err)) {
+    delete m;
+    return fail(nullptr, DG_ERR_MESH, "dg_mesh_periodic: " + err);
+  }
+  *out = m;
+  return DG_OK;
+}
+
+int dg_mesh_ogrid(int ncirc, int nrad, double stretch, double rfar, dg_mesh **out) {
+  if (!out) return fail(nullptr, DG_ERR_ARG, "dg_mesh_ogrid: null output");
+  *out = nullptr;
+  dg_mesh *m = new dg_mesh();
+  std::string err;
+  if (!dg::gen_ogrid(ncirc, nrad, stretch, rfar, m->m, err)) {
+    delete m;
+    return fail(nullptr, DG_ERR_MESH, "dg_mesh_ogrid: " + err);
+  }
+  *out = m;
+  return DG_OK;
+}
+
+int dg_mesh_read_vtk(const char *path, dg_mesh **out) {
+  if (!out || !path) return fail(nullptr, DG_ERR_ARG, "dg_mesh_read_vtk: null argument");
+  *out = nullptr;
+  dg_mesh *m = new dg_mesh();
+  std::string err;
+  if (!dg::read_vtk(path, m->m, err)) {
+    delete m;
+    return fail(nullptr, DG_ERR_MESH, "dg_mesh_read_vtk: " + err);
+  }
+  *out = m;
+  return DG_OK;
+}
+
+void dg_mesh_destroy(dg_mesh *m) { delete m; }
+
+int dg_mesh_sizes(const dg_mesh *m, int *nnodes, int *ncells, int *nedges, int *nbedges) {
+  if (!m) return fail(nullptr, DG_ERR_ARG, "dg_mesh_sizes: null mesh");
+  if (nnodes) *nnodes = m->m.nnodes;
+  if (ncells) *ncells = m->m.ncells;
+  if (nedges) *nedges = m->m.nedges;
+  if (nbedges) *nbedges = m->m.nbedges;
+  return DG_OK;
+}
+
+int dg_mesh_get(const dg_mesh *m, const char *field, void *out) {
+  if (!m || !field || !out) return fail(nullptr, DG_ERR_ARG, "dg_mesh_get: null argument");
+  const dg::Mesh &M = m->m;
+  auto cpy_i = [&](const std::vector<int> &v) { std::memcpy(out, v.data(), sizeof(int) * v.size()); };
+  auto cpy_d = [&](const std::vector<double> &v) {
+    std::memcpy(out, v.data(), sizeof(double) * v.size());
+  };
+  std::string f(field);
+  if (f == "coords") cpy_d(M.coords);
+  else if (f == "cell_xy") cpy_d(M.cell_xy);
+  else if (f == "cells") cpy_i(M.cells);
+  else if (f == "edge2cells") cpy_i(M.edge2cells);
+  else if (f == "edgeNum") cpy_i(M.edgeNum);
+  else if (f == "edge2nodes") cpy_i(M.edge2nodes);
+  else if (f == "bedge2cells") cpy_i(M.bedge2cells);
+  else if (f == "bedgeNum") cpy_i(M.bedgeNum);
+  else if (f == "bedge_type") cpy_i(M.bedge_type);
+  else if (f == "bedge2nodes") cpy_i(M.bedge2nodes);
+  else return fail(nullptr, DG_ERR_NAME, "dg_mesh_get: unknown field '" + f + "'");
+  return DG_OK;
+}
+
+}  // extern "C"
+
+/* ---------------------------- solver ---------------------------- */
+
+struct dg_solver {
+  dg_ctx *ctx = nullptr;
+  int K = 0;
+  std::map<std::string, double *> dats;
+  double *q = nullptr, *w = nullptr, *rk[3] = {nullptr, nullptr, nullptr}, *ext = nullptr;
+  unsigned long long *dt1 = nullptr;  // two ping-pong slots
+  double *t_dev = nullptr;
+  unsigned int *flag = nullptr;
+  long long step = 0;
+  bool dt_valid = false;
+  double cfl_last = 1.0;
+  bool profiling = false;
+  struct Rec { std::string name; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, int>> ktimes;
+};
+
+namespace {
+
+hipEvent_t take_event(dg_solver *s) {
+  if (!s->pool.empty()) {
+    hipEvent_t e = s->pool.back();
+    s->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+template <class F>
+hipError_t timed(dg_solver *s, const char *name, F &&f) {
+  if (!s->profiling) return f();
+  dg_solver::Rec r{name, take_event(s), take_event(s)};
+  (void)hipEventRecord(r.a, s->ctx->stream);
+  hipError_t e = f();
+  (void)hipEventRecord(r.b, s->ctx->stream);
+  s->pending.push_back(r);
+  return e;
+}
+
+void drain_profile(dg_solver *s) {
+  if (s->pending.empty()) return;
+  (void)hipStreamSynchronize(s->ctx->stream);
+  for (auto &r : s->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      auto &t = s->ktimes[r.name];
+      t.first += ms;
+      t.second += 1;
+    }
+    s->pool.push_back(r.a);
+    s->pool.push_back(r.b);
+  }
+  s->pending.clear();
+}
+
+double *dat(dg_solver *s, const char *n) { return s->dats[n]; }
+
+}  // namespace
+
+extern "C" {
+
+void dg_solver_destroy(dg_solver *s) {
+  if (!s) return;
+  if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+  for (auto &kv : s->dats) (void)hipFree(kv.second);
+  (void)hipFree(s->dt1); (void)hipFree(s->t_dev); (void)hipFree(s->flag);
+  for (auto &r : s->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : s->pool) (void)hipEventDestroy(e);
+  delete s;
+}
+
+int dg_solver_create(dg_ctx *c, const dg_mesh *mesh, dg_solver **out) {
+  if (!c || !mesh || !out) return fail(c, DG_ERR_ARG, "dg_solver_create: null argument");
+  *out = nullptr;
+  const dg::Mesh &M = mesh->m;
+  int rc = dg_decl_mesh(c, M.ncells, M.cells.data(), M.nedges, M.edge2cells.data(),
+                        M.edgeNum.data(), M.nbedges, M.bedge2cells.data(), M.bedgeNum.data(),
+                        M.bedge_type.data());
+  if (rc != DG_OK) return rc;
+  dg_solver *s = new dg_solver();
+  s->ctx = c;
+  s->K = M.ncells;
+  const size_t K = M.ncells, Np = c->Np, NF = c->NF;
+  struct { const char *n; size_t sz; } layout[] = {
+      {"q", K * Np * 4},   {"workingQ", K * Np * 4}, {"rk1", K * Np * 4}, {"rk2", K * Np * 4},
+      {"rk3", K * Np * 4}, {"exteriorQ", K * NF * 4}, {"x", K * Np},      {"y", K * Np},
+      {"xr", K * Np},      {"yr", K * Np},           {"xs", K * Np},      {"ys", K * Np},
+      {"rx", K * Np},      {"ry", K * Np},           {"sx", K * Np},      {"sy", K * Np},
+      {"nx", K * NF},      {"ny", K * NF},           {"fscale", K * NF},  {"nodeX", K * 3},
+      {"nodeY", K * 3},    {"J", K},                 {"cell_xy", K * 6},
+  };
+  for (auto &l : layout) {
+    double *p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(double) * (l.sz ? l.sz : 1));
+    if (e != hipSuccess) {
+      dg_solver_destroy(s);
+      return hipfail(c, e, "dg_solver_create: hipMalloc");
+    }
+    s->dats[l.n] = p;
+  }
+  hipError_t e = hipMalloc(&s->dt1, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&s->t_dev, sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&s->flag, sizeof(unsigned int));
+  if (e == hipSuccess) e = hipMemset(s->dt1, 0, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(s->t_dev, 0, sizeof(double));
+  if (e == hipSuccess) e = hipMemset(s->flag, 0, sizeof(unsigned int));
+  if (e == hipSuccess && K > 0)
+    e = hipMemcpy(dat(s, "cell_xy"), M.cell_xy.data(), sizeof(double) * K * 6, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    dg_solver_destroy(s);
+    return hipfail(c, e, "dg_solver_create");
+  }
+  s->q = dat(s, "q"); s->w = dat(s, "workingQ"); s->ext = dat(s, "exteriorQ");
+  s->rk[0] = dat(s, "rk1"); s->rk[1] = dat(s, "rk2"); s->rk[2] = dat(s, "rk3");
+  dg::Launch L = launch_of(c);
+  e = dg::launch_init_grid(L, s->K, dat(s, "cell_xy"), dat(s, "nodeX"), dat(s, "nodeY"),
+                           dat(s, "x"), dat(s, "y"), dat(s, "xr"), dat(s, "yr"), dat(s, "xs"),
+                           dat(s, "ys"), dat(s, "rx"), dat(s, "ry"), dat(s, "sx"), dat(s, "sy"),
+                           dat(s, "nx"), dat(s, "ny"), dat(s, "fscale"), dat(s, "J"));
+  if (e == hipSuccess) e = dg::launch_set_ic(L, s->K, s->q, s->w);
+  if (e == hipSuccess) e = dg::launch_zero(L, K * NF * 4, s->ext);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    dg_solver_destroy(s);
+    return hipfail(c, e, "dg_solver_create: set-up kernels");
+  }
+  *out = s;
+  return DG_OK;
+}
+
+int dg_solver_set_vortex(dg_solver *s, double uinf, double beta, double x0, double y0,
+                         double period_x) {
+  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_set_vortex: null solver");
+  HIPCHECK(s->ctx, dg::launch_set_vortex(launch_of(s->ctx), s->K, dat(s, "x"), dat(s, "y"), uinf,
+                                         beta, x0, y0, period_x, s->q, s->w));
+  s->dt_valid = false;
+  return DG_OK;
+}
+
+int dg_solver_set_q(dg_solver *s, const double *host_q) {
+  if (!s || !host_q) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_set_q: null argument");
+  dg_ctx *c = s->ctx;
+  const size_t n = (size_t)s->K * c->Np * 4;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(s->q, host_q, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(s->w, host_q, sizeof(double) * n, hipMemcpyHostToDevice));
+  s->dt_valid = false;
+  return DG_OK;
+}
+
+int dg_solver_get_q(dg_solver *s, double *host_q) {
+  if (!s || !host_q) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_get_q: null argument");
+  dg_ctx *c = s->ctx;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(host_q, s->q, sizeof(double) * (size_t)s->K * c->Np * 4, hipMemcpyDeviceToHost));
+  return DG_OK;
+}
+
+int dg_solver_dat(dg_solver *s, const char *name, void **p) {
+  if (!s || !name || !p) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_dat: null argument");
+  auto it = s->dats.find(name);
+  if (it == s->dats.end())
+    return fail(s->ctx, DG_ERR_NAME, std::string("dg_solver_dat: unknown op_dat '") + name + "'");
+  *p = it->second;
+  return DG_OK;
+}
+
+int dg_solver_reset_time(dg_solver *s) {
+  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_reset_time: null solver");
+  HIPCHECK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  HIPCHECK(s->ctx, hipMemset(s->t_dev, 0, sizeof(double)));
+  HIPCHECK(s->ctx, hipMemset(s->flag, 0, sizeof(unsigned int)));
+  s->dt_valid = false;
+  s->step = 0;
+  return DG_OK;
+}
+
+int dg_solver_run(dg_solver *s, int nsteps, double cfl, int mode) {
+  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_run: null solver");
+  dg_ctx *c = s->ctx;
+  if (nsteps < 0 || !(cfl > 0.0)) return fail(c, DG_ERR_ARG, "dg_solver_run: need nsteps >= 0, cfl > 0");
+  if (mode != DG_MODE_FUSED && mode != DG_MODE_REFERENCE)
+    return fail(c, DG_ERR_ARG, "dg_solver_run: unknown mode");
+  const dg::Launch L = launch_of(c);
+  const int K = s->K;
+  const size_t nstate = (size_t)K * c->Np * 4;
+  double *fscale = dat(s, "fscale");
+  s->cfl_last = cfl;
+  unsigned long long *slot[2] = {s->dt1, s->dt1 + 1};
+  if (!s->dt_valid) {
+    const int p = (int)(s->step & 1);
+    HIPCHECK(c, hipMemsetAsync(slot[p], 0, sizeof(unsigned long long), c->stream));
+    HIPCHECK(c, timed(s, "calc_dt", [&] { return dg::launch_calc_dt(L, K, s->q, fscale, slot[p]); }));
+    s->dt_valid = true;
+  }
+  if (mode == DG_MODE_FUSED) {
+    dg::StageArgs a{};
+    a.rx = dat(s, "rx"); a.ry = dat(s, "ry"); a.sx = dat(s, "sx"); a.sy = dat(s, "sy");
+    a.fscale = fscale; a.nx = dat(s, "nx"); a.ny = dat(s, "ny");
+    a.fmap = c->d_fmap; a.q = s->q; a.cfl = cfl; a.t_dev = s->t_dev; a.flag = s->flag;
+    a.ncells = K;
+    double *wA = s->w, *wB = s->rk[0];
+    for (int n = 0; n < nsteps; n++) {
+      const int p = (int)(s->step & 1);
+      a.dt1_cur = slot[p];
+      a.dt1_next = slot[p ^ 1];
+      a.w = s->q; a.out = wA;
+      HIPCHECK(c, timed(s, "rhs_stage0", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE0); }));
+      a.w = wA; a.out = wB;
+      HIPCHECK(c, timed(s, "rhs_stage1", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE1); }));
+      a.w = wB; a.out = s->q;
+      HIPCHECK(c, timed(s, "rhs_stage2", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE2); }));
+      s->step++;
+    }
+    // keep the reference invariant workingQ == q after a step
+    if (nsteps > 0) HIPCHECK(c, dg::launch_copy(L, nstate, s->q, s->w));
+    return DG_OK;
+  }
+  // DG_MODE_REFERENCE: the nine loops of src/airfoil.cpp:252-334, dt on the host
+  double t = 0.0, dt1 = 0.0;
+  unsigned long long bits = 0;
+  HIPCHECK(c, hipMemcpyAsync(&t, s->t_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(&bits, slot[s->step & 1], sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  std::memcpy(&dt1, &bits, sizeof(dt1));
+  double dt = cfl / dt1;
+  for (int n = 0; n < nsteps; n++) {
+    for (int j = 0; j < 3; j++) {
+      HIPCHECK(c, timed(s, "get_neighbour_q", [&] { return dg::launch_get_neighbour_q(L, K, c->d_fmap, s->w, s->ext); }));
+      HIPCHECK(c, timed(s, "get_bedge_q", [&] {
+        return dg::launch_get_bedge_q(L, c->nbedges, c->d_b2c, c->d_bnum, c->d_btype, dat(s, "nx"),
+                                      dat(s, "ny"), s->w, s->ext);
+      }));
+      dg::StageArgs a{};
+      a.w = s->w; a.ext = s->ext; a.rx = dat(s, "rx"); a.ry = dat(s, "ry"); a.sx = dat(s, "sx");
+      a.sy = dat(s, "sy"); a.fscale = fscale; a.nx = dat(s, "nx"); a.ny = dat(s, "ny");
+      a.out = s->rk[j]; a.ncells = K;
+      HIPCHECK(c, timed(s, "euler_rhs", [&] { return dg::launch_rhs(L, a, false, dg::EPI_RHS); }));
+      if (j != 2)
+        HIPCHECK(c, timed(s, "set_workingQ", [&] {
+          return dg::launch_set_workingQ(L, K, dt, j, s->q, s->rk[0], s->rk[1], s->w);
+        }));
+    }
+    HIPCHECK(c, timed(s, "update_Q", [&] {
+      return dg::launch_update_Q(L, K, dt, s->q, s->rk[0], s->rk[1], s->rk[2], s->w);
+    }));
+    t += dt;
+    s->step++;
+    unsigned long long *nxt = slot[s->step & 1];
+    HIPCHECK(c, hipMemsetAsync(nxt, 0, sizeof(unsigned long long), c->stream));
+    HIPCHECK(c, timed(s, "calc_dt", [&] { return dg::launch_calc_dt(L, K, s->q, fscale, nxt); }));
+    HIPCHECK(c, hipMemcpyAsync(&bits, nxt, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(&dt1, &bits, sizeof(dt1));
+    dt = cfl / dt1;
+  }
+  HIPCHECK(c, hipMemcpyAsync(s->t_dev, &t, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return DG_OK;
+}
+
+int dg_solver_time(dg_solver *s, double *t, double *dt) {
+  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_time: null solver");
+  dg_ctx *c = s->ctx;
+  double tv = 0.0;
+  unsigned long long bits = 0;
+  HIPCHECK(c, hipMemcpyAsync(&tv, s->t_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(&bits, s->dt1 + (s->step & 1), sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  double dt1;
+  std::memcpy(&dt1, &bits, sizeof(dt1));
+  if (t) *t = tv;
+  if (dt) *dt = s->dt_valid ? s->cfl_last / dt1 : 0.0;
+  return DG_OK;
+}
+
+int dg_solver_check(dg_solver *s) {
+  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_check: null solver");
+  unsigned int f = 0;
+  HIPCHECK(s->ctx, hipMemcpyAsync(&f, s->flag, sizeof(f), hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHECK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  if (f) return fail(s->ctx, DG_ERR_NONFINITE, "non-finite or non-positive-density state detected");
+  return DG_OK;
+}
+
+int dg_solver_profile(dg_solver *s, int enable) {
+  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_profile: null solver");
+  drain_profile(s);
+  s->profiling = enable != 0;
+  s->ktimes.clear();
+  return DG_OK;
+}
+
+int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int *count) {
+  if (!s || !kernel) return fail(nullptr, DG_ERR_ARG, "dg_solver_kernel_time: null argument");
+  drain_profile(s);
+  auto it = s->ktimes.find(kernel);
+  if (it == s->ktimes.end() || it->second.second == 0) {
+    if (avg_ms) *avg_ms = 0.0;
+    if (count) *count = 0;
+    return DG_OK;
+  }
+  if (avg_ms) *avg_ms = it->second.first / it->second.second;
+  if (count) *count = it->second.second;
+  return DG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+// Internal declarations shared by the C-ABI layer and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dg {
+
+// Physics constants (op_decl_const scalars of src/airfoil.cpp:183-190).
+struct Phys {
+  double gam, bc_r, bc_u, bc_v, bc_e, bc_p;
+};
+
+// Layout of the per-context operator buffer on the device (doubles):
+//   Drw[Np*Np] | Dsw[Np*Np] | LIFT[Np*NF] | Dr[Np*Np] | Ds[Np*Np] | r[Np] | s[Np]
+inline int ops_size(int Np, int NF) { return 4 * Np * Np + Np * NF + 2 * Np; }
+
+// Everything a launch needs from the context.
+struct Launch {
+  int N, Np, Nfp, NF;
+  hipStream_t stream;
+  const double *ops;   // device operator buffer
+  const int *fmask;    // device FMASK [NF]
+  Phys ph;
+  uint64_t facenodes;  // bit j set when node j lies on a face
+  int max_blocks;      // persistent grid cap (CUs x resident blocks)
+};
+
+// Arguments of the stage kernels.
+struct StageArgs {
+  const double *w;        // stage input state [K][Np][4]
+  double *ext;            // exteriorQ (drop-in euler_rhs only)
+  const double *rx, *ry, *sx, *sy;   // [K][Np]
+  const double *fscale, *nx, *ny;    // [K][NF]
+  const int *fmap;        // [K][3] face map
+  double *out;            // qRHS, or the next state
+  const double *q;        // SSPRK3 base state (stages 1, 2)
+  const unsigned long long *dt1_cur;  // device dt1 (bits of a non-negative double)
+  unsigned long long *dt1_next;       // reduction target (stage 2), zeroed in stage 0
+  double cfl;
+  double *t_dev;          // simulated time, += dt in stage 2
+  unsigned int *flag;     // bit 0: non-finite value seen
+  int ncells;
+};
+
+// Epilogue kinds of the stage kernel.
+enum Epi { EPI_RHS = 0, EPI_STAGE0 = 1, EPI_STAGE1 = 2, EPI_STAGE2 = 3 };
+
+hipError_t launch_rhs(const Launch &L, const StageArgs &a, bool gather, int epi);
+hipError_t launch_get_neighbour_q(const Launch &L, int ncells, const int *fmap,
+                                  const double *w, double *ext);
+hipError_t launch_get_bedge_q(const Launch &L, int nbedges, const int *b2c,
+                              const int *bnum, const int *btype, const double *nx,
+                              const double *ny, const double *w, double *ext);
+hipError_t launch_set_workingQ(const Launch &L, int ncells, double dt, int stage,
+                               const double *q, const double *k1, const double *k2,
+                               double *w);
+hipError_t launch_update_Q(const Launch &L, int ncells, double dt, double *q,
+                           const double *k1, const double *k2, const double *k3,
+                           double *w);
+hipError_t launch_calc_dt(const Launch &L, int ncells, const double *q,
+                          const double *fscale, unsigned long long *d_out);
+hipError_t launch_init_grid(const Launch &L, int ncells, const double *cell_xy,
+                            double *nodeX, double *nodeY, double *x, double *y,
+                            double *xr, double *yr, double *xs, double *ys,
+                            double *rx, double *ry, double *sx, double *sy,
+                            double *nx, double *ny, double *fscale, double *J);
+hipError_t launch_set_ic(const Launch &L, int ncells, double *q, double *w);
+hipError_t launch_set_vortex(const Launch &L, int ncells, const double *x,
+                             const double *y, double uinf, double beta, double x0,
+                             double y0, double period_x, double *q, double *w);
+hipError_t launch_zero(const Launch &L, size_t n, double *p);
+hipError_t launch_copy(const Launch &L, size_t n, const double *src, double *dst);
+
+}  // namespace dg

@@ -1,0 +1,709 @@
+// HIP kernels for gfx950 (MI355X): nodal DG Euler right-hand side, SSPRK3
+// stage updates, CFL reduction and set-up loops.  FP64 throughout.
+//
+// Work decomposition of the stage kernel (rhs_kernel):
+//   * a workgroup owns a tile of TE consecutive cells; the tile's state
+//     records ([TE][Np][4] doubles, contiguous in the op_dat layout) are
+//     staged in LDS with coalesced 16-byte loads;
+//   * face phase: one lane per (cell, face node) gathers the neighbour trace
+//     straight from the neighbour's state record through the face map
+//     (gather-only, owner computes: no atomics, no colouring), or builds the
+//     boundary state, and evaluates the Roe flux into LDS;
+//   * volume phase: one lane per (cell, node) evaluates the Euler fluxes F, G
+//     into LDS;
+//   * combine phase: one lane per (cell, node) applies the weak derivative
+//     rows of Drw, Dsw, the metric terms and the LIFT row, then runs the
+//     epilogue (write qRHS, or an SSPRK3 stage update, or the final update
+//     fused with the CFL max-reduction).
+// The grid is persistent (a few workgroups per CU) and walks tiles in
+// XCD-contiguous chunks so that a cell and its neighbours are usually
+// processed by workgroups of the same XCD, whose L2 then holds the shared
+// face traces.
+#include <hip/hip_runtime.h>
+
+#include "dg_internal.h"
+
+namespace dg {
+
+template <int N>
+struct Ord {
+  static constexpr int Np = (N + 1) * (N + 2) / 2;
+  static constexpr int Nfp = N + 1;
+  static constexpr int NF = 3 * (N + 1);
+};
+
+// ------------------------------------------------------------------------
+// Point-wise physics (src/euler_rhs.h:7-23, 120-220; src/get_bedge_q.h)
+// ------------------------------------------------------------------------
+
+__device__ __forceinline__ void euler_flux(double gam, const double q[4], double F[4],
+                                           double G[4]) {
+  double rho = q[0];
+  double u = q[1] / rho;
+  double v = q[2] / rho;
+  double p = (gam - 1) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
+  F[0] = q[1];
+  F[1] = q[1] * u + p;
+  F[2] = q[2] * u;
+  F[3] = u * (q[3] + p);
+  G[0] = q[2];
+  G[1] = q[1] * v;
+  G[2] = q[2] * v + p;
+  G[3] = v * (q[3] + p);
+}
+
+// Roe flux at one face node, rotated onto the normal, rotated back, times
+// fscale (src/euler_rhs.h:151-219).
+__device__ __forceinline__ void roe_node(double gam, const double mQ[4], const double eQ[4],
+                                         double nx, double ny, double fs, double out[4]) {
+  double mR0 = mQ[0], mR1 = nx * mQ[1] + ny * mQ[2], mR2 = -ny * mQ[1] + nx * mQ[2], mR3 = mQ[3];
+  double pR0 = eQ[0], pR1 = nx * eQ[1] + ny * eQ[2], pR2 = -ny * eQ[1] + nx * eQ[2], pR3 = eQ[3];
+  double mU = mR1 / mR0, mV = mR2 / mR0;
+  double mP = (gam - 1) * (mR3 - 0.5 * (mR1 * mU + mR2 * mV));
+  double pU = pR1 / pR0, pV = pR2 / pR0;
+  double pP = (gam - 1) * (pR3 - 0.5 * (pR1 * pU + pR2 * pV));
+  double fm0 = mR1, fm1 = mR1 * mU + mP, fm2 = mR2 * mU, fm3 = mU * (mR3 + mP);
+  double fp0 = pR1, fp1 = pR1 * pU + pP, fp2 = pR2 * pU, fp3 = pU * (pR3 + pP);
+  double mH = (mR3 + mP) / mR0;
+  double pH = (pR3 + pP) / pR0;
+  double ms = sqrt(mR0), ps = sqrt(pR0);
+  double rho = ms * ps;
+  double den = ms + ps;
+  double u = (ms * mU + ps * pU) / den;
+  double v = (ms * mV + ps * pV) / den;
+  double H = (ms * mH + ps * pH) / den;
+  double c2 = (gam - 1) * (H - 0.5 * (u * u + v * v));
+  double c = sqrt(c2);
+  double dW1 = -0.5 * rho * (pU - mU) / c + 0.5 * (pP - mP) / c2;
+  double dW2 = (pR0 - mR0) - (pP - mP) / c2;
+  double dW3 = rho * (pV - mV);
+  double dW4 = 0.5 * rho * (pU - mU) / c + 0.5 * (pP - mP) / c2;
+  dW1 = fabs(u - c) * dW1;
+  dW2 = fabs(u) * dW2;
+  dW3 = fabs(u) * dW3;
+  dW4 = fabs(u + c) * dW4;
+  double f0 = (fm0 + fp0) / 2.0 - (dW1 + dW2 + dW4) / 2.0;
+  double f1 = (fm1 + fp1) / 2.0 - (dW1 * (u - c) + dW2 * u + dW4 * (u + c)) / 2.0;
+  double f2 = (fm2 + fp2) / 2.0 - (dW1 * v + dW2 * v + dW3 + dW4 * v) / 2.0;
+  double f3 = (fm3 + fp3) / 2.0 -
+              (dW1 * (H - u * c) + dW2 * (u * u + v * v) + dW3 * v + dW4 * (H + u * c)) / 2.0;
+  out[0] = f0 * fs;
+  out[1] = (nx * f1 - ny * f2) * fs;
+  out[2] = (ny * f1 + nx * f2) * fs;
+  out[3] = f3 * fs;
+}
+
+// Characteristic (Riemann-invariant) far-field state; no reference
+// counterpart.  Subsonic: R+ from inside, R- from the free stream, entropy
+// and tangential velocity from the upwind side.  Supersonic: free stream on
+// inflow, interior on outflow.
+__device__ __forceinline__ void farfield_state(const Phys &ph, const double qi[4], double nx,
+                                               double ny, double qb[4]) {
+  const double g = ph.gam;
+  double rhoi = qi[0], ui = qi[1] / rhoi, vi = qi[2] / rhoi;
+  double pi_ = (g - 1.0) * (qi[3] - 0.5 * rhoi * (ui * ui + vi * vi));
+  double ci = sqrt(fabs(g * pi_ / rhoi));
+  double vni = ui * nx + vi * ny;
+  double cinf = sqrt(g * ph.bc_p / ph.bc_r);
+  double vninf = ph.bc_u * nx + ph.bc_v * ny;
+  if (fabs(vni) >= ci) {
+    if (vni < 0.0) {
+      qb[0] = ph.bc_r; qb[1] = ph.bc_r * ph.bc_u; qb[2] = ph.bc_r * ph.bc_v; qb[3] = ph.bc_e;
+    } else {
+      qb[0] = qi[0]; qb[1] = qi[1]; qb[2] = qi[2]; qb[3] = qi[3];
+    }
+    return;
+  }
+  double rp = vni + 2.0 * ci / (g - 1.0);
+  double rm = vninf - 2.0 * cinf / (g - 1.0);
+  double vnb = 0.5 * (rp + rm);
+  double cb = 0.25 * (g - 1.0) * (rp - rm);
+  double ent, ut, vt;
+  if (vnb > 0.0) {
+    ent = pi_ / pow(rhoi, g);
+    ut = ui - vni * nx; vt = vi - vni * ny;
+  } else {
+    ent = ph.bc_p / pow(ph.bc_r, g);
+    ut = ph.bc_u - vninf * nx; vt = ph.bc_v - vninf * ny;
+  }
+  double rhob = pow(cb * cb / (g * ent), 1.0 / (g - 1.0));
+  double pb = rhob * cb * cb / g;
+  double ub = ut + vnb * nx, vb = vt + vnb * ny;
+  qb[0] = rhob;
+  qb[1] = rhob * ub;
+  qb[2] = rhob * vb;
+  qb[3] = pb / (g - 1.0) + 0.5 * rhob * (ub * ub + vb * vb);
+}
+
+// Exterior state on a boundary face node (src/get_bedge_q.h:24-50 + type 3).
+__device__ __forceinline__ void boundary_state(const Phys &ph, int type, const double mQ[4],
+                                               double nx, double ny, double qb[4]) {
+  if (type == 0) {
+    qb[0] = ph.bc_r; qb[1] = ph.bc_r * ph.bc_u; qb[2] = ph.bc_r * ph.bc_v; qb[3] = ph.bc_e;
+  } else if (type == 1) {
+    qb[0] = ph.bc_r; qb[1] = ph.bc_r * ph.bc_u; qb[2] = ph.bc_r * ph.bc_v; qb[3] = mQ[3];
+  } else if (type == 2) {
+    double vn = nx * mQ[1] + ny * mQ[2];
+    qb[0] = mQ[0];
+    qb[1] = mQ[1] - 2 * vn * nx;
+    qb[2] = mQ[2] - 2 * vn * ny;
+    qb[3] = mQ[3];
+  } else {
+    farfield_state(ph, mQ, nx, ny, qb);
+  }
+}
+
+// CFL quantity of one node (src/calc_dt.h:7-12 with (N+1)^2).
+__device__ __forceinline__ double cfl_node(double gam, double n1sq, const double q[4], double fs) {
+  double rho = q[0];
+  double u = q[1] / rho;
+  double v = q[2] / rho;
+  double p = (gam - 1.0) * (q[3] - rho * (u * u + v * v) * 0.5);
+  double c = sqrt(fabs(gam * p / rho));
+  return n1sq * 0.5 * fs * (sqrt(u * u + v * v) + c);
+}
+
+__device__ __forceinline__ unsigned long long max_bits(unsigned long long a, unsigned long long b) {
+  return a > b ? a : b;
+}
+
+// Max over a workgroup of non-negative doubles held as bit patterns (a NaN
+// pattern compares above +inf, so it propagates).
+template <int NT>
+__device__ unsigned long long block_max_bits(unsigned long long v, unsigned long long *red) {
+  for (int off = 32; off > 0; off >>= 1) v = max_bits(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  unsigned long long r = 0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < NT / 64; i++) r = max_bits(r, red[i]);
+  }
+  __syncthreads();
+  return r;
+}
+
+// Maps a persistent workgroup's k-th visit to a tile so that the workgroups
+// of one XCD (blockIdx % 8 on MI355X) walk one contiguous chunk of tiles.
+__device__ __forceinline__ int xcd_tile(int k, int ntiles) {
+  const int nxcd = 8;
+  const int g = gridDim.x;
+  if (g % nxcd != 0) return blockIdx.x + k * g;
+  const int xcd = blockIdx.x % nxcd, local = blockIdx.x / nxcd, per = g / nxcd;
+  const int chunk = (ntiles + nxcd - 1) / nxcd;
+  const int t = local + k * per;
+  if (t >= chunk) return ntiles;  // this XCD's chunk is done
+  return xcd * chunk + t;
+}
+
+// ------------------------------------------------------------------------
+// Stage kernel
+// ------------------------------------------------------------------------
+
+template <int N, int TE, int NT, bool GATHER, int EPI>
+__global__ __launch_bounds__(NT) void rhs_kernel(StageArgs a, const double *__restrict__ ops,
+                                                 const int *__restrict__ fmask, Phys ph,
+                                                 unsigned long long facenodes) {
+  constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
+  constexpr int SW = Np * 4;          // LDS stride of a state record
+  constexpr int SFG = Np * 8 + 2;     // padded so cells fall in different banks
+  constexpr int SFL = NF * 4 + 2;
+  constexpr int NOPS = 2 * Np * Np + Np * NF;
+  __shared__ double sW[TE * SW];
+  __shared__ double sFG[TE * SFG];
+  __shared__ double sFl[TE * SFL];
+  __shared__ double sOp[NOPS];
+  __shared__ int sFm[NF];
+  __shared__ unsigned long long sRed[NT / 64];
+
+  const int tid = threadIdx.x;
+  for (int i = tid; i < NOPS; i += NT) sOp[i] = ops[i];
+  if (tid < NF) sFm[tid] = fmask[tid];
+  const double *sDrw = sOp, *sDsw = sOp + Np * Np, *sLIFT = sOp + 2 * Np * Np;
+
+  double dt = 0.0;
+  if (EPI != EPI_RHS) dt = a.cfl / __longlong_as_double((long long)*a.dt1_cur);
+  if (EPI == EPI_STAGE0 && blockIdx.x == 0 && tid == 0) *a.dt1_next = 0ull;
+  unsigned long long cfl_bits = 0ull;
+  bool bad = false;
+  const double n1sq = (double)((N + 1) * (N + 1));
+
+  const int ntiles = (a.ncells + TE - 1) / TE;
+  for (int k = 0;; k++) {
+    const int tile = xcd_tile(k, ntiles);
+    if (tile >= ntiles) break;
+    const int e0 = tile * TE;
+    const int ne = min(TE, a.ncells - e0);
+    __syncthreads();  // LDS reuse across tiles (and sOp/sFm on the first pass)
+    {
+      const double2 *src = reinterpret_cast<const double2 *>(a.w + (size_t)e0 * SW);
+      double2 *dst = reinterpret_cast<double2 *>(sW);
+      for (int i = tid; i < ne * SW / 2; i += NT) dst[i] = src[i];
+    }
+    __syncthreads();
+
+    // face phase: Roe flux per (cell, face node)
+    for (int it = tid; it < ne * NF; it += NT) {
+      const int e = it / NF, m = it - e * NF;
+      const int f = m / Nfp, i = m - f * Nfp;
+      const size_t gm = (size_t)(e0 + e) * NF + m;
+      const double nxv = a.nx[gm], nyv = a.ny[gm], fs = a.fscale[gm];
+      double mQ[4], pQ[4];
+      const double *ms = &sW[e * SW + sFm[m] * 4];
+      mQ[0] = ms[0]; mQ[1] = ms[1]; mQ[2] = ms[2]; mQ[3] = ms[3];
+      if (GATHER) {
+        const int code = a.fmap[(size_t)(e0 + e) * 3 + f];
+        if (code >= 0) {
+          const int nb = code >> 2, nf = code & 3;
+          const int node = sFm[nf * Nfp + (Nfp - 1 - i)];
+          const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)nb * Np + node) * 4);
+          double2 p01 = src[0], p23 = src[1];
+          pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
+        } else {
+          boundary_state(ph, -code - 1, mQ, nxv, nyv, pQ);
+        }
+      } else {
+        const double2 *src = reinterpret_cast<const double2 *>(a.ext + gm * 4);
+        double2 p01 = src[0], p23 = src[1];
+        pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
+      }
+      double fl[4];
+      roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
+      double *d = &sFl[e * SFL + m * 4];
+      d[0] = fl[0]; d[1] = fl[1]; d[2] = fl[2]; d[3] = fl[3];
+    }
+    // volume phase: F, G per (cell, node)
+    for (int it = tid; it < ne * Np; it += NT) {
+      const int e = it / Np, i = it - e * Np;
+      double qv[4], F[4], G[4];
+      const double *s = &sW[e * SW + i * 4];
+      qv[0] = s[0]; qv[1] = s[1]; qv[2] = s[2]; qv[3] = s[3];
+      euler_flux(ph.gam, qv, F, G);
+      double *d = &sFG[e * SFG + i * 8];
+      d[0] = F[0]; d[1] = F[1]; d[2] = F[2]; d[3] = F[3];
+      d[4] = G[0]; d[5] = G[1]; d[6] = G[2]; d[7] = G[3];
+    }
+    __syncthreads();
+    if (!GATHER) {  // drop-in euler_rhs zeroes exteriorQ on exit (OP_RW)
+      double2 *z = reinterpret_cast<double2 *>(a.ext + (size_t)e0 * NF * 4);
+      for (int i = tid; i < ne * NF * 2; i += NT) z[i] = make_double2(0.0, 0.0);
+    }
+
+    // combine phase
+    for (int it = tid; it < ne * Np; it += NT) {
+      const int e = it / Np, j = it - e * Np;
+      const size_t gn = (size_t)(e0 + e) * Np + j;
+      double dFdr[4] = {0, 0, 0, 0}, dFds[4] = {0, 0, 0, 0};
+      double dGdr[4] = {0, 0, 0, 0}, dGds[4] = {0, 0, 0, 0};
+      const double *fg = &sFG[e * SFG];
+#pragma unroll
+      for (int i = 0; i < Np; i++) {
+        const double ar = sDrw[j * Np + i], as = sDsw[j * Np + i];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const double Fv = fg[i * 8 + v], Gv = fg[i * 8 + 4 + v];
+          dFdr[v] += ar * Fv;
+          dFds[v] += as * Fv;
+          dGdr[v] += ar * Gv;
+          dGds[v] += as * Gv;
+        }
+      }
+      const double rxv = a.rx[gn], sxv = a.sx[gn], ryv = a.ry[gn], syv = a.sy[gn];
+      double res[4], lift[4] = {0, 0, 0, 0};
+      const double *fl = &sFl[e * SFL];
+#pragma unroll
+      for (int m = 0; m < NF; m++) {
+        const double l = sLIFT[j * NF + m];
+#pragma unroll
+        for (int v = 0; v < 4; v++) lift[v] += l * fl[m * 4 + v];
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        res[v] = ((rxv * dFdr[v] + sxv * dFds[v]) + (ryv * dGdr[v] + syv * dGds[v])) - lift[v];
+
+      double o[4];
+      const double *wv = &sW[e * SW + j * 4];
+      if (EPI == EPI_RHS) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) o[v] = res[v];
+      } else if (EPI == EPI_STAGE0) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) o[v] = wv[v] + dt * res[v];
+      } else {
+        const double2 *qs = reinterpret_cast<const double2 *>(a.q + gn * 4);
+        double2 q01 = qs[0], q23 = qs[1];
+        double qv[4] = {q01.x, q01.y, q23.x, q23.y};
+        if (EPI == EPI_STAGE1) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) o[v] = 0.75 * qv[v] + 0.25 * (wv[v] + dt * res[v]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; v++) o[v] = (qv[v] + 2.0 * (wv[v] + dt * res[v])) / 3.0;
+        }
+      }
+      double2 *dst = reinterpret_cast<double2 *>(a.out + gn * 4);
+      dst[0] = make_double2(o[0], o[1]);
+      dst[1] = make_double2(o[2], o[3]);
+      if (EPI == EPI_STAGE2) {
+        if ((facenodes >> j) & 1ull) {
+          double c = cfl_node(ph.gam, n1sq, o, a.fscale[(size_t)(e0 + e) * NF + j]) + 0.0;
+          unsigned long long b = (unsigned long long)__double_as_longlong(c);
+          cfl_bits = max_bits(cfl_bits, b);
+        }
+        bad |= !(isfinite(o[0]) && isfinite(o[1]) && isfinite(o[2]) && isfinite(o[3])) ||
+               !(o[0] > 0.0);
+      }
+    }
+  }
+  if (EPI == EPI_STAGE2) {
+    unsigned long long r = block_max_bits<NT>(cfl_bits, sRed);
+    if (tid == 0) atomicMax(a.dt1_next, r);
+    if (bad) atomicOr(a.flag, 1u);
+    if (blockIdx.x == 0 && tid == 0) *a.t_dev += dt;
+  }
+}
+
+template <int N>
+struct TileCfg {
+  static constexpr int TE = 32;
+  static constexpr int NT0 = Ord<N>::NF * TE;
+  static constexpr int NT = NT0 <= 256 ? 256 : (NT0 <= 384 ? 384 : (NT0 <= 512 ? 512 : 1024));
+};
+
+template <int N, bool GATHER, int EPI>
+static hipError_t launch_rhs_t(const Launch &L, const StageArgs &a) {
+  constexpr int TE = TileCfg<N>::TE, NT = TileCfg<N>::NT;
+  const int ntiles = (a.ncells + TE - 1) / TE;
+  int grid = L.max_blocks > 0 ? L.max_blocks : 2048;
+  if (grid > ntiles) grid = ntiles;
+  if (grid >= 8) grid -= grid % 8;
+  if (grid < 1) return hipSuccess;
+  hipLaunchKernelGGL((rhs_kernel<N, TE, NT, GATHER, EPI>), dim3(grid), dim3(NT), 0, L.stream, a,
+                     L.ops, L.fmask, L.ph, (unsigned long long)L.facenodes);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_rhs_n(const Launch &L, const StageArgs &a, bool gather, int epi) {
+  if (!gather) return launch_rhs_t<N, false, EPI_RHS>(L, a);
+  switch (epi) {
+    case EPI_RHS: return launch_rhs_t<N, true, EPI_RHS>(L, a);
+    case EPI_STAGE0: return launch_rhs_t<N, true, EPI_STAGE0>(L, a);
+    case EPI_STAGE1: return launch_rhs_t<N, true, EPI_STAGE1>(L, a);
+    case EPI_STAGE2: return launch_rhs_t<N, true, EPI_STAGE2>(L, a);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_rhs(const Launch &L, const StageArgs &a, bool gather, int epi) {
+  if (!gather && epi != EPI_RHS) return hipErrorInvalidValue;
+  switch (L.N) {
+    case 1: return launch_rhs_n<1>(L, a, gather, epi);
+    case 2: return launch_rhs_n<2>(L, a, gather, epi);
+    case 3: return launch_rhs_n<3>(L, a, gather, epi);
+    case 4: return launch_rhs_n<4>(L, a, gather, epi);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------------
+// Face-trace loops of the drop-in API
+// ------------------------------------------------------------------------
+
+// src/get_neighbour_q.h: exteriorQ[c][f][i] += workingQ[nb][FMASK[nf][Nfp-1-i]].
+// One lane per (cell, face node), gather-only.
+__global__ void get_neighbour_q_kernel(int ncells, int Np, int Nfp, const int *__restrict__ fmap,
+                                       const int *__restrict__ fmask, const double *__restrict__ w,
+                                       double *__restrict__ ext) {
+  const int NF = 3 * Nfp;
+  const size_t n = (size_t)ncells * NF;
+  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
+       it += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = it / NF;
+    const int m = (int)(it - c * NF);
+    const int f = m / Nfp, i = m - f * Nfp;
+    const int code = fmap[c * 3 + f];
+    if (code < 0) continue;
+    const int nb = code >> 2, nf = code & 3;
+    const int node = fmask[nf * Nfp + (Nfp - 1 - i)];
+    const double2 *src = reinterpret_cast<const double2 *>(w + ((size_t)nb * Np + node) * 4);
+    double2 *dst = reinterpret_cast<double2 *>(ext + it * 4);
+    double2 a0 = src[0], a1 = src[1], d0 = dst[0], d1 = dst[1];
+    dst[0] = make_double2(d0.x + a0.x, d0.y + a0.y);
+    dst[1] = make_double2(d1.x + a1.x, d1.y + a1.y);
+  }
+}
+
+// src/get_bedge_q.h: one lane per (boundary edge, face node).
+__global__ void get_bedge_q_kernel(int nbedges, int Np, int Nfp, const int *__restrict__ b2c,
+                                   const int *__restrict__ bnum, const int *__restrict__ btype,
+                                   const int *__restrict__ fmask, Phys ph,
+                                   const double *__restrict__ nx, const double *__restrict__ ny,
+                                   const double *__restrict__ w, double *__restrict__ ext) {
+  const int NF = 3 * Nfp;
+  const size_t n = (size_t)nbedges * Nfp;
+  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
+       it += (size_t)gridDim.x * blockDim.x) {
+    const size_t b = it / Nfp;
+    const int i = (int)(it - b * Nfp);
+    const size_t c = b2c[b];
+    const int f = bnum[b];
+    const int m = f * Nfp + i;
+    const double *mq = w + ((size_t)c * Np + fmask[m]) * 4;
+    double mQ[4] = {mq[0], mq[1], mq[2], mq[3]}, qb[4];
+    boundary_state(ph, btype[b], mQ, nx[c * NF + m], ny[c * NF + m], qb);
+    double *d = ext + (c * NF + m) * 4;
+    d[0] += qb[0]; d[1] += qb[1]; d[2] += qb[2]; d[3] += qb[3];
+  }
+}
+
+// src/set_workingQ.h
+__global__ void set_workingQ_kernel(size_t n, double dt, int stage, const double *__restrict__ q,
+                                    const double *__restrict__ k1, const double *__restrict__ k2,
+                                    double *__restrict__ w) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    if (stage == 0)
+      w[i] = q[i] + dt * k1[i];
+    else
+      w[i] = q[i] + dt * (k1[i] / 4.0 + k2[i] / 4.0);
+  }
+}
+
+// src/update_Q.h
+__global__ void update_Q_kernel(size_t n, double dt, double *__restrict__ q,
+                                const double *__restrict__ k1, const double *__restrict__ k2,
+                                const double *__restrict__ k3, double *__restrict__ w) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double v = q[i] + dt * (k1[i] / 6.0 + k2[i] / 6.0 + 2.0 * k3[i] / 3.0);
+    q[i] = v;
+    w[i] = v;
+  }
+}
+
+// src/calc_dt.h: per-cell max over face slots, then a device max.
+__global__ __launch_bounds__(256) void calc_dt_kernel(int ncells, int N, int Np, int Nfp,
+                                                      const int *__restrict__ fmask, double gam,
+                                                      const double *__restrict__ q,
+                                                      const double *__restrict__ fscale,
+                                                      unsigned long long *out) {
+  __shared__ unsigned long long red[4];
+  const int NF = 3 * Nfp;
+  const double n1sq = (double)((N + 1) * (N + 1));
+  unsigned long long best = 0ull;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < (size_t)ncells;
+       c += (size_t)gridDim.x * blockDim.x) {
+    for (int i = 0; i < NF; i++) {
+      const int f = fmask[i];
+      const double *qq = q + (c * Np + f) * 4;
+      double qv[4] = {qq[0], qq[1], qq[2], qq[3]};
+      double v = cfl_node(gam, n1sq, qv, fscale[c * NF + f]) + 0.0;
+      best = max_bits(best, (unsigned long long)__double_as_longlong(v));
+    }
+  }
+  unsigned long long r = block_max_bits<256>(best, red);
+  if (threadIdx.x == 0) atomicMax(out, r);
+}
+
+// src/init_grid.h: one lane per cell.
+__global__ void init_grid_kernel(int ncells, int Np, int Nfp, const double *__restrict__ ops,
+                                 const int *__restrict__ fmask, const double *__restrict__ cell_xy,
+                                 double *nodeX, double *nodeY, double *x, double *y, double *xr,
+                                 double *yr, double *xs, double *ys, double *rx, double *ry,
+                                 double *sx, double *sy, double *nx, double *ny, double *fscale,
+                                 double *Jc) {
+  const int NF = 3 * Nfp;
+  const double *Dr = ops + 2 * Np * Np + Np * NF;
+  const double *Ds = Dr + Np * Np;
+  const double *r = Ds + Np * Np;
+  const double *s = r + Np;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < (size_t)ncells;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const double *v = cell_xy + 6 * c;
+    nodeX[3 * c] = v[0]; nodeX[3 * c + 1] = v[2]; nodeX[3 * c + 2] = v[4];
+    nodeY[3 * c] = v[1]; nodeY[3 * c + 1] = v[3]; nodeY[3 * c + 2] = v[5];
+    double *X = x + c * Np, *Y = y + c * Np;
+    for (int d = 0; d < 2; d++) {
+      double *o = d == 0 ? X : Y;
+      const double a1 = 0.5 * v[2 + d], a2 = 0.5 * v[4 + d], a0 = -0.5 * v[d];
+      for (int i = 0; i < Np; i++) {
+        double t = a1 * (1.0 + r[i]);
+        t = t + a2 * (1.0 + s[i]);
+        o[i] = t + a0 * (s[i] + r[i]);
+      }
+    }
+    double *XR = xr + c * Np, *XS = xs + c * Np, *YR = yr + c * Np, *YS = ys + c * Np;
+    for (int j = 0; j < Np; j++) {
+      double a = 0, b = 0, cc = 0, dd = 0;
+      for (int i = 0; i < Np; i++) {
+        a += Dr[j * Np + i] * X[i];
+        b += Ds[j * Np + i] * X[i];
+        cc += Dr[j * Np + i] * Y[i];
+        dd += Ds[j * Np + i] * Y[i];
+      }
+      XR[j] = a; XS[j] = b; YR[j] = cc; YS[j] = dd;
+    }
+    double J0 = 0.0;
+    for (int i = 0; i < Np; i++) {
+      double J = -XS[i] * YR[i] + XR[i] * YS[i];
+      if (i == 0) J0 = J;
+      rx[c * Np + i] = YS[i] / J;
+      sx[c * Np + i] = -YR[i] / J;
+      ry[c * Np + i] = -XS[i] / J;
+      sy[c * Np + i] = XR[i] / J;
+    }
+    if (Jc) Jc[c] = J0;
+    for (int m = 0; m < NF; m++) {
+      const int f = fmask[m];
+      const int face = m / Nfp;
+      double nxv, nyv;
+      if (face == 0) { nxv = YR[f]; nyv = -XR[f]; }
+      else if (face == 1) { nxv = YS[f] - YR[f]; nyv = XR[f] - XS[f]; }
+      else { nxv = -YS[f]; nyv = XS[f]; }
+      double sJ = sqrt(nxv * nxv + nyv * nyv);
+      double J = -XS[f] * YR[f] + XR[f] * YS[f];
+      nx[c * NF + m] = nxv / sJ;
+      ny[c * NF + m] = nyv / sJ;
+      fscale[c * NF + m] = sJ / J;
+    }
+  }
+}
+
+// src/set_ic.h
+__global__ void set_ic_kernel(size_t nnodes, Phys ph, double *q, double *w) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nnodes;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double v[4] = {ph.bc_r, ph.bc_r * ph.bc_u, ph.bc_r * ph.bc_v, ph.bc_e};
+    for (int k = 0; k < 4; k++) { q[4 * i + k] = v[k]; w[4 * i + k] = v[k]; }
+  }
+}
+
+// Isentropic vortex (vortex/src/set_ic.h:1-28, with free-stream speed uinf).
+__global__ void set_vortex_kernel(size_t nnodes, double gam, const double *__restrict__ x,
+                                  const double *__restrict__ y, double uinf, double beta,
+                                  double x0, double y0, double period_x, double *q, double *w) {
+  const double pi = 3.14159265358979323846;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nnodes;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double dx = x[i] - x0;
+    if (period_x > 0.0) dx -= period_x * floor(dx / period_x + 0.5);
+    double dy = y[i] - y0;
+    double r2 = dx * dx + dy * dy;
+    double e1 = exp(1.0 - r2);
+    double u = uinf - beta * e1 * (dy / (2 * pi));
+    double v = beta * e1 * (dx / (2 * pi));
+    double rho = 1.0 - ((gam - 1) / (16 * gam * pi * pi)) * beta * beta * exp(2 * (1.0 - r2));
+    rho = pow(rho, 1.0 / (gam - 1));
+    double p = pow(rho, gam);
+    double qv[4] = {rho, rho * u, rho * v, (p / (gam - 1)) + 0.5 * rho * (u * u + v * v)};
+    for (int k = 0; k < 4; k++) { q[4 * i + k] = qv[k]; w[4 * i + k] = qv[k]; }
+  }
+}
+
+__global__ void zero_kernel(size_t n, double *p) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+__global__ void copy_kernel(size_t n, const double *__restrict__ s, double *__restrict__ d) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+static int grid_for(size_t n, int bs, int cap) {
+  size_t g = (n + bs - 1) / bs;
+  if (g > (size_t)cap) g = cap;
+  return g < 1 ? 1 : (int)g;
+}
+
+hipError_t launch_get_neighbour_q(const Launch &L, int ncells, const int *fmap, const double *w,
+                                  double *ext) {
+  size_t n = (size_t)ncells * L.NF;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(get_neighbour_q_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, L.stream,
+                     ncells, L.Np, L.Nfp, fmap, L.fmask, w, ext);
+  return hipGetLastError();
+}
+
+hipError_t launch_get_bedge_q(const Launch &L, int nbedges, const int *b2c, const int *bnum,
+                              const int *btype, const double *nx, const double *ny,
+                              const double *w, double *ext) {
+  size_t n = (size_t)nbedges * L.Nfp;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(get_bedge_q_kernel, dim3(grid_for(n, 256, 4096)), dim3(256), 0, L.stream,
+                     nbedges, L.Np, L.Nfp, b2c, bnum, btype, L.fmask, L.ph, nx, ny, w, ext);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_workingQ(const Launch &L, int ncells, double dt, int stage, const double *q,
+                               const double *k1, const double *k2, double *w) {
+  size_t n = (size_t)ncells * L.Np * 4;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(set_workingQ_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, L.stream, n,
+                     dt, stage, q, k1, k2, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_Q(const Launch &L, int ncells, double dt, double *q, const double *k1,
+                           const double *k2, const double *k3, double *w) {
+  size_t n = (size_t)ncells * L.Np * 4;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(update_Q_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, L.stream, n, dt,
+                     q, k1, k2, k3, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_calc_dt(const Launch &L, int ncells, const double *q, const double *fscale,
+                          unsigned long long *d_out) {
+  if (ncells == 0) return hipSuccess;
+  hipLaunchKernelGGL(calc_dt_kernel, dim3(grid_for(ncells, 256, 2048)), dim3(256), 0, L.stream,
+                     ncells, L.N, L.Np, L.Nfp, L.fmask, L.ph.gam, q, fscale, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_grid(const Launch &L, int ncells, const double *cell_xy, double *nodeX,
+                            double *nodeY, double *x, double *y, double *xr, double *yr,
+                            double *xs, double *ys, double *rx, double *ry, double *sx,
+                            double *sy, double *nx, double *ny, double *fscale, double *J) {
+  if (ncells == 0) return hipSuccess;
+  hipLaunchKernelGGL(init_grid_kernel, dim3(grid_for(ncells, 128, 8192)), dim3(128), 0, L.stream,
+                     ncells, L.Np, L.Nfp, L.ops, L.fmask, cell_xy, nodeX, nodeY, x, y, xr, yr, xs,
+                     ys, rx, ry, sx, sy, nx, ny, fscale, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_ic(const Launch &L, int ncells, double *q, double *w) {
+  size_t n = (size_t)ncells * L.Np;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(set_ic_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, L.stream, n, L.ph,
+                     q, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_vortex(const Launch &L, int ncells, const double *x, const double *y,
+                             double uinf, double beta, double x0, double y0, double period_x,
+                             double *q, double *w) {
+  size_t n = (size_t)ncells * L.Np;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(set_vortex_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, L.stream, n,
+                     L.ph.gam, x, y, uinf, beta, x0, y0, period_x, q, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(const Launch &L, size_t n, double *p) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, L.stream, n, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const Launch &L, size_t n, const double *src, double *dst) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, L.stream, n, src,
+                     dst);
+  return hipGetLastError();
+}
+
+}  // namespace dg

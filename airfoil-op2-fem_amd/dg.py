@@ -1,0 +1,337 @@
+"""Host-side mirror of the reference's DG Euler operator interface.
+
+``Context`` plays the part of OP2's op_init/op_decl_const/op_decl_map
+(src/airfoil.cpp:67-201) and exposes one method per op_par_loop of the hot
+path, operating on torch CUDA tensors in the op_dat layouts.  ``Mesh``
+replaces the CGNS loader (src/load_mesh.cpp) with generated or VTK meshes.
+``Solver`` is the native SSPRK3 driver of src/airfoil.cpp:204-334.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import DGError, P, c_double, c_int, c_void_p, check, lib
+
+BC_INFLOW, BC_OUTFLOW, BC_WALL, BC_FARFIELD = 0, 1, 2, 3
+MODE_FUSED, MODE_REFERENCE = 0, 1
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def generate_operators(order):
+    """Order-N DG operator tables (host only): dict of numpy arrays named as
+    the reference's constants (r, s, Dr, Ds, Drw, Dsw, LIFT, FMASK) plus the
+    reference-element mass matrix and the 1D face mass matrix."""
+    n = int(order)
+    np_, nfp = (n + 1) * (n + 2) // 2, n + 1
+    out = {"r": np.zeros(np_), "s": np.zeros(np_), "Dr": np.zeros((np_, np_)),
+           "Ds": np.zeros((np_, np_)), "Drw": np.zeros((np_, np_)), "Dsw": np.zeros((np_, np_)),
+           "LIFT": np.zeros((np_, 3 * nfp)), "FMASK": np.zeros(3 * nfp, dtype=np.int32),
+           "mass": np.zeros((np_, np_)), "face_mass": np.zeros((nfp, nfp))}
+    check(lib.dg_generate_operators(
+        n, *[_np_ptr(out[k]) for k in ("r", "s", "Dr", "Ds", "Drw", "Dsw", "LIFT", "FMASK",
+                                        "mass", "face_mass")]))
+    out.update(N=n, Np=np_, Nfp=nfp)
+    return out
+
+
+# --------------------------------------------------------------------------
+# meshes
+# --------------------------------------------------------------------------
+
+class Mesh:
+    """A triangle mesh with the tools/vtk2CGNS.cpp connectivity contract."""
+
+    def __init__(self, handle):
+        self._h = c_void_p(handle)
+        nn, nc, ne, nb = c_int(), c_int(), c_int(), c_int()
+        check(lib.dg_mesh_sizes(self._h, nn, nc, ne, nb))
+        self.nnodes, self.ncells, self.nedges, self.nbedges = nn.value, nc.value, ne.value, nb.value
+        self._cache = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.dg_mesh_destroy(h)
+            self._h = None
+
+    _SHAPES = {"coords": ("nnodes", (2,), np.float64), "cells": ("ncells", (3,), np.int32),
+               "cell_xy": ("ncells", (3, 2), np.float64), "edge2cells": ("nedges", (2,), np.int32),
+               "edgeNum": ("nedges", (2,), np.int32), "edge2nodes": ("nedges", (2,), np.int32),
+               "bedge2cells": ("nbedges", (), np.int32), "bedgeNum": ("nbedges", (), np.int32),
+               "bedge_type": ("nbedges", (), np.int32), "bedge2nodes": ("nbedges", (2,), np.int32)}
+
+    def get(self, field):
+        if field not in self._cache:
+            cnt, tail, dt = self._SHAPES[field]
+            a = np.zeros((getattr(self, cnt),) + tail, dtype=dt)
+            if a.size:
+                check(lib.dg_mesh_get(self._h, field.encode(), _np_ptr(a)))
+            self._cache[field] = a
+        return self._cache[field]
+
+    def __getattr__(self, name):
+        if name in Mesh._SHAPES:
+            return self.get(name)
+        raise AttributeError(name)
+
+
+def periodic_grid(nx, ny, x0=0.0, y0=-5.0, lx=10.0, ly=10.0, seed=0, jitter=0.1):
+    """Seeded jittered periodic grid of nx*ny quads split into 2*nx*ny CCW triangles."""
+    h = c_void_p()
+    check(lib.dg_mesh_periodic(int(nx), int(ny), float(x0), float(y0), float(lx), float(ly),
+                               int(seed), float(jitter), ctypes.byref(h)))
+    return Mesh(h.value)
+
+
+def naca0012_ogrid(ncirc, nrad, stretch=1.08, rfar=20.0):
+    """O-grid around the NACA0012 profile: 2*ncirc*nrad CCW triangles."""
+    h = c_void_p()
+    check(lib.dg_mesh_ogrid(int(ncirc), int(nrad), float(stretch), float(rfar), ctypes.byref(h)))
+    return Mesh(h.value)
+
+
+def read_vtk(path):
+    """Legacy-VTK triangle mesh, boundary edges classified as src/load_mesh.cpp:147-172."""
+    h = c_void_p()
+    check(lib.dg_mesh_read_vtk(str(path).encode(), ctypes.byref(h)))
+    return Mesh(h.value)
+
+
+# --------------------------------------------------------------------------
+# context: constants, maps and the per-loop kernels
+# --------------------------------------------------------------------------
+
+def _dev(t, name, numel=None):
+    """Pointer of a contiguous float64 CUDA tensor (checked)."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name}: expected a device tensor, got {t.device}")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name}: expected float64, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name}: expected {numel} elements, got {t.numel()}")
+    return c_void_p(t.data_ptr())
+
+
+class Context:
+    """op_init + op_decl_const + op_decl_map, and the op_par_loop drop-ins."""
+
+    _DBL1 = ("gam", "bc_mach", "bc_alpha", "bc_p", "bc_r", "bc_u", "bc_v", "bc_e")
+
+    def __init__(self, order=3, stream=None):
+        h = c_void_p()
+        check(lib.dg_ctx_create(int(order), ctypes.byref(h)))
+        self._h = h
+        n, np_, nfp = c_int(), c_int(), c_int()
+        check(lib.dg_order(self._h, n, np_, nfp), self._h)
+        self.N, self.Np, self.Nfp = n.value, np_.value, nfp.value
+        self.NF = 3 * self.Nfp
+        self.ncells = None
+        if stream is not None:
+            self.set_stream(stream)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.dg_ctx_destroy(h)
+            self._h = None
+
+    def _chk(self, rc):
+        return check(rc, self._h)
+
+    def set_stream(self, stream):
+        """stream: a torch.cuda.Stream or a raw hipStream_t integer."""
+        ptr = getattr(stream, "cuda_stream", stream)
+        self._chk(lib.dg_set_stream(self._h, c_void_p(int(ptr) if ptr else 0)))
+
+    def _dim(self, name):
+        if name in self._DBL1:
+            return 1
+        return {"ones": self.Np, "r": self.Np, "s": self.Np, "Dr": self.Np ** 2,
+                "Ds": self.Np ** 2, "Drw": self.Np ** 2, "Dsw": self.Np ** 2,
+                "LIFT": self.Np * self.NF, "FMASK": self.NF}.get(name, 1)
+
+    def set_const(self, name, value, dim=None):
+        """op_decl_const(dim, type, value) under the reference's constant name."""
+        isint = name == "FMASK"
+        a = np.ascontiguousarray(np.atleast_1d(np.asarray(value, dtype=np.int32 if isint else np.float64)).ravel())
+        d = int(dim) if dim is not None else a.size
+        if a.size < d:
+            raise ValueError(f"{name}: {a.size} values for dim {d}")
+        self._chk(lib.dg_set_const(self._h, name.encode(), d, _np_ptr(a)))
+
+    def get_const(self, name):
+        d = self._dim(name)
+        a = np.zeros(d, dtype=np.int32 if name == "FMASK" else np.float64)
+        self._chk(lib.dg_get_const(self._h, name.encode(), d, _np_ptr(a)))
+        return a[0] if d == 1 and name in self._DBL1 else a
+
+    def set_freestream(self, gam=1.4, mach=0.4, alpha=0.0, p=1.0, rho=1.0):
+        self._chk(lib.dg_set_freestream(self._h, gam, mach, alpha, p, rho))
+
+    def decl_mesh(self, mesh=None, ncells=None, cells=None, edge2cells=None, edgeNum=None,
+                  bedge2cells=None, bedgeNum=None, bedge_type=None):
+        """Declare sets and maps, from a Mesh or from raw int32 arrays."""
+        if mesh is not None:
+            ncells, cells = mesh.ncells, mesh.cells
+            edge2cells, edgeNum = mesh.edge2cells, mesh.edgeNum
+            bedge2cells, bedgeNum, bedge_type = mesh.bedge2cells, mesh.bedgeNum, mesh.bedge_type
+
+        def arr(a):
+            return None if a is None else np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+        cells, e2c, en = arr(cells), arr(edge2cells), arr(edgeNum)
+        b2c, bn, bt = arr(bedge2cells), arr(bedgeNum), arr(bedge_type)
+        ne = 0 if e2c is None else e2c.size // 2
+        nb = 0 if b2c is None else b2c.size
+        ptr = lambda a: None if a is None or a.size == 0 else _np_ptr(a)  # noqa: E731
+        self._chk(lib.dg_decl_mesh(self._h, int(ncells), ptr(cells), ne, ptr(e2c), ptr(en), nb,
+                                   ptr(b2c), ptr(bn), ptr(bt)))
+        self.ncells = int(ncells)
+
+    # ---- shapes of the op_dats ----
+    def state_numel(self):
+        return self.ncells * self.Np * 4
+
+    # ---- op_par_loop drop-ins (device tensors) ----
+    def init_grid(self, cell_xy, d):
+        """init_grid over cells; d is a dict of output tensors named as the op_dats."""
+        K, Np, NF = self.ncells, self.Np, self.NF
+        names = [("nodeX", 3), ("nodeY", 3), ("x", Np), ("y", Np), ("xr", Np), ("yr", Np),
+                 ("xs", Np), ("ys", Np), ("rx", Np), ("ry", Np), ("sx", Np), ("sy", Np),
+                 ("nx", NF), ("ny", NF), ("fscale", NF)]
+        self._chk(lib.dg_init_grid(self._h, _dev(cell_xy, "cell_xy", K * 6),
+                                   *[_dev(d[n], n, K * s) for n, s in names]))
+
+    def set_ic(self, q, workingQ):
+        n = self.state_numel()
+        self._chk(lib.dg_set_ic(self._h, _dev(q, "q", n), _dev(workingQ, "workingQ", n)))
+
+    def neighbour_zero(self, exteriorQ):
+        self._chk(lib.dg_neighbour_zero(self._h, _dev(exteriorQ, "exteriorQ", self.ncells * self.NF * 4)))
+
+    def get_neighbour_q(self, nodeX, nodeY, workingQ, exteriorQ):
+        K = self.ncells
+        self._chk(lib.dg_get_neighbour_q(self._h, _dev(nodeX, "nodeX", K * 3), _dev(nodeY, "nodeY", K * 3),
+                                         _dev(workingQ, "workingQ", self.state_numel()),
+                                         _dev(exteriorQ, "exteriorQ", K * self.NF * 4)))
+
+    def get_bedge_q(self, nx, ny, workingQ, exteriorQ):
+        K, NF = self.ncells, self.NF
+        self._chk(lib.dg_get_bedge_q(self._h, _dev(nx, "nx", K * NF), _dev(ny, "ny", K * NF),
+                                     _dev(workingQ, "workingQ", self.state_numel()),
+                                     _dev(exteriorQ, "exteriorQ", K * NF * 4)))
+
+    def euler_rhs(self, workingQ, exteriorQ, rx, ry, sx, sy, fscale, nx, ny, qRHS):
+        K, Np, NF, n = self.ncells, self.Np, self.NF, self.state_numel()
+        self._chk(lib.dg_euler_rhs(self._h, _dev(workingQ, "workingQ", n),
+                                   _dev(exteriorQ, "exteriorQ", K * NF * 4),
+                                   _dev(rx, "rx", K * Np), _dev(ry, "ry", K * Np),
+                                   _dev(sx, "sx", K * Np), _dev(sy, "sy", K * Np),
+                                   _dev(fscale, "fscale", K * NF), _dev(nx, "nx", K * NF),
+                                   _dev(ny, "ny", K * NF), _dev(qRHS, "qRHS", n)))
+
+    def rhs_fused(self, workingQ, rx, ry, sx, sy, fscale, nx, ny, qRHS):
+        K, Np, NF, n = self.ncells, self.Np, self.NF, self.state_numel()
+        self._chk(lib.dg_rhs_fused(self._h, _dev(workingQ, "workingQ", n),
+                                   _dev(rx, "rx", K * Np), _dev(ry, "ry", K * Np),
+                                   _dev(sx, "sx", K * Np), _dev(sy, "sy", K * Np),
+                                   _dev(fscale, "fscale", K * NF), _dev(nx, "nx", K * NF),
+                                   _dev(ny, "ny", K * NF), _dev(qRHS, "qRHS", n)))
+
+    def set_workingQ(self, dt, stage, q, k1, k2, workingQ):
+        n = self.state_numel()
+        self._chk(lib.dg_set_workingQ(self._h, float(dt), int(stage), _dev(q, "q", n), _dev(k1, "k1", n),
+                                      _dev(k2, "k2", n), _dev(workingQ, "workingQ", n)))
+
+    def update_Q(self, dt, q, rk1, rk2, rk3, workingQ):
+        n = self.state_numel()
+        self._chk(lib.dg_update_Q(self._h, float(dt), _dev(q, "q", n), _dev(rk1, "rk1", n),
+                                  _dev(rk2, "rk2", n), _dev(rk3, "rk3", n),
+                                  _dev(workingQ, "workingQ", n)))
+
+    def calc_dt(self, q, fscale, dt1=0.0):
+        """OP_MAX reduction: returns max(dt1, per-cell CFL quantities)."""
+        v = c_double(dt1)
+        self._chk(lib.dg_calc_dt(self._h, _dev(q, "q", self.state_numel()),
+                                 _dev(fscale, "fscale", self.ncells * self.NF), ctypes.byref(v)))
+        return v.value
+
+
+# --------------------------------------------------------------------------
+# solver
+# --------------------------------------------------------------------------
+
+class Solver:
+    """Native SSPRK3 driver owning the op_dats of one mesh (src/airfoil.cpp:204-334)."""
+
+    def __init__(self, ctx, mesh):
+        self.ctx, self.mesh = ctx, mesh
+        h = c_void_p()
+        check(lib.dg_solver_create(ctx._h, mesh._h, ctypes.byref(h)), ctx._h)
+        self._h = h
+        ctx.ncells = mesh.ncells
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.dg_solver_destroy(h)
+            self._h = None
+
+    def _chk(self, rc):
+        return check(rc, self.ctx._h)
+
+    def set_vortex(self, uinf, beta=5.0, x0=5.0, y0=0.0, period_x=10.0):
+        self._chk(lib.dg_solver_set_vortex(self._h, uinf, beta, x0, y0, period_x))
+
+    @property
+    def state_shape(self):
+        return (self.mesh.ncells, self.ctx.Np, 4)
+
+    def get_q(self):
+        q = np.zeros(self.state_shape)
+        self._chk(lib.dg_solver_get_q(self._h, _np_ptr(q)))
+        return q
+
+    def set_q(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.size != np.prod(self.state_shape):
+            raise ValueError("state has the wrong size")
+        self._chk(lib.dg_solver_set_q(self._h, _np_ptr(q)))
+
+    def dat_ptr(self, name):
+        p = c_void_p()
+        self._chk(lib.dg_solver_dat(self._h, name.encode(), ctypes.byref(p)))
+        return p.value
+
+    def run(self, nsteps, cfl=1.0, mode=MODE_FUSED):
+        self._chk(lib.dg_solver_run(self._h, int(nsteps), float(cfl), int(mode)))
+
+    def time(self):
+        t, dt = c_double(), c_double()
+        self._chk(lib.dg_solver_time(self._h, ctypes.byref(t), ctypes.byref(dt)))
+        return t.value, dt.value
+
+    def reset_time(self):
+        self._chk(lib.dg_solver_reset_time(self._h))
+
+    def check(self):
+        return lib.dg_solver_check(self._h)
+
+    def profile(self, enable=True):
+        self._chk(lib.dg_solver_profile(self._h, 1 if enable else 0))
+
+    def kernel_time(self, name):
+        ms, n = c_double(), c_int()
+        self._chk(lib.dg_solver_kernel_time(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+__all__ = ["Context", "Mesh", "Solver", "DGError", "generate_operators", "periodic_grid",
+           "naca0012_ogrid", "read_vtk", "P", "c_void_p"]

@@ -1,0 +1,168 @@
+"""Throughput benchmark of the fused HIP SSPRK3 path on one MI355X.
+
+Workload: isentropic vortex on the 724x724 jittered periodic grid
+(1,048,352 triangles), order N=3 (Np=10, 41.9M DoF), FP64, SSPRK3 with
+dt = 0.5/dt1 recomputed every step.  A step is one SSPRK3 step (three RK
+stages); the metric is DoF-updates per second per RK stage.
+
+Prints one JSON line.  Per-kernel times (HIP events on the solver's stream)
+give the roofline entry of the dominant stage kernel; the oracle's OpenMP
+restatement timed on a bounded sample on the host cores gives cpu_baseline.
+"""
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cell_bytes(N, stage):
+    """Algorithmic bytes per cell of one fused stage kernel (FP64, int32 map).
+
+    Reads: stage input state (Np*4), neighbour face traces (3*Nfp*4),
+    rx/ry/sx/sy (4*Np), nx/ny/fscale (3*3*Nfp), face map (3 ints); writes the
+    next state (Np*4).  Stages 1 and 2 also read the step's base state q."""
+    Np, NF = (N + 1) * (N + 2) // 2, 3 * (N + 1)
+    b = 8 * (Np * 4 + NF * 4 + 4 * Np + 3 * NF + Np * 4) + 4 * 3
+    if stage > 0:
+        b += 8 * Np * 4
+    return b
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def load_traffic(kernel):
+    """Measured HBM bytes per launch from the committed rocprofv3 PMC summary."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get(kernel, {}).get("bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=724, help="quads per side of the periodic grid")
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding of this path is a design, not a build)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    dg = importlib.import_module("airfoil-op2-fem_amd")
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+
+    N, n = args.order, args.n
+    gam, mach = 1.4, 0.5
+    uinf = mach * math.sqrt(gam)
+    t0 = time.perf_counter()
+    mesh = dg.periodic_grid(n, n, 0.0, -5.0, 10.0, 10.0, seed=0, jitter=0.1)
+    ctx = dg.Context(N)
+    ctx.set_freestream(gam, mach, 0.0, 1.0, 1.0)
+    solver = dg.Solver(ctx, mesh)
+    solver.set_vortex(uinf, 5.0, 5.0, 0.0, 10.0)
+    K, Np = mesh.ncells, ctx.Np
+    dof = K * Np * 4
+    print(f"[bench] mesh {n}x{n}: {K} cells, {dof} DoF, set-up {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    solver.run(args.warmup, cfl=args.cfl)
+    torch.cuda.synchronize()
+    q_start = solver.get_q() if not args.no_cpu_baseline else None
+
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    solver.run(args.steps, cfl=args.cfl)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    elapsed = t2 - t1
+    if solver.check() != 0:
+        raise SystemExit("non-finite state during the timed run")
+    ms_step = 1e3 * elapsed / max(args.steps, 1)
+    value = dof * 3 * args.steps / elapsed
+    print(f"[bench] {args.steps} steps in {elapsed:.3f}s: {ms_step:.3f} ms/step, "
+          f"{value:.4e} DoF-updates/s per RK stage", file=sys.stderr, flush=True)
+
+    # per-kernel durations with HIP events on the solver stream
+    solver.profile(True)
+    solver.run(args.profile_steps, cfl=args.cfl)
+    kt = {k: solver.kernel_time(k) for k in ("rhs_stage0", "rhs_stage1", "rhs_stage2")}
+    solver.profile(False)
+    for k, (ms, cnt) in kt.items():
+        gbs = cell_bytes(N, int(k[-1])) * K / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        print(f"[bench] {k}: {ms * 1e3:.1f} us avg over {cnt} launches, {gbs:.0f} GB/s algorithmic",
+              file=sys.stderr, flush=True)
+    dom = max(kt, key=lambda k: kt[k][0])
+    dom_ms = kt[dom][0]
+    achieved = cell_bytes(N, int(dom[-1])) * K / (dom_ms * 1e-3) / 1e9
+    traffic = load_traffic(dom)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes_per_launch": cell_bytes(N, int(dom[-1])) * K,
+                "avg_launch_ms": round(dom_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        cores = host_cores()
+        o = oracle.Oracle(dg.generate_operators(N), oracle.freestream(gam, mach, 0.0))
+        case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
+                           mesh.bedgeNum, mesh.bedge_type)
+        st = o.new_state(case, q_start)
+        kt_cpu = np.zeros(6)
+        c0 = time.perf_counter()
+        o.run(case, st, args.cpu_steps, cfl=args.cfl, nthreads=cores, ktime=kt_cpu)
+        c1 = time.perf_counter()
+        cpu_val = dof * 3 * args.cpu_steps / (c1 - c0)
+        cpu = {"value": cpu_val, "unit": "DoF-updates/s per RK stage", "cores": cores, "kind": "port",
+               "sample": f"oracle OpenMP restatement (reference op2 openmp loop structure), same "
+                         f"{n}x{n} mesh and state, {args.cpu_steps} SSPRK3 steps incl. calc_dt, "
+                         f"{c1 - c0:.1f}s wall",
+               "kernel_s": dict(zip(["get_neighbour_q", "get_bedge_q", "euler_rhs", "set_workingQ",
+                                     "update_Q", "calc_dt"], [round(float(v), 3) for v in kt_cpu]))}
+        print(f"[bench] cpu baseline: {cpu_val:.4e} DoF-updates/s per RK stage on {cores} threads",
+              file=sys.stderr, flush=True)
+
+    out = {"metric": "DoF-updates/s per RK stage (FP64, N=3)", "value": value,
+           "unit": "DoF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (seeded jittered periodic mesh, analytic vortex state)",
+           "config": {"workload": f"isentropic vortex, {n}x{n} jittered periodic grid "
+                                  f"({K} triangles), N={N}, SSPRK3 CFL {args.cfl}, fused stage kernels",
+                      "cells": K, "order": N, "dof": dof, "parallelism": "single GPU"},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,227 @@
+/*
+ * dg_euler.h -- C ABI of the MI355X (gfx950) nodal DG Euler solver.
+ *
+ * Drop-in for the OP2 par_loop backends of the reference's DG Euler hot
+ * path.  The reference driver (src/airfoil.cpp:252-334) calls, per SSPRK3
+ * stage, op_par_loop_get_neighbour_q, op_par_loop_get_bedge_q,
+ * op_par_loop_euler_rhs and op_par_loop_set_workingQ, then per step
+ * op_par_loop_update_Q and op_par_loop_calc_dt (declared at
+ * src/airfoil_op.cpp:17-98).  Each dg_<kernel> entry point below performs the
+ * same set-level loop on device memory in the reference's op_dat layouts:
+ *
+ *   q, workingQ, rk1..3, qRHS   [ncells][Np][4]      (variables fastest)
+ *   x, y, xr, .., rx, ry, sx, sy [ncells][Np]
+ *   nx, ny, fscale               [ncells][3*Nfp]
+ *   exteriorQ                    [ncells][3][Nfp][4]
+ *   nodeX, nodeY                 [ncells][3]
+ *   edge2cells, edgeNum          [nedges][2]          int32
+ *   bedge2cells, bedgeNum, bedge_type [nbedges]       int32
+ *
+ * with Np = (N+1)(N+2)/2 and Nfp = N+1 for the context's order N (the
+ * reference hardcodes N=4).  Pointers named d_* are device pointers owned by
+ * the caller; the library allocates only what a context or solver owns.
+ *
+ * Error handling: every int-returning function returns DG_OK (0) or a
+ * negative DG_ERR_* code and never exits; dg_last_error(ctx) gives the text
+ * (the reference aborts instead: cutilSafeCall, cg_error_exit, exit(1) in
+ * op_decl_const_char, src/cuda/airfoil_kernels.cu:33-107).
+ *
+ * Execution: work is enqueued on the context's stream (dg_set_stream,
+ * default the null stream) and returns without synchronising, except
+ * functions that hand a host scalar back (dg_calc_dt, dg_solver_* getters),
+ * which synchronise that stream.  One context per host thread.
+ */
+#ifndef DG_EULER_H
+#define DG_EULER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DG_OK = 0,
+  DG_ERR_ARG = -1,       /* bad argument (null pointer, size, order, name) */
+  DG_ERR_STATE = -2,     /* call out of order (e.g. no mesh declared) */
+  DG_ERR_HIP = -3,       /* HIP runtime error */
+  DG_ERR_NAME = -4,      /* unknown constant name */
+  DG_ERR_MESH = -5,      /* invalid mesh connectivity */
+  DG_ERR_NONFINITE = -6  /* non-finite or non-positive state detected */
+};
+
+/* Boundary-condition types of bedge_type (src/load_mesh.cpp:147-172) */
+enum { DG_BC_INFLOW = 0, DG_BC_OUTFLOW = 1, DG_BC_WALL = 2, DG_BC_FARFIELD = 3 };
+
+typedef struct dg_ctx dg_ctx;
+typedef struct dg_mesh dg_mesh;
+typedef struct dg_solver dg_solver;
+
+/* ------------------------------------------------------------------ */
+/* Context and constants                                              */
+/* ------------------------------------------------------------------ */
+
+/* Creates a context for polynomial order N in [1,4] on the current HIP
+ * device.  The operator tables are generated for N (Hesthaven & Warburton
+ * construction) and uploaded; dg_set_const may replace them.  Replaces
+ * op_init (src/airfoil.cpp:67). */
+int dg_ctx_create(int order, dg_ctx **out);
+void dg_ctx_destroy(dg_ctx *ctx);
+/* Last error text of ctx; with ctx == NULL, of the calling thread's last
+ * failed call that had no context (creation, mesh generation). */
+const char *dg_last_error(const dg_ctx *ctx);
+int dg_order(const dg_ctx *ctx, int *N, int *Np, int *Nfp);
+/* Stream for all later work of ctx (a hipStream_t; NULL = null stream). */
+int dg_set_stream(dg_ctx *ctx, void *hip_stream);
+
+/* Host-only operator generator (no device needed): fills the order-N tables
+ * that the reference declares as literals in src/constants/constant_*.h.  Any
+ * output pointer may be NULL.  Sizes: r, s [Np]; Dr, Ds, Drw, Dsw, mass [Np*Np];
+ * LIFT [Np*3*Nfp]; FMASK [3*Nfp]; face_mass [Nfp*Nfp]. */
+int dg_generate_operators(int order, double *r, double *s, double *Dr, double *Ds,
+                          double *Drw, double *Dsw, double *LIFT, int *FMASK,
+                          double *mass, double *face_mass);
+
+/* Replaces op_decl_const2 -> op_decl_const_char (src/airfoil_op.cpp:283-299,
+ * src/cuda/airfoil_kernels.cu:33-107).  Names: gam, bc_mach, bc_alpha, bc_p,
+ * bc_r, bc_u, bc_v, bc_e (dim 1, double); ones, r, s (Np); Dr, Ds, Drw, Dsw
+ * (Np*Np); LIFT (Np*3*Nfp); FMASK (3*Nfp, int).  dim must match.  Unknown
+ * names return DG_ERR_NAME instead of exiting. */
+int dg_set_const(dg_ctx *ctx, const char *name, int dim, const void *host_data);
+int dg_get_const(const dg_ctx *ctx, const char *name, int dim, void *host_out);
+/* Free-stream set-up of src/airfoil.cpp:79-99: bc_u, bc_v from Mach and the
+ * angle of attack (degrees), bc_e = p/(gam-1) + rho(u^2+v^2)/2. */
+int dg_set_freestream(dg_ctx *ctx, double gam, double mach, double alpha_deg,
+                      double p, double rho);
+
+/* Declares the mesh sets and maps (op_decl_set/op_decl_map of
+ * src/airfoil.cpp:134-146) from host arrays.  cell2nodes ([ncells][3],
+ * counter-clockwise) may be NULL; when given, neighbour orientation is
+ * checked.  Builds the owner-computes face map on the device.  A face not
+ * covered exactly once by the edge lists returns DG_ERR_MESH. */
+int dg_decl_mesh(dg_ctx *ctx, int ncells, const int *cell2nodes, int nedges,
+                 const int *edge2cells, const int *edgeNum, int nbedges,
+                 const int *bedge2cells, const int *bedgeNum, const int *bedge_type);
+
+/* ------------------------------------------------------------------ */
+/* The reference's per-set loops (drop-ins for op_par_loop_<kernel>)   */
+/* ------------------------------------------------------------------ */
+
+/* src/init_grid.h:1-106 over cells.  d_cell_xy is [ncells][3][2]: the three
+ * vertex coordinates of each cell (node_coords through cell2nodes). */
+int dg_init_grid(dg_ctx *ctx, const double *d_cell_xy, double *d_nodeX,
+                 double *d_nodeY, double *d_x, double *d_y, double *d_xr,
+                 double *d_yr, double *d_xs, double *d_ys, double *d_rx,
+                 double *d_ry, double *d_sx, double *d_sy, double *d_nx,
+                 double *d_ny, double *d_fscale);
+/* src/set_ic.h:1-12 over cells (uniform free stream). */
+int dg_set_ic(dg_ctx *ctx, double *d_q, double *d_workingQ);
+/* src/neighbour_zero.h:1-5 over cells. */
+int dg_neighbour_zero(dg_ctx *ctx, double *d_exteriorQ);
+/* src/get_neighbour_q.h:1-89 over edges: exteriorQ += neighbour face traces
+ * (OP_INC).  Gather-only: every face slot is written by the thread that owns
+ * it, so no atomics or colouring are needed and results are deterministic.
+ * nodeX/nodeY are accepted for signature parity; orientation comes from the
+ * declared topology (always reversed on counter-clockwise meshes). */
+int dg_get_neighbour_q(dg_ctx *ctx, const double *d_nodeX, const double *d_nodeY,
+                       const double *d_workingQ, double *d_exteriorQ);
+/* src/get_bedge_q.h:1-51 over boundary edges (OP_INC), types 0-3. */
+int dg_get_bedge_q(dg_ctx *ctx, const double *d_nx, const double *d_ny,
+                   const double *d_workingQ, double *d_exteriorQ);
+/* src/euler_rhs.h:222-291 over cells: volume term, Roe surface flux, LIFT;
+ * zeroes exteriorQ on exit (OP_RW) and writes qRHS (OP_WRITE). */
+int dg_euler_rhs(dg_ctx *ctx, const double *d_workingQ, double *d_exteriorQ,
+                 const double *d_rx, const double *d_ry, const double *d_sx,
+                 const double *d_sy, const double *d_fscale, const double *d_nx,
+                 const double *d_ny, double *d_qRHS);
+/* src/set_workingQ.h:1-12 over cells; stage 0 or 1. */
+int dg_set_workingQ(dg_ctx *ctx, double dt, int stage, const double *d_q,
+                    const double *d_k1, const double *d_k2, double *d_workingQ);
+/* src/update_Q.h:1-12 over cells. */
+int dg_update_Q(dg_ctx *ctx, double dt, double *d_q, const double *d_rk1,
+                const double *d_rk2, const double *d_rk3, double *d_workingQ);
+/* src/calc_dt.h:1-22 over cells with OP_MAX: *dt1 = max(*dt1, cell values).
+ * Synchronises the stream. */
+int dg_calc_dt(dg_ctx *ctx, const double *d_q, const double *d_fscale, double *dt1);
+
+/* ------------------------------------------------------------------ */
+/* Fused kernels (no reference counterpart; same mathematics)          */
+/* ------------------------------------------------------------------ */
+
+/* get_neighbour_q + get_bedge_q + euler_rhs in one pass: the neighbour and
+ * boundary traces are gathered straight from workingQ, exteriorQ is never
+ * formed.  Writes qRHS. */
+int dg_rhs_fused(dg_ctx *ctx, const double *d_workingQ, const double *d_rx,
+                 const double *d_ry, const double *d_sx, const double *d_sy,
+                 const double *d_fscale, const double *d_nx, const double *d_ny,
+                 double *d_qRHS);
+
+/* ------------------------------------------------------------------ */
+/* Mesh generation (host only)                                        */
+/* ------------------------------------------------------------------ */
+
+/* Seeded jittered periodic grid: nx*ny quads over [x0,x0+Lx]x[y0,y0+Ly],
+ * each split SW->NE into two CCW triangles; vertices off the seams jittered
+ * by U(-jitter*h, jitter*h) from splitmix64(seed). */
+int dg_mesh_periodic(int nx, int ny, double x0, double y0, double Lx, double Ly,
+                     uint64_t seed, double jitter, dg_mesh **out);
+/* O-grid around NACA0012 (naca0012.geo:14 profile); wall on the profile,
+ * characteristic far field on the outer circle. */
+int dg_mesh_ogrid(int ncirc, int nrad, double stretch, double rfar, dg_mesh **out);
+/* Legacy-VTK ASCII triangle mesh with tools/vtk2CGNS.cpp edge numbering and
+ * src/load_mesh.cpp:147-172 boundary classification. */
+int dg_mesh_read_vtk(const char *path, dg_mesh **out);
+void dg_mesh_destroy(dg_mesh *m);
+int dg_mesh_sizes(const dg_mesh *m, int *nnodes, int *ncells, int *nedges, int *nbedges);
+/* Copies one field to host memory: "coords" (double [nnodes][2]), "cells"
+ * (int [ncells][3]), "cell_xy" (double [ncells][3][2]), "edge2cells",
+ * "edgeNum", "edge2nodes" (int [nedges][2]), "bedge2cells", "bedgeNum",
+ * "bedge_type" (int [nbedges]), "bedge2nodes" (int [nbedges][2]). */
+int dg_mesh_get(const dg_mesh *m, const char *field, void *host_out);
+
+/* ------------------------------------------------------------------ */
+/* Solver: the SSPRK3 driver of src/airfoil.cpp:204-334 on device      */
+/* ------------------------------------------------------------------ */
+
+/* Declares the mesh on ctx, allocates the op_dats (q, workingQ, rk1..3,
+ * exteriorQ, geometry) on the device and runs init_grid and set_ic. */
+int dg_solver_create(dg_ctx *ctx, const dg_mesh *mesh, dg_solver **out);
+void dg_solver_destroy(dg_solver *s);
+/* Isentropic vortex initial state (vortex/src/set_ic.h:1-28 with the
+ * free-stream speed uinf and the advection of the centre): centre (x0,y0),
+ * strength beta, gamma from the context, periodic in x with period_x (0 =
+ * not periodic). */
+int dg_solver_set_vortex(dg_solver *s, double uinf, double beta, double x0,
+                         double y0, double period_x);
+int dg_solver_set_q(dg_solver *s, const double *host_q);
+int dg_solver_get_q(dg_solver *s, double *host_q);
+/* Device pointer of an op_dat owned by the solver: "q", "workingQ", "rk1",
+ * "rk2", "rk3", "exteriorQ", "x", "y", "rx", "ry", "sx", "sy", "nx", "ny",
+ * "fscale", "nodeX", "nodeY", "J" ([ncells], cell Jacobian). */
+int dg_solver_dat(dg_solver *s, const char *name, void **d_ptr);
+
+enum {
+  DG_MODE_FUSED = 0,     /* fused stage kernels, dt kept on the device */
+  DG_MODE_REFERENCE = 1  /* the reference's 9 loops per step, dt on the host */
+};
+/* Runs nsteps SSPRK3 steps with dt = cfl/dt1 recomputed after every step
+ * (src/airfoil.cpp:231-237,322-331; the reference uses cfl = 1).  The first
+ * call computes dt from the current q.  Asynchronous in DG_MODE_FUSED. */
+int dg_solver_run(dg_solver *s, int nsteps, double cfl, int mode);
+/* Simulated time and the dt the next step will use (synchronises). */
+int dg_solver_time(dg_solver *s, double *t, double *dt);
+/* Resets time to 0 and forces dt to be recomputed from q on the next run. */
+int dg_solver_reset_time(dg_solver *s);
+/* Returns DG_ERR_NONFINITE if any step so far produced a non-finite value
+ * in the CFL reduction (fused mode), DG_OK otherwise (synchronises). */
+int dg_solver_check(dg_solver *s);
+/* Kernel timing with HIP events on the context stream: enable, then read the
+ * average duration (ms) and launch count per kernel name ("rhs_stage0",
+ * "rhs_stage1", "rhs_stage2", "calc_dt", "euler_rhs", ...). */
+int dg_solver_profile(dg_solver *s, int enable);
+int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DG_EULER_H */

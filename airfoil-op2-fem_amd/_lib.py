@@ -34,6 +34,7 @@ _SIGS = {
     "dg_last_error": ([c_void_p], c_char_p),
     "dg_order": ([c_void_p, P(c_int), P(c_int), P(c_int)], c_int),
     "dg_set_stream": ([c_void_p, c_void_p], c_int),
+    "dg_set_variant": ([c_void_p, c_int, c_int], c_int),
     "dg_generate_operators": ([c_int] + [c_void_p] * 9, c_int),
     "dg_set_const": ([c_void_p, c_char_p, c_int, c_void_p], c_int),
     "dg_get_const": ([c_void_p, c_char_p, c_int, c_void_p], c_int),

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -28,7 +29,9 @@ struct dg_ctx {
          bc_v = 0.0, bc_e = 0.0;
   double *d_ops = nullptr;
   int *d_fmask = nullptr;
+  int cus = 0;
   int max_blocks = 0;
+  int variant = 0;
   // declared mesh
   int ncells = -1, nedges = 0, nbedges = 0;
   int *d_fmap = nullptr, *d_b2c = nullptr, *d_bnum = nullptr, *d_btype = nullptr;
@@ -67,7 +70,9 @@ dg::Launch launch_of(const dg_ctx *c) {
   uint64_t mask = 0;
   for (int f : c->FMASK) mask |= (1ull << f);
   L.facenodes = mask;
+  L.cus = c->cus;
   L.max_blocks = c->max_blocks;
+  L.variant = c->variant;
   return L;
 }
 
@@ -132,8 +137,9 @@ int dg_ctx_create(int order, dg_ctx **out) {
   if (he == hipSuccess) {
     hipDeviceProp_t prop;
     he = hipGetDeviceProperties(&prop, c->device);
-    if (he == hipSuccess) c->max_blocks = prop.multiProcessorCount * 2;
+    if (he == hipSuccess) c->cus = prop.multiProcessorCount;
   }
+  if (const char *v = std::getenv("DG_STAGE_VARIANT")) c->variant = std::atoi(v) == 1 ? 1 : 0;
   if (he == hipSuccess) he = hipMalloc(&c->d_ops, sizeof(double) * dg::ops_size(c->Np, c->NF));
   if (he == hipSuccess) he = hipMalloc(&c->d_fmask, sizeof(int) * c->NF);
   if (he == hipSuccess) he = hipMalloc(&c->d_red, sizeof(unsigned long long) * 4);
@@ -169,6 +175,15 @@ int dg_order(const dg_ctx *c, int *N, int *Np, int *Nfp) {
   if (N) *N = c->N;
   if (Np) *Np = c->Np;
   if (Nfp) *Nfp = c->Nfp;
+  return DG_OK;
+}
+
+int dg_set_variant(dg_ctx *c, int variant, int max_blocks) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_set_variant: null context");
+  if (variant != 0 && variant != 1) return fail(c, DG_ERR_ARG, "dg_set_variant: variant must be 0 or 1");
+  if (max_blocks < 0) return fail(c, DG_ERR_ARG, "dg_set_variant: max_blocks must be >= 0");
+  c->variant = variant;
+  c->max_blocks = max_blocks;
   return DG_OK;
 }
 
@@ -339,6 +354,7 @@ int dg_get_neighbour_q(dg_ctx *c, const double *nodeX, const double *nodeY, cons
   (void)nodeX; (void)nodeY;
   NEED_MESH(c, "dg_get_neighbour_q");
   if (c->ncells > 0 && (!w || !ext)) return fail(c, DG_ERR_ARG, "dg_get_neighbour_q: null array");
+  if (!al16(w) || !al16(ext)) return fail(c, DG_ERR_ARG, "dg_get_neighbour_q: arrays must be 16-byte aligned");
   HIPCHECK(c, dg::launch_get_neighbour_q(launch_of(c), c->ncells, c->d_fmap, w, ext));
   return DG_OK;
 }

@@ -23,7 +23,9 @@ struct Launch {
   const int *fmask;    // device FMASK [NF]
   Phys ph;
   uint64_t facenodes;  // bit j set when node j lies on a face
-  int max_blocks;      // persistent grid cap (CUs x resident blocks)
+  int cus;             // compute units of the device
+  int max_blocks;      // optional cap on persistent grids (0: none)
+  int variant;         // stage kernel: 0 MFMA (wave tiles), 1 LDS/VALU (workgroup tiles)
 };
 
 // Arguments of the stage kernels.
@@ -47,6 +49,7 @@ struct StageArgs {
 enum Epi { EPI_RHS = 0, EPI_STAGE0 = 1, EPI_STAGE1 = 2, EPI_STAGE2 = 3 };
 
 hipError_t launch_rhs(const Launch &L, const StageArgs &a, bool gather, int epi);
+hipError_t launch_rhs_mfma(const Launch &L, const StageArgs &a, bool gather, int epi);
 hipError_t launch_get_neighbour_q(const Launch &L, int ncells, const int *fmap,
                                   const double *w, double *ext);
 hipError_t launch_get_bedge_q(const Launch &L, int nbedges, const int *b2c,

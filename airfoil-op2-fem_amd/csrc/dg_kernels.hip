@@ -21,180 +21,10 @@
 // face traces.
 #include <hip/hip_runtime.h>
 
+#include "dg_device.h"
 #include "dg_internal.h"
 
 namespace dg {
-
-template <int N>
-struct Ord {
-  static constexpr int Np = (N + 1) * (N + 2) / 2;
-  static constexpr int Nfp = N + 1;
-  static constexpr int NF = 3 * (N + 1);
-};
-
-// ------------------------------------------------------------------------
-// Point-wise physics (src/euler_rhs.h:7-23, 120-220; src/get_bedge_q.h)
-// ------------------------------------------------------------------------
-
-__device__ __forceinline__ void euler_flux(double gam, const double q[4], double F[4],
-                                           double G[4]) {
-  double rho = q[0];
-  double u = q[1] / rho;
-  double v = q[2] / rho;
-  double p = (gam - 1) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
-  F[0] = q[1];
-  F[1] = q[1] * u + p;
-  F[2] = q[2] * u;
-  F[3] = u * (q[3] + p);
-  G[0] = q[2];
-  G[1] = q[1] * v;
-  G[2] = q[2] * v + p;
-  G[3] = v * (q[3] + p);
-}
-
-// Roe flux at one face node, rotated onto the normal, rotated back, times
-// fscale (src/euler_rhs.h:151-219).
-__device__ __forceinline__ void roe_node(double gam, const double mQ[4], const double eQ[4],
-                                         double nx, double ny, double fs, double out[4]) {
-  double mR0 = mQ[0], mR1 = nx * mQ[1] + ny * mQ[2], mR2 = -ny * mQ[1] + nx * mQ[2], mR3 = mQ[3];
-  double pR0 = eQ[0], pR1 = nx * eQ[1] + ny * eQ[2], pR2 = -ny * eQ[1] + nx * eQ[2], pR3 = eQ[3];
-  double mU = mR1 / mR0, mV = mR2 / mR0;
-  double mP = (gam - 1) * (mR3 - 0.5 * (mR1 * mU + mR2 * mV));
-  double pU = pR1 / pR0, pV = pR2 / pR0;
-  double pP = (gam - 1) * (pR3 - 0.5 * (pR1 * pU + pR2 * pV));
-  double fm0 = mR1, fm1 = mR1 * mU + mP, fm2 = mR2 * mU, fm3 = mU * (mR3 + mP);
-  double fp0 = pR1, fp1 = pR1 * pU + pP, fp2 = pR2 * pU, fp3 = pU * (pR3 + pP);
-  double mH = (mR3 + mP) / mR0;
-  double pH = (pR3 + pP) / pR0;
-  double ms = sqrt(mR0), ps = sqrt(pR0);
-  double rho = ms * ps;
-  double den = ms + ps;
-  double u = (ms * mU + ps * pU) / den;
-  double v = (ms * mV + ps * pV) / den;
-  double H = (ms * mH + ps * pH) / den;
-  double c2 = (gam - 1) * (H - 0.5 * (u * u + v * v));
-  double c = sqrt(c2);
-  double dW1 = -0.5 * rho * (pU - mU) / c + 0.5 * (pP - mP) / c2;
-  double dW2 = (pR0 - mR0) - (pP - mP) / c2;
-  double dW3 = rho * (pV - mV);
-  double dW4 = 0.5 * rho * (pU - mU) / c + 0.5 * (pP - mP) / c2;
-  dW1 = fabs(u - c) * dW1;
-  dW2 = fabs(u) * dW2;
-  dW3 = fabs(u) * dW3;
-  dW4 = fabs(u + c) * dW4;
-  double f0 = (fm0 + fp0) / 2.0 - (dW1 + dW2 + dW4) / 2.0;
-  double f1 = (fm1 + fp1) / 2.0 - (dW1 * (u - c) + dW2 * u + dW4 * (u + c)) / 2.0;
-  double f2 = (fm2 + fp2) / 2.0 - (dW1 * v + dW2 * v + dW3 + dW4 * v) / 2.0;
-  double f3 = (fm3 + fp3) / 2.0 -
-              (dW1 * (H - u * c) + dW2 * (u * u + v * v) + dW3 * v + dW4 * (H + u * c)) / 2.0;
-  out[0] = f0 * fs;
-  out[1] = (nx * f1 - ny * f2) * fs;
-  out[2] = (ny * f1 + nx * f2) * fs;
-  out[3] = f3 * fs;
-}
-
-// Characteristic (Riemann-invariant) far-field state; no reference
-// counterpart.  Subsonic: R+ from inside, R- from the free stream, entropy
-// and tangential velocity from the upwind side.  Supersonic: free stream on
-// inflow, interior on outflow.
-__device__ __forceinline__ void farfield_state(const Phys &ph, const double qi[4], double nx,
-                                               double ny, double qb[4]) {
-  const double g = ph.gam;
-  double rhoi = qi[0], ui = qi[1] / rhoi, vi = qi[2] / rhoi;
-  double pi_ = (g - 1.0) * (qi[3] - 0.5 * rhoi * (ui * ui + vi * vi));
-  double ci = sqrt(fabs(g * pi_ / rhoi));
-  double vni = ui * nx + vi * ny;
-  double cinf = sqrt(g * ph.bc_p / ph.bc_r);
-  double vninf = ph.bc_u * nx + ph.bc_v * ny;
-  if (fabs(vni) >= ci) {
-    if (vni < 0.0) {
-      qb[0] = ph.bc_r; qb[1] = ph.bc_r * ph.bc_u; qb[2] = ph.bc_r * ph.bc_v; qb[3] = ph.bc_e;
-    } else {
-      qb[0] = qi[0]; qb[1] = qi[1]; qb[2] = qi[2]; qb[3] = qi[3];
-    }
-    return;
-  }
-  double rp = vni + 2.0 * ci / (g - 1.0);
-  double rm = vninf - 2.0 * cinf / (g - 1.0);
-  double vnb = 0.5 * (rp + rm);
-  double cb = 0.25 * (g - 1.0) * (rp - rm);
-  double ent, ut, vt;
-  if (vnb > 0.0) {
-    ent = pi_ / pow(rhoi, g);
-    ut = ui - vni * nx; vt = vi - vni * ny;
-  } else {
-    ent = ph.bc_p / pow(ph.bc_r, g);
-    ut = ph.bc_u - vninf * nx; vt = ph.bc_v - vninf * ny;
-  }
-  double rhob = pow(cb * cb / (g * ent), 1.0 / (g - 1.0));
-  double pb = rhob * cb * cb / g;
-  double ub = ut + vnb * nx, vb = vt + vnb * ny;
-  qb[0] = rhob;
-  qb[1] = rhob * ub;
-  qb[2] = rhob * vb;
-  qb[3] = pb / (g - 1.0) + 0.5 * rhob * (ub * ub + vb * vb);
-}
-
-// Exterior state on a boundary face node (src/get_bedge_q.h:24-50 + type 3).
-__device__ __forceinline__ void boundary_state(const Phys &ph, int type, const double mQ[4],
-                                               double nx, double ny, double qb[4]) {
-  if (type == 0) {
-    qb[0] = ph.bc_r; qb[1] = ph.bc_r * ph.bc_u; qb[2] = ph.bc_r * ph.bc_v; qb[3] = ph.bc_e;
-  } else if (type == 1) {
-    qb[0] = ph.bc_r; qb[1] = ph.bc_r * ph.bc_u; qb[2] = ph.bc_r * ph.bc_v; qb[3] = mQ[3];
-  } else if (type == 2) {
-    double vn = nx * mQ[1] + ny * mQ[2];
-    qb[0] = mQ[0];
-    qb[1] = mQ[1] - 2 * vn * nx;
-    qb[2] = mQ[2] - 2 * vn * ny;
-    qb[3] = mQ[3];
-  } else {
-    farfield_state(ph, mQ, nx, ny, qb);
-  }
-}
-
-// CFL quantity of one node (src/calc_dt.h:7-12 with (N+1)^2).
-__device__ __forceinline__ double cfl_node(double gam, double n1sq, const double q[4], double fs) {
-  double rho = q[0];
-  double u = q[1] / rho;
-  double v = q[2] / rho;
-  double p = (gam - 1.0) * (q[3] - rho * (u * u + v * v) * 0.5);
-  double c = sqrt(fabs(gam * p / rho));
-  return n1sq * 0.5 * fs * (sqrt(u * u + v * v) + c);
-}
-
-__device__ __forceinline__ unsigned long long max_bits(unsigned long long a, unsigned long long b) {
-  return a > b ? a : b;
-}
-
-// Max over a workgroup of non-negative doubles held as bit patterns (a NaN
-// pattern compares above +inf, so it propagates).
-template <int NT>
-__device__ unsigned long long block_max_bits(unsigned long long v, unsigned long long *red) {
-  for (int off = 32; off > 0; off >>= 1) v = max_bits(v, __shfl_xor(v, off, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  unsigned long long r = 0;
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < NT / 64; i++) r = max_bits(r, red[i]);
-  }
-  __syncthreads();
-  return r;
-}
-
-// Maps a persistent workgroup's k-th visit to a tile so that the workgroups
-// of one XCD (blockIdx % 8 on MI355X) walk one contiguous chunk of tiles.
-__device__ __forceinline__ int xcd_tile(int k, int ntiles) {
-  const int nxcd = 8;
-  const int g = gridDim.x;
-  if (g % nxcd != 0) return blockIdx.x + k * g;
-  const int xcd = blockIdx.x % nxcd, local = blockIdx.x / nxcd, per = g / nxcd;
-  const int chunk = (ntiles + nxcd - 1) / nxcd;
-  const int t = local + k * per;
-  if (t >= chunk) return ntiles;  // this XCD's chunk is done
-  return xcd * chunk + t;
-}
 
 // ------------------------------------------------------------------------
 // Stage kernel
@@ -374,7 +204,8 @@ template <int N, bool GATHER, int EPI>
 static hipError_t launch_rhs_t(const Launch &L, const StageArgs &a) {
   constexpr int TE = TileCfg<N>::TE, NT = TileCfg<N>::NT;
   const int ntiles = (a.ncells + TE - 1) / TE;
-  int grid = L.max_blocks > 0 ? L.max_blocks : 2048;
+  int grid = L.cus > 0 ? 2 * L.cus : 512;
+  if (L.max_blocks > 0 && grid > L.max_blocks) grid = L.max_blocks;
   if (grid > ntiles) grid = ntiles;
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) return hipSuccess;
@@ -397,6 +228,7 @@ static hipError_t launch_rhs_n(const Launch &L, const StageArgs &a, bool gather,
 
 hipError_t launch_rhs(const Launch &L, const StageArgs &a, bool gather, int epi) {
   if (!gather && epi != EPI_RHS) return hipErrorInvalidValue;
+  if (L.variant == 0) return launch_rhs_mfma(L, a, gather, epi);
   switch (L.N) {
     case 1: return launch_rhs_n<1>(L, a, gather, epi);
     case 2: return launch_rhs_n<2>(L, a, gather, epi);

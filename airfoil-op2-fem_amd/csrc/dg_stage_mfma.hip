@@ -1,0 +1,275 @@
+// Stage kernel with the dense contractions on the FP64 matrix cores of gfx950.
+//
+// Each workgroup is one wave (64 lanes) that owns tiles of 16 consecutive
+// cells, split into 4 groups of 4 cells.  For one group the volume term is
+// four 16x16 products on v_mfma_f64_16x16x4f64,
+//
+//     D[(cell, var)][node j] = sum_k  A[(cell, var)][k] * B[k][j],
+//
+// with A = F or G of the group's 4 cells (rows: 4 cells x 4 variables,
+// columns: the Np source nodes padded to a multiple of 4) and B = Drw^T or
+// Dsw^T (padded to 16 output nodes).  B is the same for every cell, so each
+// lane keeps its B entries in registers for the whole kernel; A is staged in
+// LDS already in the operand layout, so every MFMA reads one contiguous
+// 512-byte row.  In the accumulator layout lane l holds D[4*(l/16)+r][l%16],
+// i.e. cell l/16 of the group, node l%16, variables r = 0..3: the metric
+// terms are then lane-local, the LIFT product (A = Roe fluxes, B = -LIFT^T)
+// accumulates onto the same registers, and the epilogue stores each lane's
+// 4 variables of one node as one 32-byte record (the op_dat layout), so
+// stores are coalesced.
+//
+// The Euler fluxes (one lane per (cell, node)) and the Roe fluxes (one lane
+// per (cell, face node), neighbour traces gathered from the neighbour's
+// state record through the face map) run on the vector ALUs; 16 cells per
+// tile make the Roe phase exactly three full passes of the wave at N=3.
+#include <hip/hip_runtime.h>
+
+#include "dg_device.h"
+#include "dg_internal.h"
+
+namespace dg {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <int N, bool GATHER, int EPI>
+__global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const double *__restrict__ ops,
+                                                      const int *__restrict__ fmask, Phys ph,
+                                                      unsigned long long facenodes) {
+  constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
+  constexpr int KP = (Np + 3) / 4;  // K-steps over source nodes
+  constexpr int KF = (NF + 3) / 4;  // K-steps over face nodes
+  constexpr int TE = 16, G = 4;
+  static_assert(Np <= 16, "one 16-row output tile per cell group");
+  static_assert(KF <= 2 * KP, "face tile must fit in the volume tiles it reuses");
+  // [0]: F tiles, [1]: G tiles, each [G][KP][64]; the face-flux tiles
+  // [G][KF][64] reuse the same memory after the volume products.
+  __shared__ double sA[2 * G * KP * 64];
+  __shared__ int sFm[NF];
+  __shared__ unsigned long long sRed[1];
+
+  const int lane = threadIdx.x;
+  for (int i = lane; i < 2 * G * KP * 64; i += 64) sA[i] = 0.0;
+  if (lane < NF) sFm[lane] = fmask[lane];
+
+  // B operands: lane l holds B[k = 4*ks + l/16][j = l%16]
+  const int jj = lane & 15, kk = lane >> 4;
+  const double *Drw = ops, *Dsw = ops + Np * Np, *LIFT = ops + 2 * Np * Np;
+  double bR[KP], bS[KP], bL[KF];
+#pragma unroll
+  for (int ks = 0; ks < KP; ks++) {
+    const int k = 4 * ks + kk;
+    const bool ok = jj < Np && k < Np;
+    bR[ks] = ok ? Drw[jj * Np + k] : 0.0;
+    bS[ks] = ok ? Dsw[jj * Np + k] : 0.0;
+  }
+#pragma unroll
+  for (int ks = 0; ks < KF; ks++) {
+    const int m = 4 * ks + kk;
+    bL[ks] = (jj < Np && m < NF) ? -LIFT[jj * NF + m] : 0.0;
+  }
+
+  double dt = 0.0;
+  if (EPI != EPI_RHS) dt = a.cfl / __longlong_as_double((long long)*a.dt1_cur);
+  if (EPI == EPI_STAGE0 && blockIdx.x == 0 && lane == 0) *a.dt1_next = 0ull;
+  unsigned long long cfl_bits = 0ull;
+  bool bad = false;
+  const double n1sq = (double)((N + 1) * (N + 1));
+  double *sF = sA, *sG = sA + G * KP * 64, *sL = sA;
+
+  const int ntiles = (a.ncells + TE - 1) / TE;
+  for (int it_tile = 0;; it_tile++) {
+    const int tile = xcd_tile(it_tile, ntiles);
+    if (tile >= ntiles) break;
+    const int e0 = tile * TE;
+    const int ne = min(TE, a.ncells - e0);
+    __syncthreads();
+
+    // Euler fluxes per (cell, node), written in the A-operand layout
+    for (int it = lane; it < ne * Np; it += 64) {
+      const int e = it / Np, k = it - e * Np;
+      const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)(e0 + e) * Np + k) * 4);
+      const double2 q01 = src[0], q23 = src[1];
+      double qv[4] = {q01.x, q01.y, q23.x, q23.y}, F[4], Gv[4];
+      euler_flux(ph.gam, qv, F, Gv);
+      const int off = ((e >> 2) * KP + (k >> 2)) * 64 + 16 * (k & 3) + 4 * (e & 3);
+      double2 *dF = reinterpret_cast<double2 *>(sF + off);
+      double2 *dG = reinterpret_cast<double2 *>(sG + off);
+      dF[0] = make_double2(F[0], F[1]);
+      dF[1] = make_double2(F[2], F[3]);
+      dG[0] = make_double2(Gv[0], Gv[1]);
+      dG[1] = make_double2(Gv[2], Gv[3]);
+    }
+    __syncthreads();
+
+    // volume products and metric terms, one 4-cell group at a time
+    const int ce = lane >> 4;  // cell within the group owned by this lane's outputs
+    dbl4 acc[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      acc[g] = dbl4{0.0, 0.0, 0.0, 0.0};
+      if (4 * g < ne) {
+        dbl4 d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0}, d3 = {0, 0, 0, 0}, d4 = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KP; ks++) {
+          const double aF = sF[(g * KP + ks) * 64 + lane];
+          const double aG = sG[(g * KP + ks) * 64 + lane];
+          d1 = mfma16x16x4(aF, bR[ks], d1);
+          d2 = mfma16x16x4(aF, bS[ks], d2);
+          d3 = mfma16x16x4(aG, bR[ks], d3);
+          d4 = mfma16x16x4(aG, bS[ks], d4);
+        }
+        const int e = 4 * g + ce;
+        double rxv = 0.0, sxv = 0.0, ryv = 0.0, syv = 0.0;
+        if (e < ne && jj < Np) {
+          const size_t gn = (size_t)(e0 + e) * Np + jj;
+          rxv = a.rx[gn]; sxv = a.sx[gn]; ryv = a.ry[gn]; syv = a.sy[gn];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          acc[g][r] = (rxv * d1[r] + sxv * d2[r]) + (ryv * d3[r] + syv * d4[r]);
+      }
+    }
+    __syncthreads();  // the face fluxes overwrite the F tiles
+
+    // Roe fluxes per (cell, face node), written in the A-operand layout
+    for (int it = lane; it < ne * NF; it += 64) {
+      const int e = it / NF, m = it - e * NF;
+      const int f = m / Nfp, i = m - f * Nfp;
+      const size_t cell = (size_t)(e0 + e);
+      const size_t gm = cell * NF + m;
+      const double nxv = a.nx[gm], nyv = a.ny[gm], fs = a.fscale[gm];
+      const double2 *ms = reinterpret_cast<const double2 *>(a.w + (cell * Np + sFm[m]) * 4);
+      const double2 m01 = ms[0], m23 = ms[1];
+      double mQ[4] = {m01.x, m01.y, m23.x, m23.y}, pQ[4];
+      if (GATHER) {
+        const int code = a.fmap[cell * 3 + f];
+        if (code >= 0) {
+          const int nb = code >> 2, nf = code & 3;
+          const int node = sFm[nf * Nfp + (Nfp - 1 - i)];
+          const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)nb * Np + node) * 4);
+          const double2 p01 = src[0], p23 = src[1];
+          pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
+        } else {
+          boundary_state(ph, -code - 1, mQ, nxv, nyv, pQ);
+        }
+      } else {
+        double2 *src = reinterpret_cast<double2 *>(a.ext + gm * 4);
+        const double2 p01 = src[0], p23 = src[1];
+        pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
+        src[0] = make_double2(0.0, 0.0);  // OP_RW: euler_rhs zeroes exteriorQ
+        src[1] = make_double2(0.0, 0.0);
+      }
+      double fl[4];
+      roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
+      const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + 4 * (e & 3);
+      double2 *d = reinterpret_cast<double2 *>(sL + off);
+      d[0] = make_double2(fl[0], fl[1]);
+      d[1] = make_double2(fl[2], fl[3]);
+    }
+    __syncthreads();
+
+    // LIFT products onto the volume terms, then the epilogue
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (4 * g >= ne) continue;
+      dbl4 c = acc[g];
+#pragma unroll
+      for (int ks = 0; ks < KF; ks++) c = mfma16x16x4(sL[(g * KF + ks) * 64 + lane], bL[ks], c);
+      const int e = 4 * g + ce;
+      if (e >= ne || jj >= Np) continue;
+      const size_t gn = (size_t)(e0 + e) * Np + jj;
+      double o[4];
+      if (EPI == EPI_RHS) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) o[v] = c[v];
+      } else {
+        const double2 *ws = reinterpret_cast<const double2 *>(a.w + gn * 4);
+        const double2 w01 = ws[0], w23 = ws[1];
+        const double wv[4] = {w01.x, w01.y, w23.x, w23.y};
+        if (EPI == EPI_STAGE0) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) o[v] = wv[v] + dt * c[v];
+        } else {
+          const double2 *qs = reinterpret_cast<const double2 *>(a.q + gn * 4);
+          const double2 q01 = qs[0], q23 = qs[1];
+          const double qv[4] = {q01.x, q01.y, q23.x, q23.y};
+          if (EPI == EPI_STAGE1) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) o[v] = 0.75 * qv[v] + 0.25 * (wv[v] + dt * c[v]);
+          } else {
+#pragma unroll
+            for (int v = 0; v < 4; v++) o[v] = (qv[v] + 2.0 * (wv[v] + dt * c[v])) / 3.0;
+          }
+        }
+      }
+      double2 *dst = reinterpret_cast<double2 *>(a.out + gn * 4);
+      dst[0] = make_double2(o[0], o[1]);
+      dst[1] = make_double2(o[2], o[3]);
+      if (EPI == EPI_STAGE2) {
+        if ((facenodes >> jj) & 1ull) {
+          const double cv = cfl_node(ph.gam, n1sq, o, a.fscale[(size_t)(e0 + e) * NF + jj]) + 0.0;
+          cfl_bits = max_bits(cfl_bits, (unsigned long long)__double_as_longlong(cv));
+        }
+        bad |= !(isfinite(o[0]) && isfinite(o[1]) && isfinite(o[2]) && isfinite(o[3])) ||
+               !(o[0] > 0.0);
+      }
+    }
+  }
+  if (EPI == EPI_STAGE2) {
+    const unsigned long long r = block_max_bits<64>(cfl_bits, sRed);
+    if (lane == 0) atomicMax(a.dt1_next, r);
+    if (bad) atomicOr(a.flag, 1u);
+    if (blockIdx.x == 0 && lane == 0) *a.t_dev += dt;
+  }
+}
+
+template <int N, bool GATHER, int EPI>
+static hipError_t launch_t(const Launch &L, const StageArgs &a) {
+  static int per_cu = -1;
+  if (per_cu < 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rhs_mfma_kernel<N, GATHER, EPI>, 64, 0) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 8;
+    per_cu = nb;
+  }
+  const int ntiles = (a.ncells + 15) / 16;
+  int grid = L.cus > 0 ? L.cus * per_cu : 2048;
+  if (L.max_blocks > 0 && grid > L.max_blocks) grid = L.max_blocks;
+  if (grid > ntiles) grid = ntiles;
+  if (grid >= 8) grid -= grid % 8;
+  if (grid < 1) return hipSuccess;
+  hipLaunchKernelGGL((rhs_mfma_kernel<N, GATHER, EPI>), dim3(grid), dim3(64), 0, L.stream, a, L.ops,
+                     L.fmask, L.ph, (unsigned long long)L.facenodes);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_n(const Launch &L, const StageArgs &a, bool gather, int epi) {
+  if (!gather) return launch_t<N, false, EPI_RHS>(L, a);
+  switch (epi) {
+    case EPI_RHS: return launch_t<N, true, EPI_RHS>(L, a);
+    case EPI_STAGE0: return launch_t<N, true, EPI_STAGE0>(L, a);
+    case EPI_STAGE1: return launch_t<N, true, EPI_STAGE1>(L, a);
+    case EPI_STAGE2: return launch_t<N, true, EPI_STAGE2>(L, a);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_rhs_mfma(const Launch &L, const StageArgs &a, bool gather, int epi) {
+  if (!gather && epi != EPI_RHS) return hipErrorInvalidValue;
+  switch (L.N) {
+    case 1: return launch_n<1>(L, a, gather, epi);
+    case 2: return launch_n<2>(L, a, gather, epi);
+    case 3: return launch_n<3>(L, a, gather, epi);
+    case 4: return launch_n<4>(L, a, gather, epi);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace dg

@@ -151,6 +151,10 @@ class Context:
         ptr = getattr(stream, "cuda_stream", stream)
         self._chk(lib.dg_set_stream(self._h, c_void_p(int(ptr) if ptr else 0)))
 
+    def set_variant(self, variant, max_blocks=0):
+        """Stage kernel: 0 matrix-core (default), 1 vector-ALU."""
+        self._chk(lib.dg_set_variant(self._h, int(variant), int(max_blocks)))
+
     def _dim(self, name):
         if name in self._DBL1:
             return 1

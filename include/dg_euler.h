@@ -71,6 +71,13 @@ void dg_ctx_destroy(dg_ctx *ctx);
  * failed call that had no context (creation, mesh generation). */
 const char *dg_last_error(const dg_ctx *ctx);
 int dg_order(const dg_ctx *ctx, int *N, int *Np, int *Nfp);
+/* Stage-kernel variant (no reference counterpart): 0 = matrix-core kernel
+ * (one wave per 16-cell tile, v_mfma_f64_16x16x4f64 for the volume and LIFT
+ * products; default), 1 = vector-ALU kernel (workgroup tiles of 32 cells,
+ * contractions from LDS).  max_blocks > 0 caps the persistent grid (0 = one
+ * full wave of resident workgroups).  Both give the same results to
+ * rounding.  The environment variable DG_STAGE_VARIANT sets the default. */
+int dg_set_variant(dg_ctx *ctx, int variant, int max_blocks);
 /* Stream for all later work of ctx (a hipStream_t; NULL = null stream). */
 int dg_set_stream(dg_ctx *ctx, void *hip_stream);
 

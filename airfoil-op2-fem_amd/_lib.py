@@ -65,9 +65,19 @@ _SIGS = {
     "dg_solver_get_q": ([c_void_p, c_void_p], c_int),
     "dg_solver_dat": ([c_void_p, c_char_p, P(c_void_p)], c_int),
     "dg_solver_run": ([c_void_p, c_int, c_double, c_int], c_int),
+    "dg_solver_run_graph": ([c_void_p, c_int, c_double], c_int),
+    "dg_solver_stream": ([c_void_p, P(c_void_p)], c_int),
+    "dg_solver_create_batch": ([c_void_p, c_int, c_void_p, c_void_p, c_void_p, P(c_void_p)], c_int),
+    "dg_solver_ncases": ([c_void_p, P(c_int)], c_int),
+    "dg_solver_case_set_q": ([c_void_p, c_int, c_void_p], c_int),
+    "dg_solver_case_get_q": ([c_void_p, c_int, c_void_p], c_int),
+    "dg_solver_case_time": ([c_void_p, c_int, P(c_double), P(c_double)], c_int),
+    "dg_solver_case_forces": ([c_void_p, c_int, c_double, P(c_double), P(c_double), c_void_p], c_int),
     "dg_solver_time": ([c_void_p, P(c_double), P(c_double)], c_int),
     "dg_solver_reset_time": ([c_void_p], c_int),
     "dg_solver_check": ([c_void_p], c_int),
+    "dg_solver_forces": ([c_void_p, c_double, P(c_double), P(c_double), c_void_p], c_int),
+    "dg_solver_wall_cp": ([c_void_p, P(c_int), c_void_p, c_void_p], c_int),
     "dg_solver_profile": ([c_void_p, c_int], c_int),
     "dg_solver_kernel_time": ([c_void_p, c_char_p, P(c_double), P(c_int)], c_int),
 }

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/dg_euler.h"
+#include "dg_ctx.h"
 #include "dg_internal.h"
 #include "dg_mesh.h"
 #include "dg_ops.h"
@@ -17,32 +18,7 @@ namespace {
 thread_local std::string g_err;
 }
 
-struct dg_ctx {
-  int N = 0, Np = 0, Nfp = 0, NF = 0;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  // host copies of the constants (op_decl_const names)
-  std::vector<double> r, s, Dr, Ds, Drw, Dsw, LIFT, ones;
-  std::vector<int> FMASK;
-  double gam = 1.4, bc_mach = 0.4, bc_alpha = 0.0, bc_p = 1.0, bc_r = 1.0, bc_u = 0.0,
-         bc_v = 0.0, bc_e = 0.0;
-  double *d_ops = nullptr;
-  int *d_fmask = nullptr;
-  int cus = 0;
-  int max_blocks = 0;
-  int variant = 0;
-  // declared mesh
-  int ncells = -1, nedges = 0, nbedges = 0;
-  int *d_fmap = nullptr, *d_b2c = nullptr, *d_bnum = nullptr, *d_btype = nullptr;
-  unsigned long long *d_red = nullptr;
-};
-
-struct dg_mesh {
-  dg::Mesh m;
-};
-
-namespace {
+namespace dg {
 
 int fail(dg_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
@@ -54,14 +30,8 @@ int hipfail(dg_ctx *c, hipError_t e, const char *what) {
   return fail(c, DG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIPCHECK(ctx, call)                                   \
-  do {                                                        \
-    hipError_t e_ = (call);                                   \
-    if (e_ != hipSuccess) return hipfail((ctx), e_, #call);   \
-  } while (0)
-
-dg::Launch launch_of(const dg_ctx *c) {
-  dg::Launch L;
+Launch launch_of(const dg_ctx *c) {
+  Launch L;
   L.N = c->N; L.Np = c->Np; L.Nfp = c->Nfp; L.NF = c->NF;
   L.stream = c->stream;
   L.ops = c->d_ops;
@@ -75,6 +45,29 @@ dg::Launch launch_of(const dg_ctx *c) {
   L.variant = c->variant;
   return L;
 }
+
+Phys freestream(double gam, double mach, double alpha_deg, double p, double rho) {
+  Phys ph;
+  ph.gam = gam; ph.bc_p = p; ph.bc_r = rho;
+  const double a = std::sqrt(gam * p / rho) * mach;
+  if (alpha_deg != 0.0) {
+    ph.bc_u = std::sin((M_PI / 2.0) - (alpha_deg * M_PI / 180.0)) * a;
+    ph.bc_v = std::cos((M_PI / 2.0) - (alpha_deg * M_PI / 180.0)) * a;
+    ph.bc_e = p / (gam - 1.0) + 0.5 * rho * (ph.bc_u * ph.bc_u + ph.bc_v * ph.bc_v);
+  } else {
+    ph.bc_u = a;
+    ph.bc_v = 0.0;
+    ph.bc_e = p / (gam - 1.0) + 0.5 * rho * (ph.bc_u * ph.bc_u);
+  }
+  return ph;
+}
+
+}  // namespace dg
+
+namespace {
+using dg::fail;
+using dg::hipfail;
+using dg::launch_of;
 
 int upload_ops(dg_ctx *c) {
   const int Np = c->Np, NF = c->NF;
@@ -268,17 +261,9 @@ int dg_set_freestream(dg_ctx *c, double gam, double mach, double alpha_deg, doub
   if (!c) return fail(nullptr, DG_ERR_ARG, "dg_set_freestream: null context");
   if (!(gam > 1.0) || !(p > 0.0) || !(rho > 0.0) || !(mach >= 0.0))
     return fail(c, DG_ERR_ARG, "dg_set_freestream: need gam > 1, p > 0, rho > 0, mach >= 0");
+  const dg::Phys ph = dg::freestream(gam, mach, alpha_deg, p, rho);
   c->gam = gam; c->bc_mach = mach; c->bc_alpha = alpha_deg; c->bc_p = p; c->bc_r = rho;
-  const double a = std::sqrt(gam * p / rho) * mach;
-  if (alpha_deg != 0.0) {
-    c->bc_u = std::sin((M_PI / 2.0) - (alpha_deg * M_PI / 180.0)) * a;
-    c->bc_v = std::cos((M_PI / 2.0) - (alpha_deg * M_PI / 180.0)) * a;
-    c->bc_e = p / (gam - 1.0) + 0.5 * rho * (c->bc_u * c->bc_u + c->bc_v * c->bc_v);
-  } else {
-    c->bc_u = a;
-    c->bc_v = 0.0;
-    c->bc_e = p / (gam - 1.0) + 0.5 * rho * (c->bc_u * c->bc_u);
-  }
+  c->bc_u = ph.bc_u; c->bc_v = ph.bc_v; c->bc_e = ph.bc_e;
   return DG_OK;
 }
 
@@ -508,316 +493,3 @@ int dg_mesh_get(const dg_mesh *m, const char *field, void *out) {
 
 }  // extern "C"
 
-/* ---------------------------- solver ---------------------------- */
-
-struct dg_solver {
-  dg_ctx *ctx = nullptr;
-  int K = 0;
-  std::map<std::string, double *> dats;
-  double *q = nullptr, *w = nullptr, *rk[3] = {nullptr, nullptr, nullptr}, *ext = nullptr;
-  unsigned long long *dt1 = nullptr;  // two ping-pong slots
-  double *t_dev = nullptr;
-  unsigned int *flag = nullptr;
-  long long step = 0;
-  bool dt_valid = false;
-  double cfl_last = 1.0;
-  bool profiling = false;
-  struct Rec { std::string name; hipEvent_t a, b; };
-  std::vector<Rec> pending;
-  std::vector<hipEvent_t> pool;
-  std::map<std::string, std::pair<double, int>> ktimes;
-};
-
-namespace {
-
-hipEvent_t take_event(dg_solver *s) {
-  if (!s->pool.empty()) {
-    hipEvent_t e = s->pool.back();
-    s->pool.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  (void)hipEventCreate(&e);
-  return e;
-}
-
-template <class F>
-hipError_t timed(dg_solver *s, const char *name, F &&f) {
-  if (!s->profiling) return f();
-  dg_solver::Rec r{name, take_event(s), take_event(s)};
-  (void)hipEventRecord(r.a, s->ctx->stream);
-  hipError_t e = f();
-  (void)hipEventRecord(r.b, s->ctx->stream);
-  s->pending.push_back(r);
-  return e;
-}
-
-void drain_profile(dg_solver *s) {
-  if (s->pending.empty()) return;
-  (void)hipStreamSynchronize(s->ctx->stream);
-  for (auto &r : s->pending) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
-      auto &t = s->ktimes[r.name];
-      t.first += ms;
-      t.second += 1;
-    }
-    s->pool.push_back(r.a);
-    s->pool.push_back(r.b);
-  }
-  s->pending.clear();
-}
-
-double *dat(dg_solver *s, const char *n) { return s->dats[n]; }
-
-}  // namespace
-
-extern "C" {
-
-void dg_solver_destroy(dg_solver *s) {
-  if (!s) return;
-  if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
-  for (auto &kv : s->dats) (void)hipFree(kv.second);
-  (void)hipFree(s->dt1); (void)hipFree(s->t_dev); (void)hipFree(s->flag);
-  for (auto &r : s->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
-  for (auto e : s->pool) (void)hipEventDestroy(e);
-  delete s;
-}
-
-int dg_solver_create(dg_ctx *c, const dg_mesh *mesh, dg_solver **out) {
-  if (!c || !mesh || !out) return fail(c, DG_ERR_ARG, "dg_solver_create: null argument");
-  *out = nullptr;
-  const dg::Mesh &M = mesh->m;
-  int rc = dg_decl_mesh(c, M.ncells, M.cells.data(), M.nedges, M.edge2cells.data(),
-                        M.edgeNum.data(), M.nbedges, M.bedge2cells.data(), M.bedgeNum.data(),
-                        M.bedge_type.data());
-  if (rc != DG_OK) return rc;
-  dg_solver *s = new dg_solver();
-  s->ctx = c;
-  s->K = M.ncells;
-  const size_t K = M.ncells, Np = c->Np, NF = c->NF;
-  struct { const char *n; size_t sz; } layout[] = {
-      {"q", K * Np * 4},   {"workingQ", K * Np * 4}, {"rk1", K * Np * 4}, {"rk2", K * Np * 4},
-      {"rk3", K * Np * 4}, {"exteriorQ", K * NF * 4}, {"x", K * Np},      {"y", K * Np},
-      {"xr", K * Np},      {"yr", K * Np},           {"xs", K * Np},      {"ys", K * Np},
-      {"rx", K * Np},      {"ry", K * Np},           {"sx", K * Np},      {"sy", K * Np},
-      {"nx", K * NF},      {"ny", K * NF},           {"fscale", K * NF},  {"nodeX", K * 3},
-      {"nodeY", K * 3},    {"J", K},                 {"cell_xy", K * 6},
-  };
-  for (auto &l : layout) {
-    double *p = nullptr;
-    hipError_t e = hipMalloc(&p, sizeof(double) * (l.sz ? l.sz : 1));
-    if (e != hipSuccess) {
-      dg_solver_destroy(s);
-      return hipfail(c, e, "dg_solver_create: hipMalloc");
-    }
-    s->dats[l.n] = p;
-  }
-  hipError_t e = hipMalloc(&s->dt1, 2 * sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipMalloc(&s->t_dev, sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&s->flag, sizeof(unsigned int));
-  if (e == hipSuccess) e = hipMemset(s->dt1, 0, 2 * sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipMemset(s->t_dev, 0, sizeof(double));
-  if (e == hipSuccess) e = hipMemset(s->flag, 0, sizeof(unsigned int));
-  if (e == hipSuccess && K > 0)
-    e = hipMemcpy(dat(s, "cell_xy"), M.cell_xy.data(), sizeof(double) * K * 6, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    dg_solver_destroy(s);
-    return hipfail(c, e, "dg_solver_create");
-  }
-  s->q = dat(s, "q"); s->w = dat(s, "workingQ"); s->ext = dat(s, "exteriorQ");
-  s->rk[0] = dat(s, "rk1"); s->rk[1] = dat(s, "rk2"); s->rk[2] = dat(s, "rk3");
-  dg::Launch L = launch_of(c);
-  e = dg::launch_init_grid(L, s->K, dat(s, "cell_xy"), dat(s, "nodeX"), dat(s, "nodeY"),
-                           dat(s, "x"), dat(s, "y"), dat(s, "xr"), dat(s, "yr"), dat(s, "xs"),
-                           dat(s, "ys"), dat(s, "rx"), dat(s, "ry"), dat(s, "sx"), dat(s, "sy"),
-                           dat(s, "nx"), dat(s, "ny"), dat(s, "fscale"), dat(s, "J"));
-  if (e == hipSuccess) e = dg::launch_set_ic(L, s->K, s->q, s->w);
-  if (e == hipSuccess) e = dg::launch_zero(L, K * NF * 4, s->ext);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) {
-    dg_solver_destroy(s);
-    return hipfail(c, e, "dg_solver_create: set-up kernels");
-  }
-  *out = s;
-  return DG_OK;
-}
-
-int dg_solver_set_vortex(dg_solver *s, double uinf, double beta, double x0, double y0,
-                         double period_x) {
-  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_set_vortex: null solver");
-  HIPCHECK(s->ctx, dg::launch_set_vortex(launch_of(s->ctx), s->K, dat(s, "x"), dat(s, "y"), uinf,
-                                         beta, x0, y0, period_x, s->q, s->w));
-  s->dt_valid = false;
-  return DG_OK;
-}
-
-int dg_solver_set_q(dg_solver *s, const double *host_q) {
-  if (!s || !host_q) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_set_q: null argument");
-  dg_ctx *c = s->ctx;
-  const size_t n = (size_t)s->K * c->Np * 4;
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  HIPCHECK(c, hipMemcpy(s->q, host_q, sizeof(double) * n, hipMemcpyHostToDevice));
-  HIPCHECK(c, hipMemcpy(s->w, host_q, sizeof(double) * n, hipMemcpyHostToDevice));
-  s->dt_valid = false;
-  return DG_OK;
-}
-
-int dg_solver_get_q(dg_solver *s, double *host_q) {
-  if (!s || !host_q) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_get_q: null argument");
-  dg_ctx *c = s->ctx;
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  HIPCHECK(c, hipMemcpy(host_q, s->q, sizeof(double) * (size_t)s->K * c->Np * 4, hipMemcpyDeviceToHost));
-  return DG_OK;
-}
-
-int dg_solver_dat(dg_solver *s, const char *name, void **p) {
-  if (!s || !name || !p) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_dat: null argument");
-  auto it = s->dats.find(name);
-  if (it == s->dats.end())
-    return fail(s->ctx, DG_ERR_NAME, std::string("dg_solver_dat: unknown op_dat '") + name + "'");
-  *p = it->second;
-  return DG_OK;
-}
-
-int dg_solver_reset_time(dg_solver *s) {
-  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_reset_time: null solver");
-  HIPCHECK(s->ctx, hipStreamSynchronize(s->ctx->stream));
-  HIPCHECK(s->ctx, hipMemset(s->t_dev, 0, sizeof(double)));
-  HIPCHECK(s->ctx, hipMemset(s->flag, 0, sizeof(unsigned int)));
-  s->dt_valid = false;
-  s->step = 0;
-  return DG_OK;
-}
-
-int dg_solver_run(dg_solver *s, int nsteps, double cfl, int mode) {
-  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_run: null solver");
-  dg_ctx *c = s->ctx;
-  if (nsteps < 0 || !(cfl > 0.0)) return fail(c, DG_ERR_ARG, "dg_solver_run: need nsteps >= 0, cfl > 0");
-  if (mode != DG_MODE_FUSED && mode != DG_MODE_REFERENCE)
-    return fail(c, DG_ERR_ARG, "dg_solver_run: unknown mode");
-  const dg::Launch L = launch_of(c);
-  const int K = s->K;
-  const size_t nstate = (size_t)K * c->Np * 4;
-  double *fscale = dat(s, "fscale");
-  s->cfl_last = cfl;
-  unsigned long long *slot[2] = {s->dt1, s->dt1 + 1};
-  if (!s->dt_valid) {
-    const int p = (int)(s->step & 1);
-    HIPCHECK(c, hipMemsetAsync(slot[p], 0, sizeof(unsigned long long), c->stream));
-    HIPCHECK(c, timed(s, "calc_dt", [&] { return dg::launch_calc_dt(L, K, s->q, fscale, slot[p]); }));
-    s->dt_valid = true;
-  }
-  if (mode == DG_MODE_FUSED) {
-    dg::StageArgs a{};
-    a.rx = dat(s, "rx"); a.ry = dat(s, "ry"); a.sx = dat(s, "sx"); a.sy = dat(s, "sy");
-    a.fscale = fscale; a.nx = dat(s, "nx"); a.ny = dat(s, "ny");
-    a.fmap = c->d_fmap; a.q = s->q; a.cfl = cfl; a.t_dev = s->t_dev; a.flag = s->flag;
-    a.ncells = K;
-    double *wA = s->w, *wB = s->rk[0];
-    for (int n = 0; n < nsteps; n++) {
-      const int p = (int)(s->step & 1);
-      a.dt1_cur = slot[p];
-      a.dt1_next = slot[p ^ 1];
-      a.w = s->q; a.out = wA;
-      HIPCHECK(c, timed(s, "rhs_stage0", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE0); }));
-      a.w = wA; a.out = wB;
-      HIPCHECK(c, timed(s, "rhs_stage1", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE1); }));
-      a.w = wB; a.out = s->q;
-      HIPCHECK(c, timed(s, "rhs_stage2", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE2); }));
-      s->step++;
-    }
-    // keep the reference invariant workingQ == q after a step
-    if (nsteps > 0) HIPCHECK(c, dg::launch_copy(L, nstate, s->q, s->w));
-    return DG_OK;
-  }
-  // DG_MODE_REFERENCE: the nine loops of src/airfoil.cpp:252-334, dt on the host
-  double t = 0.0, dt1 = 0.0;
-  unsigned long long bits = 0;
-  HIPCHECK(c, hipMemcpyAsync(&t, s->t_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHECK(c, hipMemcpyAsync(&bits, slot[s->step & 1], sizeof(bits), hipMemcpyDeviceToHost, c->stream));
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  std::memcpy(&dt1, &bits, sizeof(dt1));
-  double dt = cfl / dt1;
-  for (int n = 0; n < nsteps; n++) {
-    for (int j = 0; j < 3; j++) {
-      HIPCHECK(c, timed(s, "get_neighbour_q", [&] { return dg::launch_get_neighbour_q(L, K, c->d_fmap, s->w, s->ext); }));
-      HIPCHECK(c, timed(s, "get_bedge_q", [&] {
-        return dg::launch_get_bedge_q(L, c->nbedges, c->d_b2c, c->d_bnum, c->d_btype, dat(s, "nx"),
-                                      dat(s, "ny"), s->w, s->ext);
-      }));
-      dg::StageArgs a{};
-      a.w = s->w; a.ext = s->ext; a.rx = dat(s, "rx"); a.ry = dat(s, "ry"); a.sx = dat(s, "sx");
-      a.sy = dat(s, "sy"); a.fscale = fscale; a.nx = dat(s, "nx"); a.ny = dat(s, "ny");
-      a.out = s->rk[j]; a.ncells = K;
-      HIPCHECK(c, timed(s, "euler_rhs", [&] { return dg::launch_rhs(L, a, false, dg::EPI_RHS); }));
-      if (j != 2)
-        HIPCHECK(c, timed(s, "set_workingQ", [&] {
-          return dg::launch_set_workingQ(L, K, dt, j, s->q, s->rk[0], s->rk[1], s->w);
-        }));
-    }
-    HIPCHECK(c, timed(s, "update_Q", [&] {
-      return dg::launch_update_Q(L, K, dt, s->q, s->rk[0], s->rk[1], s->rk[2], s->w);
-    }));
-    t += dt;
-    s->step++;
-    unsigned long long *nxt = slot[s->step & 1];
-    HIPCHECK(c, hipMemsetAsync(nxt, 0, sizeof(unsigned long long), c->stream));
-    HIPCHECK(c, timed(s, "calc_dt", [&] { return dg::launch_calc_dt(L, K, s->q, fscale, nxt); }));
-    HIPCHECK(c, hipMemcpyAsync(&bits, nxt, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    std::memcpy(&dt1, &bits, sizeof(dt1));
-    dt = cfl / dt1;
-  }
-  HIPCHECK(c, hipMemcpyAsync(s->t_dev, &t, sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  return DG_OK;
-}
-
-int dg_solver_time(dg_solver *s, double *t, double *dt) {
-  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_time: null solver");
-  dg_ctx *c = s->ctx;
-  double tv = 0.0;
-  unsigned long long bits = 0;
-  HIPCHECK(c, hipMemcpyAsync(&tv, s->t_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHECK(c, hipMemcpyAsync(&bits, s->dt1 + (s->step & 1), sizeof(bits), hipMemcpyDeviceToHost, c->stream));
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  double dt1;
-  std::memcpy(&dt1, &bits, sizeof(dt1));
-  if (t) *t = tv;
-  if (dt) *dt = s->dt_valid ? s->cfl_last / dt1 : 0.0;
-  return DG_OK;
-}
-
-int dg_solver_check(dg_solver *s) {
-  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_check: null solver");
-  unsigned int f = 0;
-  HIPCHECK(s->ctx, hipMemcpyAsync(&f, s->flag, sizeof(f), hipMemcpyDeviceToHost, s->ctx->stream));
-  HIPCHECK(s->ctx, hipStreamSynchronize(s->ctx->stream));
-  if (f) return fail(s->ctx, DG_ERR_NONFINITE, "non-finite or non-positive-density state detected");
-  return DG_OK;
-}
-
-int dg_solver_profile(dg_solver *s, int enable) {
-  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_profile: null solver");
-  drain_profile(s);
-  s->profiling = enable != 0;
-  s->ktimes.clear();
-  return DG_OK;
-}
-
-int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int *count) {
-  if (!s || !kernel) return fail(nullptr, DG_ERR_ARG, "dg_solver_kernel_time: null argument");
-  drain_profile(s);
-  auto it = s->ktimes.find(kernel);
-  if (it == s->ktimes.end() || it->second.second == 0) {
-    if (avg_ms) *avg_ms = 0.0;
-    if (count) *count = 0;
-    return DG_OK;
-  }
-  if (avg_ms) *avg_ms = it->second.first / it->second.second;
-  if (count) *count = it->second.second;
-  return DG_OK;
-}
-
-}  // extern "C"

@@ -37,13 +37,24 @@ struct StageArgs {
   const int *fmap;        // [K][3] face map
   double *out;            // qRHS, or the next state
   const double *q;        // SSPRK3 base state (stages 1, 2)
-  const unsigned long long *dt1_cur;  // device dt1 (bits of a non-negative double)
-  unsigned long long *dt1_next;       // reduction target (stage 2), zeroed in stage 0
+  const unsigned long long *dt1_cur;  // device dt1 per case (bits of a non-negative double)
+  unsigned long long *dt1_next;       // reduction target per case (stage 2), zeroed in stage 0
   double cfl;
-  double *t_dev;          // simulated time, += dt in stage 2
+  double *t_dev;          // simulated time per case, += dt in stage 2
   unsigned int *flag;     // bit 0: non-finite value seen
   int ncells;
+  // Case table of the fused solver (ncases >= 1): case c owns cells
+  // [cell_off[c], cell_off[c+1]) and tiles [tile_off[c], tile_off[c+1]) of the
+  // concatenated arrays, with its own physics constants and dt.  ncases == 0
+  // (the per-loop entry points): one case of ncells cells, physics from the
+  // launch, no dt.
+  int ncases;
+  const int *cell_off, *tile_off;
+  const Phys *phys;
 };
+
+// Cells per tile of the matrix-core stage kernel (one wave per tile).
+constexpr int kMfmaTile = 16;
 
 // Epilogue kinds of the stage kernel.
 enum Epi { EPI_RHS = 0, EPI_STAGE0 = 1, EPI_STAGE1 = 2, EPI_STAGE2 = 3 };
@@ -74,5 +85,12 @@ hipError_t launch_set_vortex(const Launch &L, int ncells, const double *x,
                              double y0, double period_x, double *q, double *w);
 hipError_t launch_zero(const Launch &L, size_t n, double *p);
 hipError_t launch_copy(const Launch &L, size_t n, const double *src, double *dst);
+// Pressure forces on the listed wall faces: out[2*b] = (fx, fy) of face b;
+// cp[b*Nfp + i] = (p - p_inf) / qdyn at its nodes.
+hipError_t launch_wall_forces(const Launch &L, int nwall, const int *wall_cell,
+                              const int *wall_face, const double *q, const double *nx,
+                              const double *ny, const double *fscale, const double *J,
+                              const double *facew, double pinf, double qdyn, double *out,
+                              double *cp);
 
 }  // namespace dg

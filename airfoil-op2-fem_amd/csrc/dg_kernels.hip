@@ -445,6 +445,47 @@ __global__ void copy_kernel(size_t n, const double *__restrict__ s, double *__re
     d[i] = s[i];
 }
 
+// Surface pressure force of each wall face: integral of p n over the face
+// with the 1D face mass matrix as quadrature (weights facew[i] = row sums,
+// face parameter length 2, so sJ = fscale * J is half the face length).
+__global__ void wall_forces_kernel(int nwall, int Np, int Nfp, const int *__restrict__ fmask,
+                                   double gam, const int *__restrict__ wcell,
+                                   const int *__restrict__ wface, const double *__restrict__ q,
+                                   const double *__restrict__ nx, const double *__restrict__ ny,
+                                   const double *__restrict__ fscale, const double *__restrict__ J,
+                                   const double *__restrict__ facew, double pinf, double qdyn,
+                                   double *out, double *cp) {
+  const int NF = 3 * Nfp;
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nwall; b += gridDim.x * blockDim.x) {
+    const size_t c = wcell[b];
+    const int f = wface[b];
+    double fx = 0.0, fy = 0.0;
+    for (int i = 0; i < Nfp; i++) {
+      const int m = f * Nfp + i;
+      const double *qq = q + (c * Np + fmask[m]) * 4;
+      const double rho = qq[0], u = qq[1] / rho, v = qq[2] / rho;
+      const double p = (gam - 1.0) * (qq[3] - 0.5 * rho * (u * u + v * v));
+      const double sJ = fscale[c * NF + m] * J[c];
+      fx += facew[i] * sJ * p * nx[c * NF + m];
+      fy += facew[i] * sJ * p * ny[c * NF + m];
+      cp[(size_t)b * Nfp + i] = (p - pinf) / qdyn;
+    }
+    out[2 * (size_t)b] = fx;
+    out[2 * (size_t)b + 1] = fy;
+  }
+}
+
+hipError_t launch_wall_forces(const Launch &L, int nwall, const int *wall_cell, const int *wall_face,
+                              const double *q, const double *nx, const double *ny,
+                              const double *fscale, const double *J, const double *facew,
+                              double pinf, double qdyn, double *out, double *cp) {
+  if (nwall == 0) return hipSuccess;
+  hipLaunchKernelGGL(wall_forces_kernel, dim3((nwall + 127) / 128), dim3(128), 0, L.stream, nwall,
+                     L.Np, L.Nfp, L.fmask, L.ph.gam, wall_cell, wall_face, q, nx, ny, fscale, J,
+                     facew, pinf, qdyn, out, cp);
+  return hipGetLastError();
+}
+
 static int grid_for(size_t n, int bs, int cap) {
   size_t g = (n + bs - 1) / bs;
   if (g > (size_t)cap) g = cap;

@@ -37,19 +37,18 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
 
 template <int N, bool GATHER, int EPI>
 __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const double *__restrict__ ops,
-                                                      const int *__restrict__ fmask, Phys ph,
+                                                      const int *__restrict__ fmask, Phys ph0,
                                                       unsigned long long facenodes) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   constexpr int KP = (Np + 3) / 4;  // K-steps over source nodes
   constexpr int KF = (NF + 3) / 4;  // K-steps over face nodes
-  constexpr int TE = 16, G = 4;
+  constexpr int TE = kMfmaTile, G = 4;
   static_assert(Np <= 16, "one 16-row output tile per cell group");
   static_assert(KF <= 2 * KP, "face tile must fit in the volume tiles it reuses");
   // [0]: F tiles, [1]: G tiles, each [G][KP][64]; the face-flux tiles
   // [G][KF][64] reuse the same memory after the volume products.
   __shared__ double sA[2 * G * KP * 64];
   __shared__ int sFm[NF];
-  __shared__ unsigned long long sRed[1];
 
   const int lane = threadIdx.x;
   for (int i = lane; i < 2 * G * KP * 64; i += 64) sA[i] = 0.0;
@@ -72,20 +71,50 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
     bL[ks] = (jj < Np && m < NF) ? -LIFT[jj * NF + m] : 0.0;
   }
 
-  double dt = 0.0;
-  if (EPI != EPI_RHS) dt = a.cfl / __longlong_as_double((long long)*a.dt1_cur);
-  if (EPI == EPI_STAGE0 && blockIdx.x == 0 && lane == 0) *a.dt1_next = 0ull;
+  const bool multi = a.ncases > 0;
+  if (EPI == EPI_STAGE0 && blockIdx.x == 0)
+    for (int c = lane; c < a.ncases; c += 64) a.dt1_next[c] = 0ull;
+  if (EPI == EPI_STAGE2 && blockIdx.x == 0)
+    for (int c = lane; c < a.ncases; c += 64)
+      a.t_dev[c] += a.cfl / __longlong_as_double((long long)a.dt1_cur[c]);
   unsigned long long cfl_bits = 0ull;
+  int cfl_case = -1;
   bool bad = false;
   const double n1sq = (double)((N + 1) * (N + 1));
   double *sF = sA, *sG = sA + G * KP * 64, *sL = sA;
 
-  const int ntiles = (a.ncells + TE - 1) / TE;
+  const int ntiles = multi ? a.tile_off[a.ncases] : (a.ncells + TE - 1) / TE;
   for (int it_tile = 0;; it_tile++) {
     const int tile = xcd_tile(it_tile, ntiles);
     if (tile >= ntiles) break;
-    const int e0 = tile * TE;
-    const int ne = min(TE, a.ncells - e0);
+    int e0, ne, cs = 0;
+    Phys ph = ph0;
+    double dt = 0.0;
+    if (multi) {
+      int lo = 0, hi = a.ncases - 1;  // case owning this tile
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
+      }
+      cs = lo;
+      e0 = a.cell_off[cs] + (tile - a.tile_off[cs]) * TE;
+      ne = min(TE, a.cell_off[cs + 1] - e0);
+      ph = a.phys[cs];
+      if (EPI != EPI_RHS) dt = a.cfl / __longlong_as_double((long long)a.dt1_cur[cs]);
+    } else {
+      e0 = tile * TE;
+      ne = min(TE, a.ncells - e0);
+    }
+    if (EPI == EPI_STAGE2 && cs != cfl_case) {
+      // flush the running CFL max of the previous case (wave-uniform branch)
+      if (cfl_case >= 0) {
+        unsigned long long r = cfl_bits;
+        for (int off = 32; off > 0; off >>= 1) r = max_bits(r, __shfl_xor(r, off, 64));
+        if (lane == 0) atomicMax(a.dt1_next + cfl_case, r);
+      }
+      cfl_case = cs;
+      cfl_bits = 0ull;
+    }
     __syncthreads();
 
     // Euler fluxes per (cell, node), written in the A-operand layout
@@ -220,10 +249,12 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
     }
   }
   if (EPI == EPI_STAGE2) {
-    const unsigned long long r = block_max_bits<64>(cfl_bits, sRed);
-    if (lane == 0) atomicMax(a.dt1_next, r);
-    if (bad) atomicOr(a.flag, 1u);
-    if (blockIdx.x == 0 && lane == 0) *a.t_dev += dt;
+    if (cfl_case >= 0) {
+      unsigned long long r = cfl_bits;
+      for (int off = 32; off > 0; off >>= 1) r = max_bits(r, __shfl_xor(r, off, 64));
+      if (lane == 0) atomicMax(a.dt1_next + cfl_case, r);
+    }
+    if (__any(bad) && lane == 0) atomicOr(a.flag, 1u);
   }
 }
 
@@ -238,10 +269,10 @@ static hipError_t launch_t(const Launch &L, const StageArgs &a) {
       nb = 8;
     per_cu = nb;
   }
-  const int ntiles = (a.ncells + 15) / 16;
+  const int ntiles = a.ncases > 0 ? -1 : (a.ncells + 15) / 16;
   int grid = L.cus > 0 ? L.cus * per_cu : 2048;
   if (L.max_blocks > 0 && grid > L.max_blocks) grid = L.max_blocks;
-  if (grid > ntiles) grid = ntiles;
+  if (ntiles >= 0 && grid > ntiles) grid = ntiles;
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) return hipSuccess;
   hipLaunchKernelGGL((rhs_mfma_kernel<N, GATHER, EPI>), dim3(grid), dim3(64), 0, L.stream, a, L.ops,

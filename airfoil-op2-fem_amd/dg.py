@@ -314,8 +314,16 @@ class Solver:
         self._chk(lib.dg_solver_dat(self._h, name.encode(), ctypes.byref(p)))
         return p.value
 
-    def run(self, nsteps, cfl=1.0, mode=MODE_FUSED):
-        self._chk(lib.dg_solver_run(self._h, int(nsteps), float(cfl), int(mode)))
+    def run(self, nsteps, cfl=1.0, mode=MODE_FUSED, graph=False):
+        if graph and mode == MODE_FUSED:
+            self._chk(lib.dg_solver_run_graph(self._h, int(nsteps), float(cfl)))
+        else:
+            self._chk(lib.dg_solver_run(self._h, int(nsteps), float(cfl), int(mode)))
+
+    def stream(self):
+        p = c_void_p()
+        self._chk(lib.dg_solver_stream(self._h, ctypes.byref(p)))
+        return p.value or 0
 
     def time(self):
         t, dt = c_double(), c_double()
@@ -328,6 +336,23 @@ class Solver:
     def check(self):
         return lib.dg_solver_check(self._h)
 
+    def forces(self, chord=1.0):
+        """(C_L, C_D, (Fx, Fy)) from the pressure on the wall faces."""
+        cl, cd = c_double(), c_double()
+        f = np.zeros(2)
+        self._chk(lib.dg_solver_forces(self._h, float(chord), ctypes.byref(cl), ctypes.byref(cd), _np_ptr(f)))
+        return cl.value, cd.value, f
+
+    def wall_cp(self):
+        """(x, Cp) at the face nodes of every wall face, shape [nwall, Nfp]."""
+        n = c_int()
+        self._chk(lib.dg_solver_wall_cp(self._h, ctypes.byref(n), None, None))
+        x = np.zeros((n.value, self.ctx.Nfp))
+        cp = np.zeros((n.value, self.ctx.Nfp))
+        if n.value:
+            self._chk(lib.dg_solver_wall_cp(self._h, ctypes.byref(n), _np_ptr(x), _np_ptr(cp)))
+        return x, cp
+
     def profile(self, enable=True):
         self._chk(lib.dg_solver_profile(self._h, 1 if enable else 0))
 
@@ -337,5 +362,50 @@ class Solver:
         return ms.value, n.value
 
 
-__all__ = ["Context", "Mesh", "Solver", "DGError", "generate_operators", "periodic_grid",
+class BatchSolver(Solver):
+    """Many independent cases (uneven meshes, per-case Mach and angle) run by
+    one persistent kernel launch per RK stage."""
+
+    def __init__(self, ctx, meshes, mach, alpha):
+        self.ctx, self.meshes = ctx, list(meshes)
+        n = len(self.meshes)
+        hs = (c_void_p * n)(*[m._h.value for m in self.meshes])
+        ma = np.ascontiguousarray(mach, dtype=np.float64)
+        al = np.ascontiguousarray(alpha, dtype=np.float64)
+        if ma.size != n or al.size != n:
+            raise ValueError("one Mach number and one angle per case")
+        h = c_void_p()
+        check(lib.dg_solver_create_batch(ctx._h, n, ctypes.cast(hs, c_void_p), _np_ptr(ma), _np_ptr(al),
+                                         ctypes.byref(h)), ctx._h)
+        self._h = h
+        self.ncases = n
+        self.mesh = None
+
+    def case_q(self, k):
+        q = np.zeros((self.meshes[k].ncells, self.ctx.Np, 4))
+        self._chk(lib.dg_solver_case_get_q(self._h, int(k), _np_ptr(q)))
+        return q
+
+    def set_case_q(self, k, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self._chk(lib.dg_solver_case_set_q(self._h, int(k), _np_ptr(q)))
+
+    def case_time(self, k):
+        t, dt = c_double(), c_double()
+        self._chk(lib.dg_solver_case_time(self._h, int(k), ctypes.byref(t), ctypes.byref(dt)))
+        return t.value, dt.value
+
+    def case_forces(self, k, chord=1.0):
+        cl, cd = c_double(), c_double()
+        f = np.zeros(2)
+        self._chk(lib.dg_solver_case_forces(self._h, int(k), float(chord), ctypes.byref(cl),
+                                            ctypes.byref(cd), _np_ptr(f)))
+        return cl.value, cd.value, f
+
+    @property
+    def state_shape(self):
+        return (sum(m.ncells for m in self.meshes), self.ctx.Np, 4)
+
+
+__all__ = ["Context", "Mesh", "Solver", "BatchSolver", "DGError", "generate_operators", "periodic_grid",
            "naca0012_ogrid", "read_vtk", "P", "c_void_p"]

@@ -215,13 +215,51 @@ enum {
  * (src/airfoil.cpp:231-237,322-331; the reference uses cfl = 1).  The first
  * call computes dt from the current q.  Asynchronous in DG_MODE_FUSED. */
 int dg_solver_run(dg_solver *s, int nsteps, double cfl, int mode);
+/* As dg_solver_run in DG_MODE_FUSED, with two steps (one per parity of the
+ * device dt slots) captured once into a hipGraph and replayed; an odd step
+ * count runs its last step directly.  The graph is rebuilt when cfl
+ * changes.  Results are identical to dg_solver_run. */
+int dg_solver_run_graph(dg_solver *s, int nsteps, double cfl);
 /* Simulated time and the dt the next step will use (synchronises). */
 int dg_solver_time(dg_solver *s, double *t, double *dt);
+/* The solver's stream (the context's stream, or a private non-blocking
+ * stream when the context uses the null stream, which cannot be captured). */
+int dg_solver_stream(dg_solver *s, void **hip_stream);
+
+/* ------------------------------------------------------------------ */
+/* Batched sweep (no reference counterpart)                           */
+/* ------------------------------------------------------------------ */
+
+/* ncases independent cases of any sizes in one solver: the meshes are
+ * concatenated into one set of op_dats and every fused stage is one launch
+ * of the persistent matrix-core kernel over a tile table covering all
+ * cases.  Case k has free stream (gam, p, rho of the context, Mach mach[k],
+ * angle alpha_deg[k]; both NULL: the context's free stream for every case)
+ * and its own dt and time.  A case's results are bit-identical to the same
+ * case run alone in a single-case solver. */
+int dg_solver_create_batch(dg_ctx *ctx, int ncases, const dg_mesh *const *meshes,
+                           const double *mach, const double *alpha_deg, dg_solver **out);
+int dg_solver_ncases(const dg_solver *s, int *ncases);
+int dg_solver_case_set_q(dg_solver *s, int k, const double *host_q);
+int dg_solver_case_get_q(dg_solver *s, int k, double *host_q);
+int dg_solver_case_time(dg_solver *s, int k, double *t, double *dt);
+int dg_solver_case_forces(dg_solver *s, int k, double chord, double *cl, double *cd,
+                          double *fxy);
 /* Resets time to 0 and forces dt to be recomputed from q on the next run. */
 int dg_solver_reset_time(dg_solver *s);
 /* Returns DG_ERR_NONFINITE if any step so far produced a non-finite value
  * in the CFL reduction (fused mode), DG_OK otherwise (synchronises). */
 int dg_solver_check(dg_solver *s);
+/* Pressure force on the wall faces (bc 2), integrated with the 1D face mass
+ * matrix as quadrature, as lift and drag coefficients relative to the free
+ * stream of the context (angle bc_alpha, dynamic pressure rho|V|^2/2) and
+ * the chord; fxy (may be NULL) gets the force vector.  Synchronises.  No
+ * reference counterpart (the reference writes only the solution). */
+int dg_solver_forces(dg_solver *s, double chord, double *cl, double *cd, double *fxy);
+/* Surface pressure coefficient (p - p_inf)/(rho|V|^2/2) at the Nfp nodes of
+ * each wall face, in wall-face order, with the nodes' x coordinate.  With
+ * x and cp NULL only *nwall is returned. */
+int dg_solver_wall_cp(dg_solver *s, int *nwall, double *x, double *cp);
 /* Kernel timing with HIP events on the context stream: enable, then read the
  * average duration (ms) and launch count per kernel name ("rhs_stage0",
  * "rhs_stage1", "rhs_stage2", "calc_dt", "euler_rhs", ...). */

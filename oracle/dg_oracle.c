@@ -666,3 +666,29 @@ void or_vortex_l2(const or_consts *c, int ncells, const double *x, const double 
   }
   for (int v = 0; v < 4; v++) err4[v] = sqrt(acc[v]);
 }
+
+void or_wall_forces(const or_consts *c, int nwall, const int *wcell, const int *wface,
+                    const double *q, const double *nx, const double *ny,
+                    const double *fscale, const double *J, const double *facew,
+                    double *fxy) {
+  const int Np = c->Np, Nfp = c->Nfp, NF = 3 * c->Nfp;
+  double fx = 0.0, fy = 0.0;
+  for (int b = 0; b < nwall; b++) {
+    long k = wcell[b];
+    int f = wface[b];
+    double bx = 0.0, by = 0.0;
+    for (int i = 0; i < Nfp; i++) {
+      int m = f * Nfp + i;
+      const double *qq = q + (k * Np + c->FMASK[m]) * 4;
+      double rho = qq[0], u = qq[1] / rho, v = qq[2] / rho;
+      double p = (c->gam - 1.0) * (qq[3] - 0.5 * rho * (u * u + v * v));
+      double sJ = fscale[k * NF + m] * J[k];
+      bx += facew[i] * sJ * p * nx[k * NF + m];
+      by += facew[i] * sJ * p * ny[k * NF + m];
+    }
+    fx += bx;
+    fy += by;
+  }
+  fxy[0] = fx;
+  fxy[1] = fy;
+}

@@ -138,6 +138,14 @@ void or_vortex_l2(const or_consts *c, int ncells, const double *x, const double 
                   double uinf, double beta, double x0, double y0, double t,
                   double period_x, double *err4);
 
+/* Pressure force on the faces listed in (wcell, wface): sum over faces and
+ * face nodes of facew[i] * fscale * J * p * (nx, ny), faces in list order.
+ * No reference counterpart (surface C_L of the airfoil case). */
+void or_wall_forces(const or_consts *c, int nwall, const int *wcell, const int *wface,
+                    const double *q, const double *nx, const double *ny,
+                    const double *fscale, const double *J, const double *facew,
+                    double *fxy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -265,6 +265,20 @@ class Oracle:
             ext_after[k] = ex.reshape(NF, 4)
         return rhs, ext
 
+    def wall_forces(self, case, q, facew):
+        """Pressure force (Fx, Fy) on the wall (type 2) boundary faces."""
+        sel = case["bedge_type"] == 2
+        wc = np.ascontiguousarray(case["bedge2cells"][sel], dtype=np.int32)
+        wf = np.ascontiguousarray(case["bedgeNum"][sel], dtype=np.int32)
+        g = case["geo"]
+        x, y = g["nodeX"], g["nodeY"]
+        J = 0.25 * ((x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0]))
+        J = np.ascontiguousarray(J)
+        out = np.zeros(2)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (q, g["nx"], g["ny"], g["fscale"], J, facew)]
+        lib.or_wall_forces(ctypes.byref(self.c), int(wc.size), _p(wc), _p(wf), *[_p(a) for a in arrs], _p(out))
+        return out
+
     def vortex_state(self, uinf, beta, x0, y0, t, x, y, period_x=0.0):
         x = np.asarray(x, dtype=np.float64)
         y = np.asarray(y, dtype=np.float64)

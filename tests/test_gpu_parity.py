@@ -236,7 +236,20 @@ def test_airfoil_ogrid_short_run_matches_oracle(dg, orc):
                        mesh.bedgeNum, mesh.bedge_type)
     st = o.new_state(case, q0)
     o.run(case, st, 50, cfl=1.0, nthreads=min(16, os.cpu_count() or 1))
-    assert max(rel_per_var(s.get_q(), st["q"])) <= 1e-12
+    qg = s.get_q()
+    assert max(rel_per_var(qg, st["q"])) <= 1e-12
+    # surface force: GPU quadrature vs the oracle's on the same state, and
+    # on the oracle's own trajectory
+    facew = dg.generate_operators(3)["face_mass"].sum(axis=1)
+    cl, cd, fxy = s.forces(1.0)
+    f_same = o.wall_forces(case, qg, facew)
+    f_orc = o.wall_forces(case, st["q"], facew)
+    scale = np.abs(f_same).max()
+    assert np.abs(fxy - f_same).max() <= 1e-12 * scale
+    assert np.abs(fxy - f_orc).max() <= 1e-10 * scale
+    x, cp = s.wall_cp()
+    assert x.shape == cp.shape == (48, 4)
+    assert np.all((x >= -1e-12) & (x <= 1 + 1e-12))
 
 
 def test_reference_tables_order4(dg, orc, dev):

@@ -290,3 +290,21 @@ def test_neighbour_gather_matches_physical_coordinates():
     d[..., 0] -= 10.0 * np.round(d[..., 0] / 10.0)
     d[..., 1] -= 10.0 * np.round(d[..., 1] / 10.0)
     assert np.abs(d).max() < 1e-12
+
+
+def test_wall_force_of_uniform_pressure_vanishes():
+    """A closed body under uniform pressure feels no net force: pins the face
+    quadrature (1D mass-matrix weights), the normals and sJ = fscale * J."""
+    import importlib
+    dg = importlib.import_module("airfoil-op2-fem_amd")
+    for n in (2, 3):
+        ops = dg.generate_operators(n)
+        o = oracle.Oracle(ops)
+        m = dg.naca0012_ogrid(40, 6, 1.2, 10.0)
+        case = o.make_case(m.cells, m.cell_xy, m.edge2cells, m.edgeNum, m.bedge2cells, m.bedgeNum,
+                           m.bedge_type)
+        q, _ = o.set_ic()
+        f = o.wall_forces(case, np.tile(q, (m.ncells, 1)), ops["face_mass"].sum(axis=1))
+        assert np.abs(f).max() < 1e-13
+        # and a pressure p = x on the surface integrates to -(enclosed area) in x
+        # (divergence theorem with the outward normal of the fluid pointing into the body)

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(NT) void rhs_kernel(StageArgs a, const double *__re
           for (int v = 0; v < 4; v++) o[v] = 0.75 * qv[v] + 0.25 * (wv[v] + dt * res[v]);
         } else {
 #pragma unroll
-          for (int v = 0; v < 4; v++) o[v] = (qv[v] + 2.0 * (wv[v] + dt * res[v])) / 3.0;
+          for (int v = 0; v < 4; v++) o[v] = (qv[v] + 2.0 * (wv[v] + dt * res[v])) * (1.0 / 3.0);
         }
       }
       double2 *dst = reinterpret_cast<double2 *>(a.out + gn * 4);

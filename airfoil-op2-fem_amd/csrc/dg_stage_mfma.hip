@@ -4,15 +4,16 @@
 // cells, split into 4 groups of 4 cells.  For one group the volume term is
 // four 16x16 products on v_mfma_f64_16x16x4f64,
 //
-//     D[(cell, var)][node j] = sum_k  A[(cell, var)][k] * B[k][j],
+//     D[cell + 4*var][node j] = sum_k  A[cell + 4*var][k] * B[k][j],
 //
 // with A = F or G of the group's 4 cells (rows: 4 cells x 4 variables,
 // columns: the Np source nodes padded to a multiple of 4) and B = Drw^T or
 // Dsw^T (padded to 16 output nodes).  B is the same for every cell, so each
 // lane keeps its B entries in registers for the whole kernel; A is staged in
 // LDS already in the operand layout, so every MFMA reads one contiguous
-// 512-byte row.  In the accumulator layout lane l holds D[4*(l/16)+r][l%16],
-// i.e. cell l/16 of the group, node l%16, variables r = 0..3: the metric
+// 512-byte row.  The FP64 accumulator layout is row-interleaved: lane l holds
+// D[l/16 + 4*r][l%16], r = 0..3.  Rows are therefore ordered (cell + 4*var),
+// so lane l holds cell l/16 of the group, node l%16, variables r = 0..3: the metric
 // terms are then lane-local, the LIFT product (A = Roe fluxes, B = -LIFT^T)
 // accumulates onto the same registers, and the epilogue stores each lane's
 // 4 variables of one node as one 32-byte record (the op_dat layout), so
@@ -124,13 +125,13 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
       const double2 q01 = src[0], q23 = src[1];
       double qv[4] = {q01.x, q01.y, q23.x, q23.y}, F[4], Gv[4];
       euler_flux(ph.gam, qv, F, Gv);
-      const int off = ((e >> 2) * KP + (k >> 2)) * 64 + 16 * (k & 3) + 4 * (e & 3);
-      double2 *dF = reinterpret_cast<double2 *>(sF + off);
-      double2 *dG = reinterpret_cast<double2 *>(sG + off);
-      dF[0] = make_double2(F[0], F[1]);
-      dF[1] = make_double2(F[2], F[3]);
-      dG[0] = make_double2(Gv[0], Gv[1]);
-      dG[1] = make_double2(Gv[2], Gv[3]);
+      // A row = cell + 4*var, A column = source node k
+      const int off = ((e >> 2) * KP + (k >> 2)) * 64 + 16 * (k & 3) + (e & 3);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        sF[off + 4 * v] = F[v];
+        sG[off + 4 * v] = Gv[v];
+      }
     }
     __syncthreads();
 
@@ -194,10 +195,9 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
       }
       double fl[4];
       roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
-      const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + 4 * (e & 3);
-      double2 *d = reinterpret_cast<double2 *>(sL + off);
-      d[0] = make_double2(fl[0], fl[1]);
-      d[1] = make_double2(fl[2], fl[3]);
+      const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
+#pragma unroll
+      for (int v = 0; v < 4; v++) sL[off + 4 * v] = fl[v];
     }
     __syncthreads();
 
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
             for (int v = 0; v < 4; v++) o[v] = 0.75 * qv[v] + 0.25 * (wv[v] + dt * c[v]);
           } else {
 #pragma unroll
-            for (int v = 0; v < 4; v++) o[v] = (qv[v] + 2.0 * (wv[v] + dt * c[v])) / 3.0;
+            for (int v = 0; v < 4; v++) o[v] = (qv[v] + 2.0 * (wv[v] + dt * c[v])) * (1.0 / 3.0);
           }
         }
       }

@@ -20,6 +20,9 @@ def single(dg, mesh, mach, alpha):
 
 
 def test_batch_cases_bit_identical_to_single_runs(dg):
+    import os
+    if os.environ.get("DG_STAGE_VARIANT") == "1":
+        pytest.skip("batches run the matrix-core stage kernel only")
     specs = [(24, 6, 0.35, -2.0), (40, 10, 0.5, 4.0), (32, 8, 0.45, 7.5)]
     meshes = [dg.naca0012_ogrid(nc, nr, 1.15, 15.0) for nc, nr, _, _ in specs]
     ctx = dg.Context(3)

@@ -244,9 +244,11 @@ def test_airfoil_ogrid_short_run_matches_oracle(dg, orc):
     cl, cd, fxy = s.forces(1.0)
     f_same = o.wall_forces(case, qg, facew)
     f_orc = o.wall_forces(case, st["q"], facew)
-    scale = np.abs(f_same).max()
-    assert np.abs(fxy - f_same).max() <= 1e-12 * scale
-    assert np.abs(fxy - f_orc).max() <= 1e-10 * scale
+    # the net force is a small difference of O(1) pressure integrals over a
+    # perimeter of ~2 chords: tolerances are relative to that integral
+    scale = 2.0 * np.abs(qg[..., 3]).max()
+    assert np.abs(fxy - f_same).max() <= 1e-13 * scale
+    assert np.abs(fxy - f_orc).max() <= 1e-11 * scale
     x, cp = s.wall_cp()
     assert x.shape == cp.shape == (48, 4)
     assert np.all((x >= -1e-12) & (x <= 1 + 1e-12))

@@ -1,0 +1,524 @@
+// Tangent-linear and adjoint (reverse-mode) kernels of the DG Euler RHS
+// L(w) = vol(w) - LIFT * flux(w), with respect to the state, the geometric
+// factors (rx, ry, sx, sy, nx, ny, fscale) and the free-stream velocity that
+// the far-field boundary sees (bc_u, bc_v).
+//
+// Tangent:  Ld = L'(w) wd + dL/dgeo geod + dL/dbc bcd, with the point
+//           functions evaluated in dual arithmetic.
+// Adjoint:  acc += scale * L'(w)^T mu, geometry and bc accumulators
+//           += scale * (dL/dgeo)^T mu, (dL/dbc)^T mu.  Gather-only in two
+//           passes: the face pass evaluates the reverse Roe flux once per
+//           face slot and stores the interior-side (gmq) and exterior-side
+//           (gpq) cotangents; the cell pass applies the transposed weak
+//           derivatives, the reverse Euler fluxes, and gathers its own gmq and
+//           the neighbours' gpq of the slots that read its nodes.
+#include <hip/hip_runtime.h>
+
+#include "dg_adjoint.h"
+#include "dg_device.h"
+#include "dg_dual.h"
+
+namespace dg {
+
+// ------------------------------------------------------------------------
+// Point functions in generic arithmetic (same formulas as dg_device.h)
+// ------------------------------------------------------------------------
+
+template <class T>
+__device__ __forceinline__ void euler_flux_t(double gam, const T q[4], T F[4], T G[4]) {
+  T rho = q[0];
+  T u = q[1] / rho;
+  T v = q[2] / rho;
+  T p = (gam - 1) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
+  F[0] = q[1]; F[1] = q[1] * u + p; F[2] = q[2] * u; F[3] = u * (q[3] + p);
+  G[0] = q[2]; G[1] = q[1] * v; G[2] = q[2] * v + p; G[3] = v * (q[3] + p);
+}
+
+template <class T>
+__device__ __forceinline__ void roe_t(double gam, const T mQ[4], const T eQ[4], T nx, T ny, T fs,
+                                      T out[4]) {
+  T mR0 = mQ[0], mR1 = nx * mQ[1] + ny * mQ[2], mR2 = -ny * mQ[1] + nx * mQ[2], mR3 = mQ[3];
+  T pR0 = eQ[0], pR1 = nx * eQ[1] + ny * eQ[2], pR2 = -ny * eQ[1] + nx * eQ[2], pR3 = eQ[3];
+  T mU = mR1 / mR0, mV = mR2 / mR0;
+  T mP = (gam - 1) * (mR3 - 0.5 * (mR1 * mU + mR2 * mV));
+  T pU = pR1 / pR0, pV = pR2 / pR0;
+  T pP = (gam - 1) * (pR3 - 0.5 * (pR1 * pU + pR2 * pV));
+  T fm0 = mR1, fm1 = mR1 * mU + mP, fm2 = mR2 * mU, fm3 = mU * (mR3 + mP);
+  T fp0 = pR1, fp1 = pR1 * pU + pP, fp2 = pR2 * pU, fp3 = pU * (pR3 + pP);
+  T mH = (mR3 + mP) / mR0, pH = (pR3 + pP) / pR0;
+  T ms = sqrt(mR0), ps = sqrt(pR0);
+  T rho = ms * ps, den = ms + ps;
+  T u = (ms * mU + ps * pU) / den, v = (ms * mV + ps * pV) / den, H = (ms * mH + ps * pH) / den;
+  T c2 = (gam - 1) * (H - 0.5 * (u * u + v * v));
+  T c = sqrt(c2);
+  T W1 = -0.5 * rho * (pU - mU) / c + 0.5 * (pP - mP) / c2;
+  T W2 = (pR0 - mR0) - (pP - mP) / c2;
+  T W3 = rho * (pV - mV);
+  T W4 = 0.5 * rho * (pU - mU) / c + 0.5 * (pP - mP) / c2;
+  W1 = fabs(u - c) * W1;
+  W2 = fabs(u) * W2;
+  W3 = fabs(u) * W3;
+  W4 = fabs(u + c) * W4;
+  T f0 = (fm0 + fp0) / 2.0 - (W1 + W2 + W4) / 2.0;
+  T f1 = (fm1 + fp1) / 2.0 - (W1 * (u - c) + W2 * u + W4 * (u + c)) / 2.0;
+  T f2 = (fm2 + fp2) / 2.0 - (W1 * v + W2 * v + W3 + W4 * v) / 2.0;
+  T f3 = (fm3 + fp3) / 2.0 - (W1 * (H - u * c) + W2 * (u * u + v * v) + W3 * v + W4 * (H + u * c)) / 2.0;
+  out[0] = f0 * fs;
+  out[1] = (nx * f1 - ny * f2) * fs;
+  out[2] = (ny * f1 + nx * f2) * fs;
+  out[3] = f3 * fs;
+}
+
+// Boundary states of types 0-3 (src/get_bedge_q.h + characteristic far
+// field) with the free-stream velocity as an input.
+template <class T>
+__device__ __forceinline__ void boundary_t(const Phys &ph, int type, T bcu, T bcv, const T mQ[4],
+                                           T nx, T ny, T qb[4]) {
+  const double g = ph.gam;
+  T bce = ph.bc_p / (g - 1.0) + 0.5 * ph.bc_r * (bcu * bcu + bcv * bcv);
+  if (type == 0) {
+    qb[0] = T(ph.bc_r); qb[1] = ph.bc_r * bcu; qb[2] = ph.bc_r * bcv; qb[3] = bce;
+  } else if (type == 1) {
+    qb[0] = T(ph.bc_r); qb[1] = ph.bc_r * bcu; qb[2] = ph.bc_r * bcv; qb[3] = mQ[3];
+  } else if (type == 2) {
+    T vn = nx * mQ[1] + ny * mQ[2];
+    qb[0] = mQ[0]; qb[1] = mQ[1] - 2 * vn * nx; qb[2] = mQ[2] - 2 * vn * ny; qb[3] = mQ[3];
+  } else {
+    T rhoi = mQ[0], ui = mQ[1] / rhoi, vi = mQ[2] / rhoi;
+    T pi_ = (g - 1.0) * (mQ[3] - 0.5 * rhoi * (ui * ui + vi * vi));
+    T ci = sqrt(fabs(g * pi_ / rhoi));
+    T vni = ui * nx + vi * ny;
+    const double cinf = ::sqrt(g * ph.bc_p / ph.bc_r);
+    T vninf = bcu * nx + bcv * ny;
+    if (val(fabs(vni)) >= val(ci)) {
+      if (val(vni) < 0.0) {
+        qb[0] = T(ph.bc_r); qb[1] = ph.bc_r * bcu; qb[2] = ph.bc_r * bcv; qb[3] = bce;
+      } else {
+        for (int k = 0; k < 4; k++) qb[k] = mQ[k];
+      }
+      return;
+    }
+    T rp = vni + 2.0 * ci / (g - 1.0);
+    T rm = vninf - 2.0 * cinf / (g - 1.0);
+    T vnb = 0.5 * (rp + rm);
+    T cb = 0.25 * (g - 1.0) * (rp - rm);
+    T ent, ut, vt;
+    if (val(vnb) > 0.0) {
+      ent = pi_ / pow(rhoi, g);
+      ut = ui - vni * nx; vt = vi - vni * ny;
+    } else {
+      ent = T(ph.bc_p / ::pow(ph.bc_r, g));
+      ut = bcu - vninf * nx; vt = bcv - vninf * ny;
+    }
+    T rhob = pow(cb * cb / (g * ent), 1.0 / (g - 1.0));
+    T pb = rhob * cb * cb / g;
+    T ub = ut + vnb * nx, vb = vt + vnb * ny;
+    qb[0] = rhob; qb[1] = rhob * ub; qb[2] = rhob * vb;
+    qb[3] = pb / (g - 1.0) + 0.5 * rhob * (ub * ub + vb * vb);
+  }
+}
+
+// ------------------------------------------------------------------------
+// Reverse-mode point functions
+// ------------------------------------------------------------------------
+
+// Reverse of euler_flux: accumulates F^T bF + G^T bG into bq.
+__device__ __forceinline__ void euler_flux_vjp(double gam, const double q[4], const double bF[4],
+                                               const double bG[4], double bq[4]) {
+  const double g1 = gam - 1.0;
+  const double rho = q[0], u = q[1] / rho, v = q[2] / rho;
+  const double p = g1 * (q[3] - 0.5 * (q[1] * u + q[2] * v));
+  double bu = bF[1] * q[1] + bF[2] * q[2] + bF[3] * (q[3] + p);
+  double bv = bG[1] * q[1] + bG[2] * q[2] + bG[3] * (q[3] + p);
+  double bp = bF[1] + bF[3] * u + bG[2] + bG[3] * v;
+  double b1 = bF[0] + bF[1] * u + bG[1] * v;
+  double b2 = bF[2] * u + bG[0] + bG[2] * v;
+  double b3 = bF[3] * u + bG[3] * v;
+  // p = g1 (q3 - 0.5 (q1 u + q2 v))
+  b3 += g1 * bp;
+  b1 += -0.5 * g1 * u * bp;
+  bu += -0.5 * g1 * q[1] * bp;
+  b2 += -0.5 * g1 * v * bp;
+  bv += -0.5 * g1 * q[2] * bp;
+  // u = q1/rho, v = q2/rho
+  b1 += bu / rho;
+  b2 += bv / rho;
+  const double b0 = -(bu * u + bv * v) / rho;
+  bq[0] += b0; bq[1] += b1; bq[2] += b2; bq[3] += b3;
+}
+
+// Reverse of roe_t: given the cotangent o[4] of the scaled flux, returns the
+// cotangents of mQ, eQ (accumulated) and of nx, ny, fs.
+__device__ void roe_vjp(double gam, const double mQ[4], const double eQ[4], double nx, double ny,
+                        double fs, const double o[4], double bmQ[4], double beQ[4], double &bnx,
+                        double &bny, double &bfs) {
+  const double g1 = gam - 1.0;
+  // forward sweep
+  const double mR0 = mQ[0], mR1 = nx * mQ[1] + ny * mQ[2], mR2 = -ny * mQ[1] + nx * mQ[2], mR3 = mQ[3];
+  const double pR0 = eQ[0], pR1 = nx * eQ[1] + ny * eQ[2], pR2 = -ny * eQ[1] + nx * eQ[2], pR3 = eQ[3];
+  const double mU = mR1 / mR0, mV = mR2 / mR0;
+  const double mP = g1 * (mR3 - 0.5 * (mR1 * mU + mR2 * mV));
+  const double pU = pR1 / pR0, pV = pR2 / pR0;
+  const double pP = g1 * (pR3 - 0.5 * (pR1 * pU + pR2 * pV));
+  const double fm1 = mR1 * mU + mP, fm2 = mR2 * mU, fm3 = mU * (mR3 + mP);
+  const double fp1 = pR1 * pU + pP, fp2 = pR2 * pU, fp3 = pU * (pR3 + pP);
+  const double mH = (mR3 + mP) / mR0, pH = (pR3 + pP) / pR0;
+  const double ms = sqrt(mR0), ps = sqrt(pR0);
+  const double rho = ms * ps, den = ms + ps;
+  const double u = (ms * mU + ps * pU) / den, v = (ms * mV + ps * pV) / den;
+  const double H = (ms * mH + ps * pH) / den;
+  const double c2 = g1 * (H - 0.5 * (u * u + v * v));
+  const double c = sqrt(c2);
+  const double dU = pU - mU, dP = pP - mP, dV = pV - mV, dR = pR0 - mR0;
+  const double W1 = -0.5 * rho * dU / c + 0.5 * dP / c2;
+  const double W2 = dR - dP / c2;
+  const double W3 = rho * dV;
+  const double W4 = 0.5 * rho * dU / c + 0.5 * dP / c2;
+  const double a1 = fabs(u - c), a2 = fabs(u), a4 = fabs(u + c);
+  const double X1 = a1 * W1, X2 = a2 * W2, X3 = a2 * W3, X4 = a4 * W4;
+  const double f0 = (mR1 + pR1) / 2.0 - (X1 + X2 + X4) / 2.0;
+  const double f1 = (fm1 + fp1) / 2.0 - (X1 * (u - c) + X2 * u + X4 * (u + c)) / 2.0;
+  const double f2 = (fm2 + fp2) / 2.0 - (X1 * v + X2 * v + X3 + X4 * v) / 2.0;
+  const double f3 = (fm3 + fp3) / 2.0 -
+                    (X1 * (H - u * c) + X2 * (u * u + v * v) + X3 * v + X4 * (H + u * c)) / 2.0;
+  // reverse sweep
+  bfs = o[0] * f0 + o[1] * (nx * f1 - ny * f2) + o[2] * (ny * f1 + nx * f2) + o[3] * f3;
+  const double b1o = o[1] * fs, b2o = o[2] * fs;
+  bnx = b1o * f1 + b2o * f2;
+  bny = -b1o * f2 + b2o * f1;
+  const double bf0 = o[0] * fs, bf1 = nx * b1o + ny * b2o, bf2 = -ny * b1o + nx * b2o, bf3 = o[3] * fs;
+  double bmR1 = 0, bpR1 = 0, bmR0 = 0, bpR0 = 0, bmR2 = 0, bpR2 = 0, bmR3 = 0, bpR3 = 0;
+  double bfm1 = bf1 / 2, bfp1 = bf1 / 2, bfm2 = bf2 / 2, bfp2 = bf2 / 2, bfm3 = bf3 / 2, bfp3 = bf3 / 2;
+  bmR1 += bf0 / 2; bpR1 += bf0 / 2;
+  double bX1 = -bf0 / 2, bX2 = -bf0 / 2, bX3 = 0, bX4 = -bf0 / 2;
+  double bu = 0, bv = 0, bc = 0, bH = 0;
+  double t = -bf1 / 2;
+  bX1 += t * (u - c); bX2 += t * u; bX4 += t * (u + c);
+  bu += t * (X1 + X2 + X4); bc += t * (-X1 + X4);
+  t = -bf2 / 2;
+  bX1 += t * v; bX2 += t * v; bX3 += t; bX4 += t * v;
+  bv += t * (X1 + X2 + X4);
+  t = -bf3 / 2;
+  bX1 += t * (H - u * c); bX2 += t * (u * u + v * v); bX3 += t * v; bX4 += t * (H + u * c);
+  bH += t * (X1 + X4);
+  bu += t * (-X1 * c + 2.0 * u * X2 + X4 * c);
+  bc += t * (-X1 * u + X4 * u);
+  bv += t * (2.0 * v * X2 + X3);
+  // X = a * W
+  const double bW1 = a1 * bX1, bW2 = a2 * bX2, bW3 = a2 * bX3, bW4 = a4 * bX4;
+  const double ba1 = W1 * bX1, ba2 = W2 * bX2 + W3 * bX3, ba4 = W4 * bX4;
+  const double s1 = (u - c) >= 0.0 ? 1.0 : -1.0, s2 = u >= 0.0 ? 1.0 : -1.0, s4 = (u + c) >= 0.0 ? 1.0 : -1.0;
+  bu += ba1 * s1 + ba2 * s2 + ba4 * s4;
+  bc += -ba1 * s1 + ba4 * s4;
+  // W terms
+  double brho = 0, bdU = 0, bdP = 0, bdV = 0, bdR = 0, bc2 = 0;
+  brho += bW1 * (-0.5 * dU / c); bdU += bW1 * (-0.5 * rho / c);
+  bc += bW1 * (0.5 * rho * dU / (c * c)); bdP += bW1 * (0.5 / c2); bc2 += bW1 * (-0.5 * dP / (c2 * c2));
+  bdR += bW2; bdP += -bW2 / c2; bc2 += bW2 * dP / (c2 * c2);
+  brho += bW3 * dV; bdV += bW3 * rho;
+  brho += bW4 * (0.5 * dU / c); bdU += bW4 * (0.5 * rho / c);
+  bc += bW4 * (-0.5 * rho * dU / (c * c)); bdP += bW4 * (0.5 / c2); bc2 += bW4 * (-0.5 * dP / (c2 * c2));
+  double bpU = bdU, bmU = -bdU, bpP = bdP, bmP = -bdP, bpV = bdV, bmV = -bdV;
+  bpR0 += bdR; bmR0 -= bdR;
+  // c = sqrt(c2); c2 = g1 (H - (u^2+v^2)/2)
+  bc2 += bc / (2.0 * c);
+  bH += g1 * bc2; bu += -g1 * u * bc2; bv += -g1 * v * bc2;
+  // Roe averages
+  double bms = bu * (mU - u) / den + bv * (mV - v) / den + bH * (mH - H) / den;
+  double bps = bu * (pU - u) / den + bv * (pV - v) / den + bH * (pH - H) / den;
+  bmU += bu * ms / den; bpU += bu * ps / den;
+  bmV += bv * ms / den; bpV += bv * ps / den;
+  double bmH = bH * ms / den, bpH = bH * ps / den;
+  bms += brho * ps; bps += brho * ms;
+  bmR0 += bms / (2.0 * ms); bpR0 += bps / (2.0 * ps);
+  // mH = (mR3 + mP)/mR0
+  bmR3 += bmH / mR0; bmP += bmH / mR0; bmR0 += -bmH * mH / mR0;
+  bpR3 += bpH / pR0; bpP += bpH / pR0; bpR0 += -bpH * pH / pR0;
+  // fluxes in the rotated frame
+  bmR1 += bfm1 * mU; bmU += bfm1 * mR1 + bfm2 * mR2 + bfm3 * (mR3 + mP); bmP += bfm1 + bfm3 * mU;
+  bmR2 += bfm2 * mU; bmR3 += bfm3 * mU;
+  bpR1 += bfp1 * pU; bpU += bfp1 * pR1 + bfp2 * pR2 + bfp3 * (pR3 + pP); bpP += bfp1 + bfp3 * pU;
+  bpR2 += bfp2 * pU; bpR3 += bfp3 * pU;
+  // pressures
+  bmR3 += g1 * bmP; bmR1 += -0.5 * g1 * mU * bmP; bmU += -0.5 * g1 * mR1 * bmP;
+  bmR2 += -0.5 * g1 * mV * bmP; bmV += -0.5 * g1 * mR2 * bmP;
+  bpR3 += g1 * bpP; bpR1 += -0.5 * g1 * pU * bpP; bpU += -0.5 * g1 * pR1 * bpP;
+  bpR2 += -0.5 * g1 * pV * bpP; bpV += -0.5 * g1 * pR2 * bpP;
+  // velocities
+  bmR1 += bmU / mR0; bmR2 += bmV / mR0; bmR0 += -(bmU * mU + bmV * mV) / mR0;
+  bpR1 += bpU / pR0; bpR2 += bpV / pR0; bpR0 += -(bpU * pU + bpV * pV) / pR0;
+  // rotations
+  bmQ[0] += bmR0; bmQ[3] += bmR3;
+  bmQ[1] += nx * bmR1 - ny * bmR2; bmQ[2] += ny * bmR1 + nx * bmR2;
+  beQ[0] += bpR0; beQ[3] += bpR3;
+  beQ[1] += nx * bpR1 - ny * bpR2; beQ[2] += ny * bpR1 + nx * bpR2;
+  bnx += bmR1 * mQ[1] + bmR2 * mQ[2] + bpR1 * eQ[1] + bpR2 * eQ[2];
+  bny += bmR1 * mQ[2] - bmR2 * mQ[1] + bpR1 * eQ[2] - bpR2 * eQ[1];
+}
+
+// Reverse of boundary_t through its dual Jacobian (8 directions: the four
+// interior variables, the normal, the free-stream velocity).
+__device__ void boundary_vjp(const Phys &ph, int type, const double mQ[4], double nx, double ny,
+                             const double bqb[4], double bmQ[4], double &bnx, double &bny,
+                             double &bbcu, double &bbcv) {
+  for (int dir = 0; dir < 8; dir++) {
+    Dual q[4], qb[4];
+    for (int k = 0; k < 4; k++) q[k] = Dual(mQ[k], dir == k ? 1.0 : 0.0);
+    Dual dnx(nx, dir == 4 ? 1.0 : 0.0), dny(ny, dir == 5 ? 1.0 : 0.0);
+    Dual du(ph.bc_u, dir == 6 ? 1.0 : 0.0), dv(ph.bc_v, dir == 7 ? 1.0 : 0.0);
+    boundary_t<Dual>(ph, type, du, dv, q, dnx, dny, qb);
+    double s = 0.0;
+    for (int k = 0; k < 4; k++) s += bqb[k] * qb[k].d;
+    if (dir < 4) bmQ[dir] += s;
+    else if (dir == 4) bnx += s;
+    else if (dir == 5) bny += s;
+    else if (dir == 6) bbcu += s;
+    else bbcv += s;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Kernels
+// ------------------------------------------------------------------------
+
+template <int N>
+__global__ void tlm_face_kernel(TlmArgs a, Phys ph, double bcud, double bcvd) {
+  constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
+  const size_t n = (size_t)a.ncells * NF;
+  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
+       it += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = it / NF;
+    const int m = (int)(it - c * NF);
+    const int f = m / Nfp, i = m - f * Nfp;
+    const int node = a.fmask[m];
+    Dual mQ[4], pQ[4], fl[4];
+    for (int v = 0; v < 4; v++) mQ[v] = Dual(a.w[(c * Np + node) * 4 + v], a.wd[(c * Np + node) * 4 + v]);
+    Dual nx(a.nx[it], a.nxd ? a.nxd[it] : 0.0), ny(a.ny[it], a.nyd ? a.nyd[it] : 0.0);
+    Dual fs(a.fscale[it], a.fsd ? a.fsd[it] : 0.0);
+    const int code = a.fmap[c * 3 + f];
+    if (code >= 0) {
+      const size_t nb = code >> 2;
+      const int nn = a.fmask[(code & 3) * Nfp + (Nfp - 1 - i)];
+      for (int v = 0; v < 4; v++) pQ[v] = Dual(a.w[(nb * Np + nn) * 4 + v], a.wd[(nb * Np + nn) * 4 + v]);
+    } else {
+      boundary_t<Dual>(ph, -code - 1, Dual(ph.bc_u, bcud), Dual(ph.bc_v, bcvd), mQ, nx, ny, pQ);
+    }
+    roe_t<Dual>(ph.gam, mQ, pQ, nx, ny, fs, fl);
+    for (int v = 0; v < 4; v++) a.fluxd[it * 4 + v] = fl[v].d;
+  }
+}
+
+template <int N>
+__global__ void tlm_cell_kernel(TlmArgs a, Phys ph, const double *__restrict__ ops) {
+  constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
+  const double *Drw = ops, *Dsw = ops + Np * Np, *LIFT = ops + 2 * Np * Np;
+  const size_t n = (size_t)a.ncells * Np;
+  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
+       it += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = it / Np;
+    const int j = (int)(it - c * Np);
+    Dual dFdr[4], dFds[4], dGdr[4], dGds[4];
+    for (int i = 0; i < Np; i++) {
+      Dual q[4], F[4], G[4];
+      for (int v = 0; v < 4; v++) q[v] = Dual(a.w[(c * Np + i) * 4 + v], a.wd[(c * Np + i) * 4 + v]);
+      euler_flux_t<Dual>(ph.gam, q, F, G);
+      const double ar = Drw[j * Np + i], as = Dsw[j * Np + i];
+      for (int v = 0; v < 4; v++) {
+        dFdr[v] += ar * F[v]; dFds[v] += as * F[v]; dGdr[v] += ar * G[v]; dGds[v] += as * G[v];
+      }
+    }
+    Dual rx(a.rx[it], a.rxd ? a.rxd[it] : 0.0), sx(a.sx[it], a.sxd ? a.sxd[it] : 0.0);
+    Dual ry(a.ry[it], a.ryd ? a.ryd[it] : 0.0), sy(a.sy[it], a.syd ? a.syd[it] : 0.0);
+    for (int v = 0; v < 4; v++) {
+      Dual r = (rx * dFdr[v] + sx * dFds[v]) + (ry * dGdr[v] + sy * dGds[v]);
+      double lift = 0.0;
+      for (int m = 0; m < NF; m++) lift += LIFT[j * NF + m] * a.fluxd[(c * NF + m) * 4 + v];
+      a.out[it * 4 + v] = r.d - lift;
+    }
+  }
+}
+
+template <int N>
+__global__ void adj_face_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops) {
+  constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
+  const double *LIFT = ops + 2 * Np * Np;
+  const size_t n = (size_t)a.ncells * NF;
+  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
+       it += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = it / NF;
+    const int m = (int)(it - c * NF);
+    const int f = m / Nfp, i = m - f * Nfp;
+    // cotangent of the scaled flux at this slot: L = vol - LIFT flux
+    double psi[4] = {0, 0, 0, 0};
+    for (int j = 0; j < Np; j++) {
+      const double l = LIFT[j * NF + m];
+      for (int v = 0; v < 4; v++) psi[v] -= l * a.mu[(c * Np + j) * 4 + v];
+    }
+    const int node = a.fmask[m];
+    double mQ[4], pQ[4];
+    for (int v = 0; v < 4; v++) mQ[v] = a.w[(c * Np + node) * 4 + v];
+    const double nx = a.nx[it], ny = a.ny[it], fs = a.fscale[it];
+    const int code = a.fmap[c * 3 + f];
+    if (code >= 0) {
+      const size_t nb = code >> 2;
+      const int nn = a.fmask[(code & 3) * Nfp + (Nfp - 1 - i)];
+      for (int v = 0; v < 4; v++) pQ[v] = a.w[(nb * Np + nn) * 4 + v];
+    } else {
+      Dual q[4], qb[4];
+      for (int v = 0; v < 4; v++) q[v] = Dual(mQ[v]);
+      boundary_t<Dual>(ph, -code - 1, Dual(ph.bc_u), Dual(ph.bc_v), q, Dual(nx), Dual(ny), qb);
+      for (int v = 0; v < 4; v++) pQ[v] = qb[v].v;
+    }
+    double bm[4] = {0, 0, 0, 0}, bp[4] = {0, 0, 0, 0}, bnx = 0, bny = 0, bfs = 0;
+    roe_vjp(ph.gam, mQ, pQ, nx, ny, fs, psi, bm, bp, bnx, bny, bfs);
+    if (code < 0) {
+      double bbu = 0, bbv = 0;
+      boundary_vjp(ph, -code - 1, mQ, nx, ny, bp, bm, bnx, bny, bbu, bbv);
+      a.g_bcu[it] += a.scale * bbu;
+      a.g_bcv[it] += a.scale * bbv;
+      for (int v = 0; v < 4; v++) bp[v] = 0.0;
+    }
+    for (int v = 0; v < 4; v++) {
+      a.gmq[it * 4 + v] = bm[v];
+      a.gpq[it * 4 + v] = bp[v];
+    }
+    a.g_nx[it] += a.scale * bnx;
+    a.g_ny[it] += a.scale * bny;
+    a.g_fs[it] += a.scale * bfs;
+  }
+}
+
+template <int N>
+__global__ void adj_cell_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops,
+                                const int *__restrict__ node_slots) {
+  constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
+  const double *Drw = ops, *Dsw = ops + Np * Np;
+  const size_t n = (size_t)a.ncells * Np;
+  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
+       it += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = it / Np;
+    const int i = (int)(it - c * Np);
+    // transposed weak derivatives with the metric terms
+    double phF[4] = {0, 0, 0, 0}, phG[4] = {0, 0, 0, 0};
+    for (int j = 0; j < Np; j++) {
+      const size_t gj = c * Np + j;
+      const double cr = Drw[j * Np + i], cs = Dsw[j * Np + i];
+      const double aF = cr * a.rx[gj] + cs * a.sx[gj], aG = cr * a.ry[gj] + cs * a.sy[gj];
+      for (int v = 0; v < 4; v++) {
+        const double mu = a.mu[gj * 4 + v];
+        phF[v] += aF * mu;
+        phG[v] += aG * mu;
+      }
+    }
+    double wi[4], lam[4] = {0, 0, 0, 0};
+    for (int v = 0; v < 4; v++) wi[v] = a.w[it * 4 + v];
+    euler_flux_vjp(ph.gam, wi, phF, phG, lam);
+    // face contributions that read this node
+    for (int s = 0; s < 2; s++) {
+      const int m = node_slots[2 * i + s];
+      if (m < 0) continue;
+      for (int v = 0; v < 4; v++) lam[v] += a.gmq[(c * NF + m) * 4 + v];
+      const int code = a.fmap[c * 3 + m / Nfp];
+      if (code >= 0) {
+        const size_t nb = code >> 2;
+        const int mp = (code & 3) * Nfp + (Nfp - 1 - m % Nfp);
+        for (int v = 0; v < 4; v++) lam[v] += a.gpq[(nb * NF + mp) * 4 + v];
+      }
+    }
+    for (int v = 0; v < 4; v++) a.acc[it * 4 + v] += a.scale * lam[v];
+    // metric cotangents: d/drx[i] = mu[i] . (Drw F)[i], etc.
+    double dFdr[4] = {0, 0, 0, 0}, dFds[4] = {0, 0, 0, 0}, dGdr[4] = {0, 0, 0, 0}, dGds[4] = {0, 0, 0, 0};
+    for (int k = 0; k < Np; k++) {
+      double q[4], F[4], G[4];
+      for (int v = 0; v < 4; v++) q[v] = a.w[(c * Np + k) * 4 + v];
+      euler_flux(ph.gam, q, F, G);
+      const double ar = Drw[i * Np + k], as = Dsw[i * Np + k];
+      for (int v = 0; v < 4; v++) {
+        dFdr[v] += ar * F[v]; dFds[v] += as * F[v]; dGdr[v] += ar * G[v]; dGds[v] += as * G[v];
+      }
+    }
+    double grx = 0, gsx = 0, gry = 0, gsy = 0;
+    for (int v = 0; v < 4; v++) {
+      const double mu = a.mu[it * 4 + v];
+      grx += mu * dFdr[v]; gsx += mu * dFds[v]; gry += mu * dGdr[v]; gsy += mu * dGds[v];
+    }
+    a.g_rx[it] += a.scale * grx;
+    a.g_sx[it] += a.scale * gsx;
+    a.g_ry[it] += a.scale * gry;
+    a.g_sy[it] += a.scale * gsy;
+  }
+}
+
+__global__ void lincomb_kernel(size_t n, double a, const double *__restrict__ x, double b,
+                               const double *__restrict__ y, double *z) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    z[i] = a * x[i] + (y ? b * y[i] : 0.0);
+}
+
+__global__ void record_dt_kernel(const unsigned long long *dt1, double cfl, double *dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = cfl / __longlong_as_double((long long)*dt1);
+}
+
+static int grid_of(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 16384) g = 16384;
+  return g < 1 ? 1 : (int)g;
+}
+
+template <int N>
+static hipError_t tlm_n(const Launch &L, const TlmArgs &a, double bcud, double bcvd) {
+  const size_t nf = (size_t)a.ncells * Ord<N>::NF, nc = (size_t)a.ncells * Ord<N>::Np;
+  hipLaunchKernelGGL(tlm_face_kernel<N>, dim3(grid_of(nf)), dim3(256), 0, L.stream, a, L.ph, bcud, bcvd);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(tlm_cell_kernel<N>, dim3(grid_of(nc)), dim3(256), 0, L.stream, a, L.ph, L.ops);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t adj_n(const Launch &L, const AdjArgs &a, const int *node_slots) {
+  const size_t nf = (size_t)a.ncells * Ord<N>::NF, nc = (size_t)a.ncells * Ord<N>::Np;
+  hipLaunchKernelGGL(adj_face_kernel<N>, dim3(grid_of(nf)), dim3(256), 0, L.stream, a, L.ph, L.ops);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(adj_cell_kernel<N>, dim3(grid_of(nc)), dim3(256), 0, L.stream, a, L.ph, L.ops,
+                     node_slots);
+  return hipGetLastError();
+}
+
+hipError_t launch_rhs_tlm(const Launch &L, const TlmArgs &a, double bcud, double bcvd) {
+  if (a.ncells == 0) return hipSuccess;
+  switch (L.N) {
+    case 1: return tlm_n<1>(L, a, bcud, bcvd);
+    case 2: return tlm_n<2>(L, a, bcud, bcvd);
+    case 3: return tlm_n<3>(L, a, bcud, bcvd);
+    case 4: return tlm_n<4>(L, a, bcud, bcvd);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_rhs_adj(const Launch &L, const AdjArgs &a, const int *node_slots) {
+  if (a.ncells == 0) return hipSuccess;
+  switch (L.N) {
+    case 1: return adj_n<1>(L, a, node_slots);
+    case 2: return adj_n<2>(L, a, node_slots);
+    case 3: return adj_n<3>(L, a, node_slots);
+    case 4: return adj_n<4>(L, a, node_slots);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_lincomb(const Launch &L, size_t n, double a, const double *x, double b,
+                          const double *y, double *z) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(lincomb_kernel, dim3(grid_of(n)), dim3(256), 0, L.stream, n, a, x, b, y, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_record_dt(const Launch &L, const unsigned long long *dt1, double cfl, double *dst) {
+  hipLaunchKernelGGL(record_dt_kernel, dim3(1), dim3(64), 0, L.stream, dt1, cfl, dst);
+  return hipGetLastError();
+}
+
+}  // namespace dg

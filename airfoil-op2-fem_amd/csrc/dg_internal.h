@@ -51,6 +51,10 @@ struct StageArgs {
   int ncases;
   const int *cell_off, *tile_off;
   const Phys *phys;
+  // Frozen time step (per case): when non-null the stage updates use
+  // dt_fixed[case] instead of cfl / dt1 (sensitivity runs replay a recorded
+  // dt sequence).
+  const double *dt_fixed;
 };
 
 // Cells per tile of the matrix-core stage kernel (one wave per tile).

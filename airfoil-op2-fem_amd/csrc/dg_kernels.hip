@@ -52,7 +52,7 @@ __global__ __launch_bounds__(NT) void rhs_kernel(StageArgs a, const double *__re
   const double *sDrw = sOp, *sDsw = sOp + Np * Np, *sLIFT = sOp + 2 * Np * Np;
 
   double dt = 0.0;
-  if (EPI != EPI_RHS) dt = a.cfl / __longlong_as_double((long long)*a.dt1_cur);
+  if (EPI != EPI_RHS) dt = a.dt_fixed ? *a.dt_fixed : a.cfl / __longlong_as_double((long long)*a.dt1_cur);
   if (EPI == EPI_STAGE0 && blockIdx.x == 0 && tid == 0) *a.dt1_next = 0ull;
   unsigned long long cfl_bits = 0ull;
   bool bad = false;

@@ -70,8 +70,28 @@ double naca0012_thickness(double x) {
                         0.105174696 * x * x * x * x);
 }
 
+double hicks_henne(double x, double xpk, double width) {
+  if (x <= 0.0 || x >= 1.0) return 0.0;
+  const double e = std::log(0.5) / std::log(xpk);
+  return std::pow(std::sin(M_PI * std::pow(x, e)), width);
+}
+
+double ogrid_surface_x(int kc, int ncirc) {
+  return 0.5 + 0.5 * std::cos(2.0 * M_PI * kc / ncirc);
+}
+
+int ogrid_surface_side(int kc, int ncirc) {
+  if (kc == 0 || 2 * kc == ncirc) return 0;
+  return 2 * kc < ncirc ? 1 : -1;
+}
+
 bool gen_ogrid(int ncirc, int nrad, double stretch, double rfar, Mesh &m,
                std::string &err) {
+  return gen_ogrid_shaped(ncirc, nrad, stretch, rfar, nullptr, m, err);
+}
+
+bool gen_ogrid_shaped(int ncirc, int nrad, double stretch, double rfar,
+                      const std::vector<double> *surf_dy, Mesh &m, std::string &err) {
   if (ncirc < 8 || nrad < 1 || stretch <= 0.0 || rfar <= 1.0) {
     err = "invalid O-grid parameters";
     return false;
@@ -85,6 +105,12 @@ bool gen_ogrid(int ncirc, int nrad, double stretch, double rfar, Mesh &m,
   m.nnodes = ncirc * nr1;
   m.coords.resize(2 * (size_t)m.nnodes);
   m.node_tag.assign(m.nnodes, 0);
+  m.node_k.assign(m.nnodes, 0);
+  m.node_w.assign(m.nnodes, 0.0);
+  if (surf_dy && (int)surf_dy->size() != ncirc) {
+    err = "surface offsets need one value per circumferential node";
+    return false;
+  }
   // radial blending parameter: geometric growth by `stretch` per cell
   std::vector<double> rho(nr1);
   double total = 0.0, h = 1.0;
@@ -100,12 +126,15 @@ bool gen_ogrid(int ncirc, int nrad, double stretch, double rfar, Mesh &m,
     double ys = naca0012_thickness(xs);
     if (phi > M_PI) ys = -ys;
     if (kc == 0 || 2 * kc == ncirc) ys = 0.0;
+    if (surf_dy) ys += (*surf_dy)[kc];
     double xf = 0.5 + rfar * std::cos(phi), yf = rfar * std::sin(phi);
     for (int j = 0; j <= nrad; j++) {
       size_t n = (size_t)j * ncirc + kc;
       m.coords[2 * n] = xs + (xf - xs) * rho[j];
       m.coords[2 * n + 1] = ys + (yf - ys) * rho[j];
       m.node_tag[n] = j == 0 ? 1 : (j == nrad ? 2 : 0);
+      m.node_k[n] = kc;
+      m.node_w[n] = 1.0 - rho[j];  // d(node y)/d(surface y offset)
     }
   }
   m.ncells = 2 * ncirc * nrad;

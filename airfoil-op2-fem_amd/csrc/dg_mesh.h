@@ -26,6 +26,8 @@ struct Mesh {
   std::vector<double> cell_xy;   // [ncells][3][2] vertex coordinates per cell
                                  // (unwrapped across periodic seams)
   std::vector<int> node_tag;     // per node: 0 interior, 1 wall ring, 2 far-field ring
+  std::vector<int> node_k;       // O-grids: circumferential index of the node's radial line
+  std::vector<double> node_w;    // O-grids: d(node y) / d(surface y offset of its line)
   int nedges = 0, nbedges = 0;
   std::vector<int> edge2nodes, edge2cells, edgeNum;      // [nedges][2]
   std::vector<int> bedge2nodes;                          // [nbedges][2]
@@ -49,6 +51,18 @@ bool gen_periodic(int nx, int ny, double x0, double y0, double Lx, double Ly,
 // Inner ring edges are walls (bc 2), outer ring edges far field (bc 3).
 bool gen_ogrid(int ncirc, int nrad, double stretch, double rfar, Mesh &m,
                std::string &err);
+
+// The same O-grid with the surface points of line k moved by surf_dy[k] in
+// y; interior nodes of line k move by (1 - rho_j) surf_dy[k] (rho_j the
+// radial blending parameter), the far-field ring stays fixed.
+bool gen_ogrid_shaped(int ncirc, int nrad, double stretch, double rfar,
+                      const std::vector<double> *surf_dy, Mesh &m, std::string &err);
+// Chord position of the surface point of radial line kc, and its side:
+// +1 upper surface, -1 lower, 0 leading or trailing edge.
+double ogrid_surface_x(int kc, int ncirc);
+int ogrid_surface_side(int kc, int ncirc);
+// Hicks-Henne bump sin(pi x^(ln 0.5 / ln xpk))^width on (0, 1), peak 1 at xpk.
+double hicks_henne(double x, double xpk, double width);
 
 // NACA0012 half thickness at chord position x in [0,1] (naca0012.geo:14).
 double naca0012_thickness(double x);

@@ -1,0 +1,785 @@
+// Lift sensitivities of the airfoil case: dC_L/d(alpha) and dC_L/d(bump
+// amplitudes) by a hand-written discrete adjoint of the SSPRK3 solver, with a
+// tangent-linear run and frozen-dt forward evaluations (for finite
+// differences) beside it.  No reference counterpart.
+//
+// Problem: O-grid around NACA0012 whose surface carries Hicks-Henne bumps
+// (nbu on the upper surface, nbl on the lower, peaks at evenly spaced chord
+// positions), free stream (Mach, alpha); the objective is C_L of the state
+// after nsteps SSPRK3 steps.  dt is frozen to the sequence recorded by the
+// first forward run (the CFL max is not differentiable), so the forward,
+// tangent and adjoint differentiate the same discrete map.
+//
+// Adjoint of one step in the solver's Shu-Osher form
+//   w1 = q + dt L(q), w2 = 3/4 q + 1/4 (w1 + dt L(w1)), q' = 1/3 q + 2/3 (w2 + dt L(w2)):
+//   M = 2/3 lam'; A = M + dt L'(w2)^T M;  lam = 1/3 lam' + 3/4 A;
+//   M = 1/4 A;    B = M + dt L'(w1)^T M;  lam += B + dt L'(q)^T B,
+// with the geometry and free-stream cotangents accumulated alongside.  The
+// states q, w1, w2 of each step are recomputed from checkpoints stored every
+// ckpt steps (one level of revolve-style recomputation).
+//
+// Geometry chain: bump amplitudes -> surface offsets -> node y (linear,
+// gen_ogrid_shaped) -> cell vertices -> init_grid factors (rx, ry, sx, sy,
+// nx, ny, fscale, J), the last step differentiated on the host in dual
+// arithmetic, one direction per vertex coordinate.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dg_euler.h"
+#include "dg_adjoint.h"
+#include "dg_ctx.h"
+#include "dg_dual.h"
+#include "dg_internal.h"
+#include "dg_mesh.h"
+#include "dg_ops.h"
+
+using dg::fail;
+using dg::hipfail;
+
+namespace {
+
+// init_grid of one cell in generic arithmetic, as init_grid_kernel
+// (src/init_grid.h:1-106).
+template <class T>
+void init_grid_cell(const dg_ctx *c, const T v[6], T *rx, T *ry, T *sx, T *sy, T *nx, T *ny, T *fs,
+                    T &Jc) {
+  const int Np = c->Np, Nfp = c->Nfp, NF = c->NF;
+  std::vector<T> X(Np), Y(Np), XR(Np), XS(Np), YR(Np), YS(Np), J(Np);
+  for (int d = 0; d < 2; d++) {
+    std::vector<T> &o = d == 0 ? X : Y;
+    T a1 = 0.5 * v[2 + d], a2 = 0.5 * v[4 + d], a0 = -0.5 * v[d];
+    for (int i = 0; i < Np; i++) o[i] = a1 * (1.0 + c->r[i]) + a2 * (1.0 + c->s[i]) + a0 * (c->s[i] + c->r[i]);
+  }
+  for (int j = 0; j < Np; j++) {
+    T a = 0.0, b = 0.0, cc = 0.0, dd = 0.0;
+    for (int i = 0; i < Np; i++) {
+      a += c->Dr[j * Np + i] * X[i];
+      b += c->Ds[j * Np + i] * X[i];
+      cc += c->Dr[j * Np + i] * Y[i];
+      dd += c->Ds[j * Np + i] * Y[i];
+    }
+    XR[j] = a; XS[j] = b; YR[j] = cc; YS[j] = dd;
+  }
+  for (int i = 0; i < Np; i++) {
+    J[i] = -XS[i] * YR[i] + XR[i] * YS[i];
+    rx[i] = YS[i] / J[i]; sx[i] = -YR[i] / J[i]; ry[i] = -XS[i] / J[i]; sy[i] = XR[i] / J[i];
+  }
+  Jc = J[0];
+  for (int m = 0; m < NF; m++) {
+    const int f = c->FMASK[m], face = m / Nfp;
+    T a, b;
+    if (face == 0) { a = YR[f]; b = -XR[f]; }
+    else if (face == 1) { a = YS[f] - YR[f]; b = XR[f] - XS[f]; }
+    else { a = -YS[f]; b = XS[f]; }
+    T sJ = sqrt(a * a + b * b);
+    nx[m] = a / sJ; ny[m] = b / sJ; fs[m] = sJ / J[f];
+  }
+}
+
+struct DevBuf {
+  std::vector<void *> ptrs;
+  hipError_t err = hipSuccess;
+  template <class T>
+  T *alloc(size_t n) {
+    T *p = nullptr;
+    if (err == hipSuccess) err = hipMalloc(&p, sizeof(T) * (n ? n : 1));
+    if (err == hipSuccess) {
+      ptrs.push_back(p);
+      if (n) err = hipMemset(p, 0, sizeof(T) * n);
+    }
+    return p;
+  }
+  ~DevBuf() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+struct dg_sens {
+  dg_ctx *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  int ncirc = 0, nrad = 0, nbu = 0, nbl = 0;
+  double stretch = 1.0, rfar = 20.0, width = 3.0;
+  std::vector<double> xpk;     // peak positions of the bumps (upper ones first)
+  std::vector<int> side;       // +1 upper, -1 lower
+  double mach = 0.4, alpha = 0.0;
+  std::vector<double> bumps;   // current shape
+  dg::Mesh mesh;
+  int K = 0, Np = 0, NF = 0, Nfp = 0;
+  DevBuf buf;
+  // device geometry and maps
+  double *cell_xy, *nodeX, *nodeY, *x, *y, *xr, *yr, *xs, *ys, *rx, *ry, *sx, *sy, *nx, *ny, *fs, *J;
+  int *fmap, *node_slots, *cell_off, *tile_off;
+  dg::Phys *phys_d;
+  unsigned long long *dt1;
+  double *tdev;
+  unsigned int *flag;
+  // states and work buffers
+  double *q, *wA, *wB, *qw;
+  double *lam, *lamq, *M, *A, *B, *gmq, *gpq;
+  double *Ld, *fluxd, *qd, *w1d, *w2d, *tmp;
+  double *g_rx, *g_ry, *g_sx, *g_sy, *g_nx, *g_ny, *g_fs, *g_bcu, *g_bcv;
+  double *rxd, *ryd, *sxd, *syd, *nxd, *nyd, *fsd;
+  // walls
+  int nwall = 0;
+  std::vector<int> h_wcell, h_wface;
+  int *wcell, *wface;
+  double *facew, *wforce, *wcp;
+  std::vector<double> h_facew;
+  // recorded time steps
+  std::vector<double> h_dts;
+  double *dts = nullptr;
+  int nsteps = 0;
+  double cfl = 1.0;
+  double fwd_ms = 0.0, grad_ms = 0.0, tlm_ms = 0.0;
+  std::vector<double *> ckpt, seg;
+};
+
+namespace {
+
+dg::Launch launch_a(const dg_sens *s) {
+  dg::Launch L = dg::launch_of(s->ctx);
+  L.stream = s->stream;
+  L.ph = dg::freestream(s->ctx->gam, s->mach, s->alpha, s->ctx->bc_p, s->ctx->bc_r);
+  return L;
+}
+
+std::vector<double> surface_offsets(const dg_sens *s, const std::vector<double> &b) {
+  std::vector<double> dy(s->ncirc, 0.0);
+  for (int k = 0; k < s->ncirc; k++) {
+    const double xs = dg::ogrid_surface_x(k, s->ncirc);
+    const int sd = dg::ogrid_surface_side(k, s->ncirc);
+    for (size_t i = 0; i < b.size(); i++)
+      if (s->side[i] == sd) dy[k] += b[i] * dg::hicks_henne(xs, s->xpk[i], s->width);
+  }
+  return dy;
+}
+
+// Builds the shaped mesh for the amplitudes b and refreshes the device
+// geometry (init_grid on the device, as the solver does).
+int set_shape(dg_sens *s, const std::vector<double> &b) {
+  dg_ctx *c = s->ctx;
+  std::vector<double> dy = surface_offsets(s, b);
+  std::string err;
+  if (!dg::gen_ogrid_shaped(s->ncirc, s->nrad, s->stretch, s->rfar, &dy, s->mesh, err))
+    return fail(c, DG_ERR_MESH, "dg_sens: " + err);
+  s->bumps = b;
+  HIPCHECK(c, hipMemcpyAsync(s->cell_xy, s->mesh.cell_xy.data(), sizeof(double) * 6 * s->K,
+                             hipMemcpyHostToDevice, s->stream));
+  HIPCHECK(c, dg::launch_init_grid(launch_a(s), s->K, s->cell_xy, s->nodeX, s->nodeY, s->x, s->y,
+                                   s->xr, s->yr, s->xs, s->ys, s->rx, s->ry, s->sx, s->sy, s->nx,
+                                   s->ny, s->fs, s->J));
+  HIPCHECK(c, hipStreamSynchronize(s->stream));
+  return DG_OK;
+}
+
+dg::StageArgs stage_args(dg_sens *s) {
+  dg::StageArgs a{};
+  a.rx = s->rx; a.ry = s->ry; a.sx = s->sx; a.sy = s->sy;
+  a.fscale = s->fs; a.nx = s->nx; a.ny = s->ny; a.fmap = s->fmap;
+  a.cfl = s->cfl; a.t_dev = s->tdev; a.flag = s->flag;
+  a.ncells = s->K; a.ncases = 1; a.cell_off = s->cell_off; a.tile_off = s->tile_off;
+  a.phys = s->phys_d;
+  return a;
+}
+
+// One SSPRK3 step from qin (in place) with w1 -> wA, w2 -> wB; frozen dt
+// dts[n] unless recording (then dt = cfl / dt1 and dts[n] is recorded).
+hipError_t step(dg_sens *s, double *qin, int n, bool record) {
+  dg::Launch L = launch_a(s);
+  dg::StageArgs a = stage_args(s);
+  const int p = n & 1;
+  a.dt1_cur = s->dt1 + p;
+  a.dt1_next = s->dt1 + (p ^ 1);
+  hipError_t e = hipSuccess;
+  if (record) e = dg::launch_record_dt(L, a.dt1_cur, s->cfl, s->dts + n);
+  else a.dt_fixed = s->dts + n;
+  if (e != hipSuccess) return e;
+  a.q = qin;
+  a.w = qin; a.out = s->wA;
+  if ((e = dg::launch_rhs(L, a, true, dg::EPI_STAGE0)) != hipSuccess) return e;
+  a.w = s->wA; a.out = s->wB;
+  if ((e = dg::launch_rhs(L, a, true, dg::EPI_STAGE1)) != hipSuccess) return e;
+  a.w = s->wB; a.out = qin;
+  return dg::launch_rhs(L, a, true, dg::EPI_STAGE2);
+}
+
+// w1 -> wA and w2 -> wB of step n from q (q unchanged)
+hipError_t stages01(dg_sens *s, const double *q, int n) {
+  dg::Launch L = launch_a(s);
+  dg::StageArgs a = stage_args(s);
+  a.dt1_cur = s->dt1 + (n & 1);
+  a.dt1_next = s->dt1 + ((n & 1) ^ 1);
+  a.dt_fixed = s->dts + n;
+  a.q = q;
+  a.w = q; a.out = s->wA;
+  hipError_t e = dg::launch_rhs(L, a, true, dg::EPI_STAGE0);
+  if (e != hipSuccess) return e;
+  a.w = s->wA; a.out = s->wB;
+  return dg::launch_rhs(L, a, true, dg::EPI_STAGE1);
+}
+
+int set_ic(dg_sens *s, double *q) {
+  HIPCHECK(s->ctx, dg::launch_set_ic(launch_a(s), s->K, q, s->wA));
+  return DG_OK;
+}
+
+// Pressure force (Fx, Fy) of the current state q (device kernel, host sum).
+int forces(dg_sens *s, const double *q, double *fx, double *fy) {
+  dg_ctx *c = s->ctx;
+  dg::Launch L = launch_a(s);
+  const double vinf2 = L.ph.bc_u * L.ph.bc_u + L.ph.bc_v * L.ph.bc_v;
+  const double qdyn = 0.5 * L.ph.bc_r * vinf2;
+  HIPCHECK(c, dg::launch_wall_forces(L, s->nwall, s->wcell, s->wface, q, s->nx, s->ny, s->fs, s->J,
+                                     s->facew, L.ph.bc_p, qdyn, s->wforce, s->wcp));
+  std::vector<double> f(2 * (size_t)s->nwall);
+  HIPCHECK(c, hipMemcpyAsync(f.data(), s->wforce, sizeof(double) * f.size(), hipMemcpyDeviceToHost, s->stream));
+  HIPCHECK(c, hipStreamSynchronize(s->stream));
+  double a = 0.0, b = 0.0;
+  for (int w = 0; w < s->nwall; w++) { a += f[2 * w]; b += f[2 * w + 1]; }
+  *fx = a; *fy = b;
+  return DG_OK;
+}
+
+double qdyn_of(const dg_sens *s) {
+  dg::Phys ph = dg::freestream(s->ctx->gam, s->mach, s->alpha, s->ctx->bc_p, s->ctx->bc_r);
+  return 0.5 * ph.bc_r * (ph.bc_u * ph.bc_u + ph.bc_v * ph.bc_v);
+}
+
+double lift_of(const dg_sens *s, double fx, double fy) {
+  const double al = s->alpha * M_PI / 180.0;
+  return (fy * std::cos(al) - fx * std::sin(al)) / qdyn_of(s);
+}
+
+int run_forward(dg_sens *s, bool record, int ckpt_every, double *cl) {
+  dg_ctx *c = s->ctx;
+  HIPCHECK(c, hipMemsetAsync(s->tdev, 0, sizeof(double), s->stream));
+  if (set_ic(s, s->q) != DG_OK) return DG_ERR_HIP;
+  if (record) {
+    HIPCHECK(c, hipMemsetAsync(s->dt1, 0, 2 * sizeof(unsigned long long), s->stream));
+    HIPCHECK(c, dg::launch_calc_dt(launch_a(s), s->K, s->q, s->fs, s->dt1));
+  }
+  for (int n = 0; n < s->nsteps; n++) {
+    if (ckpt_every > 0 && n % ckpt_every == 0)
+      HIPCHECK(c, hipMemcpyAsync(s->ckpt[n / ckpt_every], s->q, sizeof(double) * (size_t)s->K * s->Np * 4,
+                                 hipMemcpyDeviceToDevice, s->stream));
+    HIPCHECK(c, step(s, s->q, n, record));
+  }
+  double fx, fy;
+  int rc = forces(s, s->q, &fx, &fy);
+  if (rc != DG_OK) return rc;
+  *cl = lift_of(s, fx, fy);
+  return DG_OK;
+}
+
+// Tangent of C_L (host, dual arithmetic) and its cotangent: q at the wall
+// cells, geometry of the wall slots and of J.
+struct WallData {
+  std::vector<double> q, nx, ny, fs, J;
+};
+
+int wall_data(dg_sens *s, const double *q, WallData &d) {
+  dg_ctx *c = s->ctx;
+  const size_t K = s->K, Np = s->Np, NF = s->NF;
+  d.q.resize(K * Np * 4); d.nx.resize(K * NF); d.ny.resize(K * NF); d.fs.resize(K * NF); d.J.resize(K);
+  HIPCHECK(c, hipStreamSynchronize(s->stream));
+  HIPCHECK(c, hipMemcpy(d.q.data(), q, sizeof(double) * d.q.size(), hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(d.nx.data(), s->nx, sizeof(double) * d.nx.size(), hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(d.ny.data(), s->ny, sizeof(double) * d.ny.size(), hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(d.fs.data(), s->fs, sizeof(double) * d.fs.size(), hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(d.J.data(), s->J, sizeof(double) * d.J.size(), hipMemcpyDeviceToHost));
+  return DG_OK;
+}
+
+// Per-cell Jacobian of the init_grid factors with respect to the 6 vertex
+// coordinates; out[d] holds the derivative arrays for direction d.
+struct CellGeoJac {
+  std::vector<double> rx, ry, sx, sy, nx, ny, fs, J;  // [6][Np] / [6][NF] / [6]
+};
+
+void cell_geo_jac(const dg_sens *s, const double *v, CellGeoJac &jac) {
+  const dg_ctx *c = s->ctx;
+  const int Np = s->Np, NF = s->NF;
+  jac.rx.assign(6 * Np, 0.0); jac.ry = jac.rx; jac.sx = jac.rx; jac.sy = jac.rx;
+  jac.nx.assign(6 * NF, 0.0); jac.ny = jac.nx; jac.fs = jac.nx; jac.J.assign(6, 0.0);
+  std::vector<dg::Dual> rx(Np), ry(Np), sx(Np), sy(Np), nx(NF), ny(NF), fs(NF);
+  for (int d = 0; d < 6; d++) {
+    dg::Dual vd[6];
+    for (int k = 0; k < 6; k++) vd[k] = dg::Dual(v[k], k == d ? 1.0 : 0.0);
+    dg::Dual Jc;
+    init_grid_cell<dg::Dual>(c, vd, rx.data(), ry.data(), sx.data(), sy.data(), nx.data(), ny.data(), fs.data(), Jc);
+    for (int i = 0; i < Np; i++) {
+      jac.rx[d * Np + i] = rx[i].d; jac.ry[d * Np + i] = ry[i].d;
+      jac.sx[d * Np + i] = sx[i].d; jac.sy[d * Np + i] = sy[i].d;
+    }
+    for (int m = 0; m < NF; m++) {
+      jac.nx[d * NF + m] = nx[m].d; jac.ny[d * NF + m] = ny[m].d; jac.fs[d * NF + m] = fs[m].d;
+    }
+    jac.J[d] = Jc.d;
+  }
+}
+
+// d(node y)/d(bump i) for every node (x does not move)
+double node_dy_db(const dg_sens *s, int node, int i) {
+  const int k = s->mesh.node_k[node];
+  if (dg::ogrid_surface_side(k, s->ncirc) != s->side[i]) return 0.0;
+  return s->mesh.node_w[node] * dg::hicks_henne(dg::ogrid_surface_x(k, s->ncirc), s->xpk[i], s->width);
+}
+
+}  // namespace
+
+extern "C" {
+
+void dg_sens_destroy(dg_sens *s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (double *p : s->ckpt) (void)hipFree(p);
+  for (double *p : s->seg) (void)hipFree(p);
+  if (s->dts) (void)hipFree(s->dts);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, int nbu, int nbl,
+                   double mach, double alpha_deg, dg_sens **out) {
+  if (!c || !out) return fail(c, DG_ERR_ARG, "dg_sens_create: null argument");
+  *out = nullptr;
+  if (nbu < 0 || nbl < 0 || nbu + nbl > 64 || !(mach > 0.0))
+    return fail(c, DG_ERR_ARG, "dg_sens_create: need 0..64 bumps and mach > 0");
+  dg_sens *s = new dg_sens();
+  s->ctx = c;
+  s->ncirc = ncirc; s->nrad = nrad; s->stretch = stretch; s->rfar = rfar;
+  s->nbu = nbu; s->nbl = nbl; s->mach = mach; s->alpha = alpha_deg;
+  for (int i = 0; i < nbu; i++) { s->xpk.push_back((i + 1.0) / (nbu + 1.0)); s->side.push_back(1); }
+  for (int i = 0; i < nbl; i++) { s->xpk.push_back((i + 1.0) / (nbl + 1.0)); s->side.push_back(-1); }
+  std::string err;
+  std::vector<double> zero(ncirc, 0.0);
+  if (!dg::gen_ogrid_shaped(ncirc, nrad, stretch, rfar, &zero, s->mesh, err)) {
+    delete s;
+    return fail(c, DG_ERR_MESH, "dg_sens_create: " + err);
+  }
+  const dg::Mesh &M = s->mesh;
+  std::vector<int> fmap;
+  if (!dg::build_face_map(M.ncells, M.nedges, M.edge2cells.data(), M.edgeNum.data(), M.nbedges,
+                          M.bedge2cells.data(), M.bedgeNum.data(), M.bedge_type.data(),
+                          M.cells.data(), fmap, err)) {
+    delete s;
+    return fail(c, DG_ERR_MESH, "dg_sens_create: " + err);
+  }
+  s->K = M.ncells; s->Np = c->Np; s->NF = c->NF; s->Nfp = c->Nfp;
+  const size_t K = s->K, Np = s->Np, NF = s->NF, ST = K * Np * 4, FT = K * NF * 4;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete s; return hipfail(c, e, "dg_sens_create: stream"); }
+  DevBuf &b = s->buf;
+  s->cell_xy = b.alloc<double>(K * 6); s->nodeX = b.alloc<double>(K * 3); s->nodeY = b.alloc<double>(K * 3);
+  s->x = b.alloc<double>(K * Np); s->y = b.alloc<double>(K * Np);
+  s->xr = b.alloc<double>(K * Np); s->yr = b.alloc<double>(K * Np);
+  s->xs = b.alloc<double>(K * Np); s->ys = b.alloc<double>(K * Np);
+  s->rx = b.alloc<double>(K * Np); s->ry = b.alloc<double>(K * Np);
+  s->sx = b.alloc<double>(K * Np); s->sy = b.alloc<double>(K * Np);
+  s->nx = b.alloc<double>(K * NF); s->ny = b.alloc<double>(K * NF); s->fs = b.alloc<double>(K * NF);
+  s->J = b.alloc<double>(K);
+  s->fmap = b.alloc<int>(fmap.size());
+  s->node_slots = b.alloc<int>(2 * Np);
+  s->cell_off = b.alloc<int>(2); s->tile_off = b.alloc<int>(2);
+  s->phys_d = b.alloc<dg::Phys>(1);
+  s->dt1 = b.alloc<unsigned long long>(2);
+  s->tdev = b.alloc<double>(1);
+  s->flag = b.alloc<unsigned int>(1);
+  s->q = b.alloc<double>(ST); s->wA = b.alloc<double>(ST); s->wB = b.alloc<double>(ST); s->qw = b.alloc<double>(ST);
+  s->lam = b.alloc<double>(ST); s->lamq = b.alloc<double>(ST); s->M = b.alloc<double>(ST);
+  s->A = b.alloc<double>(ST); s->B = b.alloc<double>(ST);
+  s->gmq = b.alloc<double>(FT); s->gpq = b.alloc<double>(FT);
+  s->Ld = b.alloc<double>(ST); s->fluxd = b.alloc<double>(FT); s->qd = b.alloc<double>(ST);
+  s->w1d = b.alloc<double>(ST); s->w2d = b.alloc<double>(ST); s->tmp = b.alloc<double>(ST);
+  s->g_rx = b.alloc<double>(K * Np); s->g_ry = b.alloc<double>(K * Np);
+  s->g_sx = b.alloc<double>(K * Np); s->g_sy = b.alloc<double>(K * Np);
+  s->g_nx = b.alloc<double>(K * NF); s->g_ny = b.alloc<double>(K * NF); s->g_fs = b.alloc<double>(K * NF);
+  s->g_bcu = b.alloc<double>(K * NF); s->g_bcv = b.alloc<double>(K * NF);
+  s->rxd = b.alloc<double>(K * Np); s->ryd = b.alloc<double>(K * Np);
+  s->sxd = b.alloc<double>(K * Np); s->syd = b.alloc<double>(K * Np);
+  s->nxd = b.alloc<double>(K * NF); s->nyd = b.alloc<double>(K * NF); s->fsd = b.alloc<double>(K * NF);
+  for (int bb = 0; bb < M.nbedges; bb++)
+    if (M.bedge_type[bb] == DG_BC_WALL) { s->h_wcell.push_back(M.bedge2cells[bb]); s->h_wface.push_back(M.bedgeNum[bb]); }
+  s->nwall = (int)s->h_wcell.size();
+  s->wcell = b.alloc<int>(s->nwall); s->wface = b.alloc<int>(s->nwall);
+  s->facew = b.alloc<double>(s->Nfp); s->wforce = b.alloc<double>(2 * (size_t)s->nwall);
+  s->wcp = b.alloc<double>((size_t)s->nwall * s->Nfp);
+  e = b.err;
+  dg::Operators o;
+  const char *oe = nullptr;
+  dg::build_operators(c->N, o, &oe);
+  s->h_facew.assign(s->Nfp, 0.0);
+  for (int i = 0; i < s->Nfp; i++)
+    for (int j = 0; j < s->Nfp; j++) s->h_facew[i] += o.face_mass[i * s->Nfp + j];
+  std::vector<int> slots(2 * Np, -1);
+  for (int m = 0; m < (int)NF; m++) {
+    const int nd = c->FMASK[m];
+    if (slots[2 * nd] < 0) slots[2 * nd] = m; else slots[2 * nd + 1] = m;
+  }
+  const int coff[2] = {0, (int)K}, toff[2] = {0, (int)((K + dg::kMfmaTile - 1) / dg::kMfmaTile)};
+  const dg::Phys ph = dg::freestream(c->gam, mach, alpha_deg, c->bc_p, c->bc_r);
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  up(s->fmap, fmap.data(), sizeof(int) * fmap.size());
+  up(s->node_slots, slots.data(), sizeof(int) * slots.size());
+  up(s->cell_off, coff, sizeof(coff));
+  up(s->tile_off, toff, sizeof(toff));
+  up(s->phys_d, &ph, sizeof(ph));
+  up(s->wcell, s->h_wcell.data(), sizeof(int) * s->nwall);
+  up(s->wface, s->h_wface.data(), sizeof(int) * s->nwall);
+  up(s->facew, s->h_facew.data(), sizeof(double) * s->Nfp);
+  if (e != hipSuccess) {
+    dg_sens_destroy(s);
+    return hipfail(c, e, "dg_sens_create: allocation");
+  }
+  std::vector<double> b0(nbu + nbl, 0.0);
+  int rc = set_shape(s, b0);
+  if (rc != DG_OK) { dg_sens_destroy(s); return rc; }
+  *out = s;
+  return DG_OK;
+}
+
+int dg_sens_record(dg_sens *s, const double *bumps, int nsteps, double cfl, double *cl) {
+  if (!s || nsteps < 1 || !(cfl > 0.0) || !cl) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_sens_record: bad argument");
+  dg_ctx *c = s->ctx;
+  std::vector<double> b(s->nbu + s->nbl, 0.0);
+  if (bumps) b.assign(bumps, bumps + b.size());
+  int rc = set_shape(s, b);
+  if (rc != DG_OK) return rc;
+  s->nsteps = nsteps;
+  s->cfl = cfl;
+  if (s->dts) (void)hipFree(s->dts);
+  s->dts = nullptr;
+  HIPCHECK(c, hipMalloc(&s->dts, sizeof(double) * nsteps));
+  auto t0 = std::chrono::steady_clock::now();
+  rc = run_forward(s, true, 0, cl);
+  if (rc != DG_OK) return rc;
+  s->fwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  s->h_dts.resize(nsteps);
+  HIPCHECK(c, hipMemcpy(s->h_dts.data(), s->dts, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
+  return DG_OK;
+}
+
+int dg_sens_eval(dg_sens *s, double alpha_deg, const double *bumps, double *cl) {
+  if (!s || !cl || s->nsteps < 1) return fail(s ? s->ctx : nullptr, DG_ERR_STATE, "dg_sens_eval: record first");
+  const double a0 = s->alpha;
+  const std::vector<double> b0 = s->bumps;
+  std::vector<double> b(s->nbu + s->nbl, 0.0);
+  if (bumps) b.assign(bumps, bumps + b.size());
+  s->alpha = alpha_deg;
+  int rc = set_shape(s, b);
+  if (rc == DG_OK) rc = run_forward(s, false, 0, cl);
+  s->alpha = a0;
+  int rc2 = set_shape(s, b0);
+  return rc != DG_OK ? rc : rc2;
+}
+
+int dg_sens_tangent(dg_sens *s, double dalpha, const double *dbumps, double *cl, double *dcl) {
+  if (!s || !dcl || s->nsteps < 1) return fail(s ? s->ctx : nullptr, DG_ERR_STATE, "dg_sens_tangent: record first");
+  dg_ctx *c = s->ctx;
+  auto t0 = std::chrono::steady_clock::now();
+  const size_t K = s->K, Np = s->Np, NF = s->NF, ST = K * Np * 4;
+  const int nb = s->nbu + s->nbl;
+  // geometry tangents on the host
+  std::vector<double> rxd(K * Np), ryd(K * Np), sxd(K * Np), syd(K * Np), nxd(K * NF), nyd(K * NF), fsd(K * NF), Jd(K);
+  {
+    std::vector<double> ndy(s->mesh.nnodes, 0.0);
+    for (int n = 0; n < s->mesh.nnodes; n++)
+      for (int i = 0; i < nb; i++) if (dbumps && dbumps[i] != 0.0) ndy[n] += dbumps[i] * node_dy_db(s, n, i);
+    std::vector<dg::Dual> a(Np), bb(Np), cc(Np), dd(Np), e(NF), f(NF), g(NF);
+    for (size_t k = 0; k < K; k++) {
+      dg::Dual v[6];
+      for (int t = 0; t < 3; t++) {
+        const int node = s->mesh.cells[3 * k + t];
+        v[2 * t] = dg::Dual(s->mesh.cell_xy[6 * k + 2 * t], 0.0);
+        v[2 * t + 1] = dg::Dual(s->mesh.cell_xy[6 * k + 2 * t + 1], ndy[node]);
+      }
+      dg::Dual Jc;
+      init_grid_cell<dg::Dual>(c, v, a.data(), bb.data(), cc.data(), dd.data(), e.data(), f.data(), g.data(), Jc);
+      for (size_t i = 0; i < Np; i++) { rxd[k * Np + i] = a[i].d; ryd[k * Np + i] = bb[i].d; sxd[k * Np + i] = cc[i].d; syd[k * Np + i] = dd[i].d; }
+      for (size_t m = 0; m < NF; m++) { nxd[k * NF + m] = e[m].d; nyd[k * NF + m] = f[m].d; fsd[k * NF + m] = g[m].d; }
+      Jd[k] = Jc.d;
+    }
+  }
+  HIPCHECK(c, hipMemcpy(s->rxd, rxd.data(), sizeof(double) * K * Np, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(s->ryd, ryd.data(), sizeof(double) * K * Np, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(s->sxd, sxd.data(), sizeof(double) * K * Np, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(s->syd, syd.data(), sizeof(double) * K * Np, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(s->nxd, nxd.data(), sizeof(double) * K * NF, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(s->nyd, nyd.data(), sizeof(double) * K * NF, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(s->fsd, fsd.data(), sizeof(double) * K * NF, hipMemcpyHostToDevice));
+  // free-stream tangents (alpha in degrees)
+  dg::Launch L = launch_a(s);
+  const double rad = M_PI / 180.0;
+  const double bcud = -L.ph.bc_v * rad * dalpha, bcvd = L.ph.bc_u * rad * dalpha;
+  // initial state and its tangent
+  HIPCHECK(c, hipMemsetAsync(s->tdev, 0, sizeof(double), s->stream));
+  if (set_ic(s, s->q) != DG_OK) return DG_ERR_HIP;
+  {
+    std::vector<double> q0d(ST);
+    const double r = L.ph.bc_r;
+    for (size_t i = 0; i < K * Np; i++) {
+      q0d[4 * i] = 0.0; q0d[4 * i + 1] = r * bcud; q0d[4 * i + 2] = r * bcvd;
+      q0d[4 * i + 3] = r * (L.ph.bc_u * bcud + L.ph.bc_v * bcvd);
+    }
+    HIPCHECK(c, hipMemcpy(s->qd, q0d.data(), sizeof(double) * ST, hipMemcpyHostToDevice));
+  }
+  dg::TlmArgs t{};
+  t.ncells = s->K;
+  t.rx = s->rx; t.ry = s->ry; t.sx = s->sx; t.sy = s->sy;
+  t.rxd = s->rxd; t.ryd = s->ryd; t.sxd = s->sxd; t.syd = s->syd;
+  t.nx = s->nx; t.ny = s->ny; t.fscale = s->fs; t.nxd = s->nxd; t.nyd = s->nyd; t.fsd = s->fsd;
+  t.fmap = s->fmap; t.fmask = c->d_fmask; t.fluxd = s->fluxd; t.out = s->Ld;
+  for (int n = 0; n < s->nsteps; n++) {
+    const double dt = s->h_dts[n];
+    HIPCHECK(c, stages01(s, s->q, n));
+    t.w = s->q; t.wd = s->qd;
+    HIPCHECK(c, dg::launch_rhs_tlm(L, t, bcud, bcvd));
+    HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->qd, dt, s->Ld, s->w1d));
+    t.w = s->wA; t.wd = s->w1d;
+    HIPCHECK(c, dg::launch_rhs_tlm(L, t, bcud, bcvd));
+    HIPCHECK(c, dg::launch_lincomb(L, ST, 0.75, s->qd, 0.25, s->w1d, s->tmp));
+    HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->tmp, 0.25 * dt, s->Ld, s->w2d));
+    t.w = s->wB; t.wd = s->w2d;
+    HIPCHECK(c, dg::launch_rhs_tlm(L, t, bcud, bcvd));
+    HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0 / 3.0, s->qd, 2.0 / 3.0, s->w2d, s->tmp));
+    HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->tmp, 2.0 / 3.0 * dt, s->Ld, s->qd));
+    HIPCHECK(c, step(s, s->q, n, false));
+  }
+  // C_L and its tangent on the host
+  WallData wd;
+  int rc = wall_data(s, s->q, wd);
+  if (rc != DG_OK) return rc;
+  std::vector<double> qdh(ST);
+  HIPCHECK(c, hipMemcpy(qdh.data(), s->qd, sizeof(double) * ST, hipMemcpyDeviceToHost));
+  const double g1 = c->gam - 1.0;
+  dg::Dual Fx = 0.0, Fy = 0.0;
+  for (int w = 0; w < s->nwall; w++) {
+    const size_t k = s->h_wcell[w];
+    const int f = s->h_wface[w];
+    dg::Dual bx = 0.0, by = 0.0;
+    for (int i = 0; i < s->Nfp; i++) {
+      const int m = f * s->Nfp + i;
+      const size_t node = k * Np + c->FMASK[m];
+      dg::Dual qq[4];
+      for (int v = 0; v < 4; v++) qq[v] = dg::Dual(wd.q[4 * node + v], qdh[4 * node + v]);
+      dg::Dual rho = qq[0], u = qq[1] / rho, vv = qq[2] / rho;
+      dg::Dual p = g1 * (qq[3] - 0.5 * rho * (u * u + vv * vv));
+      dg::Dual sJ = dg::Dual(wd.fs[k * NF + m], fsd[k * NF + m]) * dg::Dual(wd.J[k], Jd[k]);
+      bx += s->h_facew[i] * sJ * p * dg::Dual(wd.nx[k * NF + m], nxd[k * NF + m]);
+      by += s->h_facew[i] * sJ * p * dg::Dual(wd.ny[k * NF + m], nyd[k * NF + m]);
+    }
+    Fx += bx; Fy += by;
+  }
+  const double al = s->alpha * rad, qd0 = qdyn_of(s);
+  dg::Dual A(al, rad * dalpha);
+  dg::Dual CL = (Fy * dg::cos(A) - Fx * dg::sin(A)) / qd0;
+  if (cl) *cl = CL.v;
+  *dcl = CL.d;
+  s->tlm_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return DG_OK;
+}
+
+int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha, double *dcl_db) {
+  if (!s || s->nsteps < 1 || ckpt_every < 1)
+    return fail(s ? s->ctx : nullptr, DG_ERR_STATE, "dg_sens_gradient: record first; ckpt_every >= 1");
+  dg_ctx *c = s->ctx;
+  const size_t K = s->K, Np = s->Np, NF = s->NF, ST = K * Np * 4;
+  const int nb = s->nbu + s->nbl;
+  const int nck = (s->nsteps + ckpt_every - 1) / ckpt_every;
+  while ((int)s->ckpt.size() < nck) {
+    double *p = nullptr;
+    HIPCHECK(c, hipMalloc(&p, sizeof(double) * ST));
+    s->ckpt.push_back(p);
+  }
+  while ((int)s->seg.size() < ckpt_every) {
+    double *p = nullptr;
+    HIPCHECK(c, hipMalloc(&p, sizeof(double) * ST));
+    s->seg.push_back(p);
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  double clv = 0.0;
+  int rc = run_forward(s, false, ckpt_every, &clv);
+  if (rc != DG_OK) return rc;
+  // cotangent of C_L with respect to q_N, the wall geometry and alpha
+  WallData wd;
+  rc = wall_data(s, s->q, wd);
+  if (rc != DG_OK) return rc;
+  const double rad = M_PI / 180.0, al = s->alpha * rad, qd0 = qdyn_of(s);
+  double Fx = 0.0, Fy = 0.0;
+  const double g1 = c->gam - 1.0;
+  for (int w = 0; w < s->nwall; w++) {
+    const size_t k = s->h_wcell[w];
+    const int f = s->h_wface[w];
+    double bx = 0.0, by = 0.0;
+    for (int i = 0; i < s->Nfp; i++) {
+      const int m = f * s->Nfp + i;
+      const size_t node = k * Np + c->FMASK[m];
+      const double rho = wd.q[4 * node], u = wd.q[4 * node + 1] / rho, v = wd.q[4 * node + 2] / rho;
+      const double p = g1 * (wd.q[4 * node + 3] - 0.5 * rho * (u * u + v * v));
+      const double sJ = wd.fs[k * NF + m] * wd.J[k];
+      bx += s->h_facew[i] * sJ * p * wd.nx[k * NF + m];
+      by += s->h_facew[i] * sJ * p * wd.ny[k * NF + m];
+    }
+    Fx += bx; Fy += by;
+  }
+  const double kx = -std::sin(al) / qd0, ky = std::cos(al) / qd0;
+  std::vector<double> lamN(ST, 0.0), hnx(K * NF, 0.0), hny(K * NF, 0.0), hfs(K * NF, 0.0), hJ(K, 0.0);
+  for (int w = 0; w < s->nwall; w++) {
+    const size_t k = s->h_wcell[w];
+    const int f = s->h_wface[w];
+    for (int i = 0; i < s->Nfp; i++) {
+      const int m = f * s->Nfp + i;
+      const size_t node = k * Np + c->FMASK[m];
+      const double q0 = wd.q[4 * node], q1 = wd.q[4 * node + 1], q2 = wd.q[4 * node + 2], q3 = wd.q[4 * node + 3];
+      const double u = q1 / q0, v = q2 / q0;
+      const double p = g1 * (q3 - 0.5 * q0 * (u * u + v * v));
+      const double nxv = wd.nx[k * NF + m], nyv = wd.ny[k * NF + m], fsv = wd.fs[k * NF + m], Jv = wd.J[k];
+      const double wq = s->h_facew[i];
+      const double kn = kx * nxv + ky * nyv;
+      const double dp = wq * fsv * Jv * kn;
+      lamN[4 * node] += dp * g1 * 0.5 * (u * u + v * v);
+      lamN[4 * node + 1] += dp * (-g1 * u);
+      lamN[4 * node + 2] += dp * (-g1 * v);
+      lamN[4 * node + 3] += dp * g1;
+      hnx[k * NF + m] += wq * fsv * Jv * p * kx;
+      hny[k * NF + m] += wq * fsv * Jv * p * ky;
+      hfs[k * NF + m] += wq * Jv * p * kn;
+      hJ[k] += wq * fsv * p * kn;
+    }
+  }
+  double dalpha = (-Fy * std::sin(al) - Fx * std::cos(al)) / qd0 * rad;
+  HIPCHECK(c, hipMemcpy(s->lam, lamN.data(), sizeof(double) * ST, hipMemcpyHostToDevice));
+  double *acc[9] = {s->g_rx, s->g_ry, s->g_sx, s->g_sy, s->g_nx, s->g_ny, s->g_fs, s->g_bcu, s->g_bcv};
+  for (int a = 0; a < 9; a++)
+    HIPCHECK(c, hipMemsetAsync(acc[a], 0, sizeof(double) * (a < 4 ? K * Np : K * NF), s->stream));
+  // reverse sweep over checkpoint segments
+  dg::Launch L = launch_a(s);
+  dg::AdjArgs a{};
+  a.ncells = s->K;
+  a.rx = s->rx; a.ry = s->ry; a.sx = s->sx; a.sy = s->sy; a.nx = s->nx; a.ny = s->ny; a.fscale = s->fs;
+  a.fmap = s->fmap; a.fmask = c->d_fmask; a.gmq = s->gmq; a.gpq = s->gpq;
+  a.g_rx = s->g_rx; a.g_ry = s->g_ry; a.g_sx = s->g_sx; a.g_sy = s->g_sy;
+  a.g_nx = s->g_nx; a.g_ny = s->g_ny; a.g_fs = s->g_fs; a.g_bcu = s->g_bcu; a.g_bcv = s->g_bcv;
+  for (int sg = nck - 1; sg >= 0; sg--) {
+    const int n0 = sg * ckpt_every, n1 = std::min(s->nsteps, n0 + ckpt_every);
+    HIPCHECK(c, hipMemcpyAsync(s->seg[0], s->ckpt[sg], sizeof(double) * ST, hipMemcpyDeviceToDevice, s->stream));
+    for (int n = n0; n < n1 - 1; n++) {
+      HIPCHECK(c, hipMemcpyAsync(s->qw, s->seg[n - n0], sizeof(double) * ST, hipMemcpyDeviceToDevice, s->stream));
+      HIPCHECK(c, step(s, s->qw, n, false));
+      HIPCHECK(c, hipMemcpyAsync(s->seg[n - n0 + 1], s->qw, sizeof(double) * ST, hipMemcpyDeviceToDevice, s->stream));
+    }
+    for (int n = n1 - 1; n >= n0; n--) {
+      const double dt = s->h_dts[n];
+      const double *qn = s->seg[n - n0];
+      HIPCHECK(c, stages01(s, qn, n));  // w1 -> wA, w2 -> wB
+      a.scale = dt;
+      // q' = 1/3 q + 2/3 (w2 + dt L(w2))
+      HIPCHECK(c, dg::launch_lincomb(L, ST, 2.0 / 3.0, s->lam, 0.0, nullptr, s->M));
+      HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->M, 0.0, nullptr, s->A));
+      a.w = s->wB; a.mu = s->M; a.acc = s->A;
+      HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
+      HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0 / 3.0, s->lam, 0.75, s->A, s->lamq));
+      // w2 = 3/4 q + 1/4 (w1 + dt L(w1))
+      HIPCHECK(c, dg::launch_lincomb(L, ST, 0.25, s->A, 0.0, nullptr, s->M));
+      HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->M, 0.0, nullptr, s->B));
+      a.w = s->wA; a.mu = s->M; a.acc = s->B;
+      HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
+      // w1 = q + dt L(q)
+      HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->lamq, 1.0, s->B, s->lamq));
+      a.w = qn; a.mu = s->B; a.acc = s->lamq;
+      HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
+      std::swap(s->lam, s->lamq);
+    }
+  }
+  // initial state: q0 = free stream(alpha)
+  std::vector<double> lam0(ST);
+  HIPCHECK(c, hipStreamSynchronize(s->stream));
+  HIPCHECK(c, hipMemcpy(lam0.data(), s->lam, sizeof(double) * ST, hipMemcpyDeviceToHost));
+  const dg::Phys ph = L.ph;
+  const double dbcu = -ph.bc_v * rad, dbcv = ph.bc_u * rad;  // per degree
+  {
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (size_t i = 0; i < K * Np; i++) { s1 += lam0[4 * i + 1]; s2 += lam0[4 * i + 2]; s3 += lam0[4 * i + 3]; }
+    dalpha += ph.bc_r * (s1 * dbcu + s2 * dbcv) + s3 * ph.bc_r * (ph.bc_u * dbcu + ph.bc_v * dbcv);
+  }
+  // free-stream velocity seen by the boundary
+  std::vector<double> gbu(K * NF), gbv(K * NF);
+  HIPCHECK(c, hipMemcpy(gbu.data(), s->g_bcu, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(gbv.data(), s->g_bcv, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
+  {
+    double su = 0.0, sv = 0.0;
+    for (size_t i = 0; i < K * NF; i++) { su += gbu[i]; sv += gbv[i]; }
+    dalpha += su * dbcu + sv * dbcv;
+  }
+  // geometry: device cotangents + those of C_L, chained to the bumps
+  if (dcl_db && nb > 0) {
+    std::vector<double> grx(K * Np), gry(K * Np), gsx(K * Np), gsy(K * Np), gnx(K * NF), gny(K * NF), gfs(K * NF);
+    HIPCHECK(c, hipMemcpy(grx.data(), s->g_rx, sizeof(double) * K * Np, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(gry.data(), s->g_ry, sizeof(double) * K * Np, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(gsx.data(), s->g_sx, sizeof(double) * K * Np, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(gsy.data(), s->g_sy, sizeof(double) * K * Np, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(gnx.data(), s->g_nx, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(gny.data(), s->g_ny, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(gfs.data(), s->g_fs, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
+    std::vector<double> gnode_y(s->mesh.nnodes, 0.0);
+    CellGeoJac jac;
+    for (size_t k = 0; k < K; k++) {
+      double gv[6] = {0, 0, 0, 0, 0, 0};
+      bool any = hJ[k] != 0.0;
+      for (size_t i = 0; i < Np && !any; i++)
+        any = grx[k * Np + i] != 0.0 || gry[k * Np + i] != 0.0 || gsx[k * Np + i] != 0.0 || gsy[k * Np + i] != 0.0;
+      for (size_t m = 0; m < NF && !any; m++)
+        any = gnx[k * NF + m] + hnx[k * NF + m] != 0.0 || gny[k * NF + m] + hny[k * NF + m] != 0.0 ||
+              gfs[k * NF + m] + hfs[k * NF + m] != 0.0;
+      if (!any) continue;
+      cell_geo_jac(s, &s->mesh.cell_xy[6 * k], jac);
+      for (int d = 0; d < 6; d++) {
+        double acc_d = hJ[k] * jac.J[d];
+        for (size_t i = 0; i < Np; i++)
+          acc_d += grx[k * Np + i] * jac.rx[d * Np + i] + gry[k * Np + i] * jac.ry[d * Np + i] +
+                   gsx[k * Np + i] * jac.sx[d * Np + i] + gsy[k * Np + i] * jac.sy[d * Np + i];
+        for (size_t m = 0; m < NF; m++)
+          acc_d += (gnx[k * NF + m] + hnx[k * NF + m]) * jac.nx[d * NF + m] +
+                   (gny[k * NF + m] + hny[k * NF + m]) * jac.ny[d * NF + m] +
+                   (gfs[k * NF + m] + hfs[k * NF + m]) * jac.fs[d * NF + m];
+        gv[d] = acc_d;
+      }
+      for (int t = 0; t < 3; t++) gnode_y[s->mesh.cells[3 * k + t]] += gv[2 * t + 1];
+    }
+    for (int i = 0; i < nb; i++) {
+      double g = 0.0;
+      for (int n = 0; n < s->mesh.nnodes; n++)
+        if (gnode_y[n] != 0.0) g += gnode_y[n] * node_dy_db(s, n, i);
+      dcl_db[i] = g;
+    }
+  }
+  if (cl) *cl = clv;
+  if (dcl_dalpha) *dcl_dalpha = dalpha;
+  s->grad_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return DG_OK;
+}
+
+int dg_sens_timing(const dg_sens *s, double *forward_ms, double *gradient_ms, double *tangent_ms) {
+  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_sens_timing: null argument");
+  if (forward_ms) *forward_ms = s->fwd_ms;
+  if (gradient_ms) *gradient_ms = s->grad_ms;
+  if (tangent_ms) *tangent_ms = s->tlm_ms;
+  return DG_OK;
+}
+
+int dg_sens_dts(const dg_sens *s, int n, double *dts) {
+  if (!s || !dts || n != s->nsteps) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_sens_dts: bad argument");
+  std::memcpy(dts, s->h_dts.data(), sizeof(double) * n);
+  return DG_OK;
+}
+
+}  // extern "C"

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
     for (int c = lane; c < a.ncases; c += 64) a.dt1_next[c] = 0ull;
   if (EPI == EPI_STAGE2 && blockIdx.x == 0)
     for (int c = lane; c < a.ncases; c += 64)
-      a.t_dev[c] += a.cfl / __longlong_as_double((long long)a.dt1_cur[c]);
+      a.t_dev[c] += a.dt_fixed ? a.dt_fixed[c] : a.cfl / __longlong_as_double((long long)a.dt1_cur[c]);
   unsigned long long cfl_bits = 0ull;
   int cfl_case = -1;
   bool bad = false;
@@ -101,7 +101,8 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
       e0 = a.cell_off[cs] + (tile - a.tile_off[cs]) * TE;
       ne = min(TE, a.cell_off[cs + 1] - e0);
       ph = a.phys[cs];
-      if (EPI != EPI_RHS) dt = a.cfl / __longlong_as_double((long long)a.dt1_cur[cs]);
+      if (EPI != EPI_RHS)
+        dt = a.dt_fixed ? a.dt_fixed[cs] : a.cfl / __longlong_as_double((long long)a.dt1_cur[cs]);
     } else {
       e0 = tile * TE;
       ne = min(TE, a.ncells - e0);

@@ -57,7 +57,55 @@ def load_traffic(kernel):
         return None
 
 
+def summarize_trace(db):
+    """Per-kernel durations from a rocprofv3 --kernel-trace --stats database."""
+    import sqlite3
+    cur = sqlite3.connect(db).cursor()
+    rows = cur.execute("select name, total_calls, total_duration, average, percentage "
+                       "from top_kernels order by total_duration desc").fetchall()
+    print("| kernel | calls | total (us) | average (us) | % |")
+    print("|---|---|---|---|---|")
+    for name, calls, tot, avg, pct in rows:
+        print(f"| `{name.split('(')[0]}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.2f} |")
+
+
+def summarize_pmc(fetch_db, write_db, cfetch_db, cwrite_db, copy_bytes):
+    """HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE runs, each scaled by
+    what the counter reads for a plain device copy of copy_bytes bytes."""
+    import collections
+    import sqlite3
+
+    def per_kernel(db):
+        d = collections.defaultdict(list)
+        for name, val in sqlite3.connect(db).execute("select kernel_name, value from counters_collection"):
+            d[name.split("(")[0]].append(val)
+        return {k: sum(v) / len(v) for k, v in d.items()}
+    f, w, cf, cw = (per_kernel(x) for x in (fetch_db, write_db, cfetch_db, cwrite_db))
+    copy = max(cf, key=lambda k: cf[k])
+    rscale = copy_bytes / (cf[copy] * 1024.0)
+    wscale = copy_bytes / (cw[copy] * 1024.0)
+    print(f"calibration: plain copy of {copy_bytes} B ({copy}): FETCH_SIZE {cf[copy]:.1f} KiB, "
+          f"WRITE_SIZE {cw[copy]:.1f} KiB -> read scale {rscale:.4f}, write scale {wscale:.4f}")
+    print("| kernel | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | HBM read (B) | HBM write (B) | traffic (B) |")
+    print("|---|---|---|---|---|---|")
+    out = {}
+    for k in sorted(f, key=lambda k: -f[k]):
+        if k not in w:
+            continue
+        rd, wr = f[k] * 1024.0 * rscale, w[k] * 1024.0 * wscale
+        print(f"| `{k}` | {f[k]:.1f} | {w[k]:.1f} | {rd:.4e} | {wr:.4e} | {rd + wr:.4e} |")
+        for tag in ("rhs_mfma_kernel<3, true, 1>", "rhs_mfma_kernel<3, true, 2>", "rhs_mfma_kernel<3, true, 3>"):
+            if tag in k:
+                out["rhs_stage" + str(int(tag[-2]) - 1)] = {"bytes_per_launch": round(rd + wr),
+                                                           "read": round(rd), "write": round(wr)}
+    print(json.dumps(out))
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--summarize-trace":
+        return summarize_trace(sys.argv[2])
+    if len(sys.argv) > 6 and sys.argv[1] == "--summarize-pmc":
+        return summarize_pmc(*sys.argv[2:6], float(sys.argv[6]))
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)

@@ -78,6 +78,17 @@ _SIGS = {
     "dg_solver_check": ([c_void_p], c_int),
     "dg_solver_forces": ([c_void_p, c_double, P(c_double), P(c_double), c_void_p], c_int),
     "dg_solver_wall_cp": ([c_void_p, P(c_int), c_void_p, c_void_p], c_int),
+    "dg_rhs_tangent": ([c_void_p] + [c_void_p] * 16 + [c_double, c_double, c_void_p, c_void_p], c_int),
+    "dg_rhs_adjoint": ([c_void_p, c_void_p, c_void_p, c_double] + [c_void_p] * 17, c_int),
+    "dg_sens_create": ([c_void_p, c_int, c_int, c_double, c_double, c_int, c_int, c_double, c_double,
+                        P(c_void_p)], c_int),
+    "dg_sens_destroy": ([c_void_p], None),
+    "dg_sens_record": ([c_void_p, c_void_p, c_int, c_double, P(c_double)], c_int),
+    "dg_sens_eval": ([c_void_p, c_double, c_void_p, P(c_double)], c_int),
+    "dg_sens_tangent": ([c_void_p, c_double, c_void_p, P(c_double), P(c_double)], c_int),
+    "dg_sens_gradient": ([c_void_p, c_int, P(c_double), P(c_double), c_void_p], c_int),
+    "dg_sens_timing": ([c_void_p, P(c_double), P(c_double), P(c_double)], c_int),
+    "dg_sens_dts": ([c_void_p, c_int, c_void_p], c_int),
     "dg_solver_profile": ([c_void_p, c_int], c_int),
     "dg_solver_kernel_time": ([c_void_p, c_char_p, P(c_double), P(c_int)], c_int),
 }

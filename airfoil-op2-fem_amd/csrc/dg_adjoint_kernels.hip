@@ -282,7 +282,7 @@ __device__ void boundary_vjp(const Phys &ph, int type, const double mQ[4], doubl
 // ------------------------------------------------------------------------
 
 template <int N>
-__global__ void tlm_face_kernel(TlmArgs a, Phys ph, double bcud, double bcvd) {
+__global__ __launch_bounds__(256, 2) void tlm_face_kernel(TlmArgs a, Phys ph, double bcud, double bcvd) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   const size_t n = (size_t)a.ncells * NF;
   for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
@@ -309,7 +309,7 @@ __global__ void tlm_face_kernel(TlmArgs a, Phys ph, double bcud, double bcvd) {
 }
 
 template <int N>
-__global__ void tlm_cell_kernel(TlmArgs a, Phys ph, const double *__restrict__ ops) {
+__global__ __launch_bounds__(256, 2) void tlm_cell_kernel(TlmArgs a, Phys ph, const double *__restrict__ ops) {
   constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
   const double *Drw = ops, *Dsw = ops + Np * Np, *LIFT = ops + 2 * Np * Np;
   const size_t n = (size_t)a.ncells * Np;
@@ -339,7 +339,7 @@ __global__ void tlm_cell_kernel(TlmArgs a, Phys ph, const double *__restrict__ o
 }
 
 template <int N>
-__global__ void adj_face_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops) {
+__global__ __launch_bounds__(256, 2) void adj_face_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   const double *LIFT = ops + 2 * Np * Np;
   const size_t n = (size_t)a.ncells * NF;
@@ -389,7 +389,7 @@ __global__ void adj_face_kernel(AdjArgs a, Phys ph, const double *__restrict__ o
 }
 
 template <int N>
-__global__ void adj_cell_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops,
+__global__ __launch_bounds__(256, 2) void adj_cell_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops,
                                 const int *__restrict__ node_slots) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   const double *Drw = ops, *Dsw = ops + Np * Np;

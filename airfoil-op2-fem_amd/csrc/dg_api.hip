@@ -159,7 +159,7 @@ void dg_ctx_destroy(dg_ctx *c) {
   if (!c) return;
   (void)hipFree(c->d_ops); (void)hipFree(c->d_fmask); (void)hipFree(c->d_red);
   (void)hipFree(c->d_fmap); (void)hipFree(c->d_b2c); (void)hipFree(c->d_bnum);
-  (void)hipFree(c->d_btype);
+  (void)hipFree(c->d_btype); (void)hipFree(c->d_node_slots);
   delete c;
 }
 

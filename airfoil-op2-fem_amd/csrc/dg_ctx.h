@@ -28,6 +28,7 @@ struct dg_ctx {
   int ncells = -1, nedges = 0, nbedges = 0;
   int *d_fmap = nullptr, *d_b2c = nullptr, *d_bnum = nullptr, *d_btype = nullptr;
   unsigned long long *d_red = nullptr;
+  int *d_node_slots = nullptr;  // [Np][2] face slots reading each node (adjoint)
 };
 
 struct dg_mesh {

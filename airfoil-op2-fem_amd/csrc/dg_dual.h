@@ -46,6 +46,13 @@ DG_HD inline Dual pow(Dual a, double e) {
 }
 DG_HD inline Dual sin(Dual a) { return Dual(::sin(a.v), ::cos(a.v) * a.d); }
 DG_HD inline Dual cos(Dual a) { return Dual(::cos(a.v), -::sin(a.v) * a.d); }
+// double overloads, so that unqualified calls inside namespace dg keep
+// resolving to the plain functions for double arguments
+DG_HD inline double sqrt(double a) { return ::sqrt(a); }
+DG_HD inline double fabs(double a) { return ::fabs(a); }
+DG_HD inline double pow(double a, double e) { return ::pow(a, e); }
+DG_HD inline double sin(double a) { return ::sin(a); }
+DG_HD inline double cos(double a) { return ::cos(a); }
 DG_HD inline double val(Dual a) { return a.v; }
 DG_HD inline double val(double a) { return a; }
 

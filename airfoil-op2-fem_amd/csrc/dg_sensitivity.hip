@@ -768,6 +768,54 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   return DG_OK;
 }
 
+int dg_rhs_tangent(dg_ctx *c, const double *w, const double *wd, const double *rx, const double *ry,
+                   const double *sx, const double *sy, const double *fscale, const double *nx,
+                   const double *ny, const double *rxd, const double *ryd, const double *sxd,
+                   const double *syd, const double *fsd, const double *nxd, const double *nyd,
+                   double bcud, double bcvd, double *scratch, double *out) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_rhs_tangent: null context");
+  if (c->ncells < 0) return fail(c, DG_ERR_STATE, "dg_rhs_tangent: no mesh declared");
+  if (c->ncells > 0 && (!w || !wd || !rx || !ry || !sx || !sy || !fscale || !nx || !ny || !scratch || !out))
+    return fail(c, DG_ERR_ARG, "dg_rhs_tangent: null array");
+  dg::TlmArgs t{};
+  t.ncells = c->ncells; t.w = w; t.wd = wd;
+  t.rx = rx; t.ry = ry; t.sx = sx; t.sy = sy; t.nx = nx; t.ny = ny; t.fscale = fscale;
+  t.rxd = rxd; t.ryd = ryd; t.sxd = sxd; t.syd = syd; t.nxd = nxd; t.nyd = nyd; t.fsd = fsd;
+  t.fmap = c->d_fmap; t.fmask = c->d_fmask; t.fluxd = scratch; t.out = out;
+  HIPCHECK(c, dg::launch_rhs_tlm(dg::launch_of(c), t, bcud, bcvd));
+  return DG_OK;
+}
+
+int dg_rhs_adjoint(dg_ctx *c, const double *w, const double *mu, double scale, const double *rx,
+                   const double *ry, const double *sx, const double *sy, const double *fscale,
+                   const double *nx, const double *ny, double *scratch, double *acc, double *grx,
+                   double *gry, double *gsx, double *gsy, double *gfs, double *gnx, double *gny,
+                   double *gbcu, double *gbcv) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_rhs_adjoint: null context");
+  if (c->ncells < 0) return fail(c, DG_ERR_STATE, "dg_rhs_adjoint: no mesh declared");
+  if (c->ncells > 0 && (!w || !mu || !rx || !ry || !sx || !sy || !fscale || !nx || !ny || !scratch ||
+                        !acc || !grx || !gry || !gsx || !gsy || !gfs || !gnx || !gny || !gbcu || !gbcv))
+    return fail(c, DG_ERR_ARG, "dg_rhs_adjoint: null array");
+  if (!c->d_node_slots) {
+    std::vector<int> slots(2 * c->Np, -1);
+    for (int m = 0; m < c->NF; m++) {
+      const int nd = c->FMASK[m];
+      if (slots[2 * nd] < 0) slots[2 * nd] = m; else slots[2 * nd + 1] = m;
+    }
+    HIPCHECK(c, hipMalloc(&c->d_node_slots, sizeof(int) * slots.size()));
+    HIPCHECK(c, hipMemcpy(c->d_node_slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice));
+  }
+  dg::AdjArgs a{};
+  a.ncells = c->ncells; a.scale = scale; a.w = w; a.mu = mu;
+  a.rx = rx; a.ry = ry; a.sx = sx; a.sy = sy; a.nx = nx; a.ny = ny; a.fscale = fscale;
+  a.fmap = c->d_fmap; a.fmask = c->d_fmask;
+  a.gmq = scratch; a.gpq = scratch + (size_t)c->ncells * c->NF * 4;
+  a.acc = acc; a.g_rx = grx; a.g_ry = gry; a.g_sx = gsx; a.g_sy = gsy;
+  a.g_nx = gnx; a.g_ny = gny; a.g_fs = gfs; a.g_bcu = gbcu; a.g_bcv = gbcv;
+  HIPCHECK(c, dg::launch_rhs_adj(dg::launch_of(c), a, c->d_node_slots));
+  return DG_OK;
+}
+
 int dg_sens_timing(const dg_sens *s, double *forward_ms, double *gradient_ms, double *tangent_ms) {
   if (!s) return fail(nullptr, DG_ERR_ARG, "dg_sens_timing: null argument");
   if (forward_ms) *forward_ms = s->fwd_ms;

@@ -249,6 +249,36 @@ class Context:
                                    _dev(fscale, "fscale", K * NF), _dev(nx, "nx", K * NF),
                                    _dev(ny, "ny", K * NF), _dev(qRHS, "qRHS", n)))
 
+    def rhs_tangent(self, w, wd, geo, geod=None, bcd=(0.0, 0.0), scratch=None, out=None):
+        """L'(w) wd + dL/dgeo geod + dL/d(bc_u, bc_v) bcd.  geo/geod: dicts of
+        device tensors rx, ry, sx, sy, fscale, nx, ny (geod entries optional)."""
+        import torch
+        K, Np, NF, n = self.ncells, self.Np, self.NF, self.state_numel()
+        scratch = scratch if scratch is not None else torch.empty(K * NF * 4, dtype=torch.float64, device=w.device)
+        out = out if out is not None else torch.empty(n, dtype=torch.float64, device=w.device)
+        geod = geod or {}
+        sizes = dict(rx=K * Np, ry=K * Np, sx=K * Np, sy=K * Np, fscale=K * NF, nx=K * NF, ny=K * NF)
+        g = [_dev(geo[k], k, sizes[k]) for k in ("rx", "ry", "sx", "sy", "fscale", "nx", "ny")]
+        gd = [(_dev(geod[k], k + "d", sizes[k]) if k in geod else None)
+              for k in ("rx", "ry", "sx", "sy", "fscale", "nx", "ny")]
+        self._chk(lib.dg_rhs_tangent(self._h, _dev(w, "w", n), _dev(wd, "wd", n), *g, *gd,
+                                     float(bcd[0]), float(bcd[1]), _dev(scratch, "scratch", K * NF * 4),
+                                     _dev(out, "out", n)))
+        return out
+
+    def rhs_adjoint(self, w, mu, scale, geo, acc, gacc, scratch=None):
+        """acc += scale L'(w)^T mu; gacc (dict: rx, ry, sx, sy, fscale, nx, ny,
+        bcu, bcv) += scale (dL/dgeo)^T mu."""
+        import torch
+        K, Np, NF, n = self.ncells, self.Np, self.NF, self.state_numel()
+        scratch = scratch if scratch is not None else torch.empty(2 * K * NF * 4, dtype=torch.float64, device=w.device)
+        sizes = dict(rx=K * Np, ry=K * Np, sx=K * Np, sy=K * Np, fscale=K * NF, nx=K * NF, ny=K * NF,
+                     bcu=K * NF, bcv=K * NF)
+        g = [_dev(geo[k], k, sizes[k]) for k in ("rx", "ry", "sx", "sy", "fscale", "nx", "ny")]
+        ga = [_dev(gacc[k], "g" + k, sizes[k]) for k in ("rx", "ry", "sx", "sy", "fscale", "nx", "ny", "bcu", "bcv")]
+        self._chk(lib.dg_rhs_adjoint(self._h, _dev(w, "w", n), _dev(mu, "mu", n), float(scale), *g,
+                                     _dev(scratch, "scratch", 2 * K * NF * 4), _dev(acc, "acc", n), *ga))
+
     def set_workingQ(self, dt, stage, q, k1, k2, workingQ):
         n = self.state_numel()
         self._chk(lib.dg_set_workingQ(self._h, float(dt), int(stage), _dev(q, "q", n), _dev(k1, "k1", n),
@@ -360,6 +390,72 @@ class Solver:
         ms, n = c_double(), c_int()
         self._chk(lib.dg_solver_kernel_time(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+class Sensitivity:
+    """dC_L/d(alpha, bump amplitudes) of the NACA0012 O-grid case by the
+    discrete adjoint, with tangent-linear and frozen-dt evaluations."""
+
+    def __init__(self, ctx, ncirc, nrad, stretch, rfar, nbu, nbl, mach, alpha):
+        self.ctx, self.nb = ctx, int(nbu) + int(nbl)
+        h = c_void_p()
+        check(lib.dg_sens_create(ctx._h, int(ncirc), int(nrad), float(stretch), float(rfar), int(nbu),
+                                 int(nbl), float(mach), float(alpha), ctypes.byref(h)), ctx._h)
+        self._h = h
+        self.nsteps = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.dg_sens_destroy(h)
+            self._h = None
+
+    def _chk(self, rc):
+        return check(rc, self.ctx._h)
+
+    def _b(self, b):
+        if b is None:
+            return None, None
+        a = np.ascontiguousarray(b, dtype=np.float64)
+        if a.size != self.nb:
+            raise ValueError(f"expected {self.nb} bump values")
+        return a, _np_ptr(a)
+
+    def record(self, bumps=None, nsteps=400, cfl=1.0):
+        a, p = self._b(bumps)
+        cl = c_double()
+        self._chk(lib.dg_sens_record(self._h, p, int(nsteps), float(cfl), ctypes.byref(cl)))
+        self.nsteps = int(nsteps)
+        return cl.value
+
+    def eval(self, alpha, bumps=None):
+        a, p = self._b(bumps)
+        cl = c_double()
+        self._chk(lib.dg_sens_eval(self._h, float(alpha), p, ctypes.byref(cl)))
+        return cl.value
+
+    def tangent(self, dalpha=0.0, dbumps=None):
+        a, p = self._b(dbumps)
+        cl, dcl = c_double(), c_double()
+        self._chk(lib.dg_sens_tangent(self._h, float(dalpha), p, ctypes.byref(cl), ctypes.byref(dcl)))
+        return cl.value, dcl.value
+
+    def gradient(self, ckpt_every=20):
+        cl, da = c_double(), c_double()
+        db = np.zeros(self.nb)
+        self._chk(lib.dg_sens_gradient(self._h, int(ckpt_every), ctypes.byref(cl), ctypes.byref(da),
+                                       _np_ptr(db) if self.nb else None))
+        return cl.value, da.value, db
+
+    def timing(self):
+        f, g, t = c_double(), c_double(), c_double()
+        self._chk(lib.dg_sens_timing(self._h, ctypes.byref(f), ctypes.byref(g), ctypes.byref(t)))
+        return dict(forward_ms=f.value, gradient_ms=g.value, tangent_ms=t.value)
+
+    def dts(self):
+        d = np.zeros(self.nsteps)
+        self._chk(lib.dg_sens_dts(self._h, self.nsteps, _np_ptr(d)))
+        return d
 
 
 class BatchSolver(Solver):

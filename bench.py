@@ -101,7 +101,58 @@ def summarize_pmc(fetch_db, write_db, cfetch_db, cwrite_db, copy_bytes):
     print(json.dumps(out))
 
 
+def adjoint_bench(args):
+    """Lift sensitivities of the NACA0012 O-grid (256 x 128, stretch 1.08,
+    far field at 20 chords, 65,536 cells) with 16 Hicks-Henne bumps (8 upper,
+    8 lower, seeded amplitudes in +-1e-3), Mach 0.4, alpha 2 deg, N=3:
+    400 SSPRK3 steps recorded, adjoint with checkpoints every 20 steps,
+    checked against the tangent-linear run and central differences."""
+    import numpy as np
+    import torch
+    dg = importlib.import_module("airfoil-op2-fem_amd")
+    torch.cuda.set_device(0)
+    ctx = dg.Context(args.order)
+    ctx.set_freestream(1.4, 0.4, 2.0, 1.0, 1.0)
+    s = dg.Sensitivity(ctx, 256, 128, 1.08, 20.0, 8, 8, 0.4, 2.0)
+    bumps = np.random.default_rng(2024).uniform(-1e-3, 1e-3, 16)
+    cl = s.record(bumps, nsteps=args.adj_steps, cfl=1.0)
+    s.record(bumps, nsteps=args.adj_steps, cfl=1.0)  # warm
+    cl_g, da, db = s.gradient(ckpt_every=20)
+    cl_g, da, db = s.gradient(ckpt_every=20)
+    tm = s.timing()
+    h = 1e-6
+    checks = {}
+    fd = (s.eval(2.0 + h, bumps) - s.eval(2.0 - h, bumps)) / (2 * h)
+    _, tl = s.tangent(1.0, None)
+    checks["alpha"] = {"adjoint": da, "tangent": tl, "fd": fd, "rel_fd": abs(fd - da) / abs(da),
+                       "rel_tangent": abs(tl - da) / abs(da)}
+    for i in (0, 7, 8, 15):
+        bp, bm = bumps.copy(), bumps.copy()
+        bp[i] += h
+        bm[i] -= h
+        fd = (s.eval(2.0, bp) - s.eval(2.0, bm)) / (2 * h)
+        e = np.zeros(16)
+        e[i] = 1.0
+        _, tl = s.tangent(0.0, e)
+        checks[f"bump{i}"] = {"adjoint": db[i], "tangent": tl, "fd": fd,
+                              "rel_fd": abs(fd - db[i]) / max(abs(db[i]), 1e-300),
+                              "rel_tangent": abs(tl - db[i]) / max(abs(db[i]), 1e-300)}
+    print(json.dumps({"metric": "adjoint gradient cost / forward cost", "value": tm["gradient_ms"] / tm["forward_ms"],
+                      "unit": "ratio", "higher_is_better": False, "dtype": "f64",
+                      "config": {"workload": "NACA0012 O-grid 256x128 (65536 cells), 16 Hicks-Henne bumps, "
+                                             f"M 0.4, alpha 2 deg, N={args.order}, {args.adj_steps} SSPRK3 steps, "
+                                             "checkpoints every 20 steps, frozen dt"},
+                      "cl": cl, "dcl_dalpha_per_deg": da, "dcl_dbumps": db.tolist(), "timing_ms": tm,
+                      "checks": checks}))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--adjoint":
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--adjoint", action="store_true")
+        ap.add_argument("--adj-steps", type=int, default=400)
+        ap.add_argument("--order", type=int, default=3)
+        return adjoint_bench(ap.parse_args())
     if len(sys.argv) > 2 and sys.argv[1] == "--summarize-trace":
         return summarize_trace(sys.argv[2])
     if len(sys.argv) > 6 and sys.argv[1] == "--summarize-pmc":

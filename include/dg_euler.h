@@ -266,6 +266,62 @@ int dg_solver_wall_cp(dg_solver *s, int *nwall, double *x, double *cp);
 int dg_solver_profile(dg_solver *s, int enable);
 int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int *count);
 
+/* ------------------------------------------------------------------ */
+/* Tangent-linear and adjoint RHS (no reference counterpart)          */
+/* ------------------------------------------------------------------ */
+
+/* d_out = L'(w) wd + dL/dgeo geod + dL/d(bc_u, bc_v) (bcud, bcvd) on the
+ * declared mesh, L the RHS of dg_rhs_fused.  Geometry tangents may be NULL
+ * (zero).  d_scratch holds ncells*3*Nfp*4 doubles. */
+int dg_rhs_tangent(dg_ctx *ctx, const double *d_w, const double *d_wd, const double *d_rx,
+                   const double *d_ry, const double *d_sx, const double *d_sy,
+                   const double *d_fscale, const double *d_nx, const double *d_ny,
+                   const double *d_rxd, const double *d_ryd, const double *d_sxd,
+                   const double *d_syd, const double *d_fsd, const double *d_nxd,
+                   const double *d_nyd, double bcud, double bcvd, double *d_scratch,
+                   double *d_out);
+/* d_acc += scale L'(w)^T mu; the geometry cotangents d_g* ([ncells][Np] for
+ * rx..sy, [ncells][3*Nfp] for fscale, nx, ny, bcu, bcv) += scale (dL/dgeo)^T
+ * mu.  Gather-only, deterministic.  d_scratch holds 2*ncells*3*Nfp*4 doubles. */
+int dg_rhs_adjoint(dg_ctx *ctx, const double *d_w, const double *d_mu, double scale,
+                   const double *d_rx, const double *d_ry, const double *d_sx,
+                   const double *d_sy, const double *d_fscale, const double *d_nx,
+                   const double *d_ny, double *d_scratch, double *d_acc, double *d_grx,
+                   double *d_gry, double *d_gsx, double *d_gsy, double *d_gfs, double *d_gnx,
+                   double *d_gny, double *d_gbcu, double *d_gbcv);
+
+/* ------------------------------------------------------------------ */
+/* Lift sensitivities of the airfoil case (no reference counterpart)  */
+/* ------------------------------------------------------------------ */
+
+typedef struct dg_sens dg_sens;
+/* O-grid around NACA0012 (as dg_mesh_ogrid) whose surface carries nbu
+ * Hicks-Henne bumps on the upper and nbl on the lower surface (peaks at
+ * chord positions i/(n+1), width exponent 3; a positive amplitude moves the
+ * surface up), free stream (Mach, alpha in degrees) of the context's gamma,
+ * p and rho.  The objective is C_L after the recorded number of steps. */
+int dg_sens_create(dg_ctx *ctx, int ncirc, int nrad, double stretch, double rfar, int nbu,
+                   int nbl, double mach, double alpha_deg, dg_sens **out);
+void dg_sens_destroy(dg_sens *s);
+/* Sets the bump amplitudes (NULL: all zero), runs nsteps SSPRK3 steps with
+ * dt = cfl/dt1 and records the dt sequence, which every later evaluation of
+ * this object replays (dt frozen).  Returns C_L. */
+int dg_sens_record(dg_sens *s, const double *bumps, int nsteps, double cfl, double *cl);
+/* C_L with the recorded dt sequence at another angle / shape (for finite
+ * differences); the recorded point is restored afterwards. */
+int dg_sens_eval(dg_sens *s, double alpha_deg, const double *bumps, double *cl);
+/* Tangent-linear run at the recorded point: dC_L for the direction
+ * (dalpha [deg], dbumps[nbu+nbl]). */
+int dg_sens_tangent(dg_sens *s, double dalpha, const double *dbumps, double *cl, double *dcl);
+/* Discrete adjoint at the recorded point with checkpoints every ckpt_every
+ * steps: dC_L/dalpha (per degree) and dC_L/dbumps[nbu+nbl]. */
+int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
+                     double *dcl_dbumps);
+/* Wall-clock times (ms) of the last record, gradient and tangent calls. */
+int dg_sens_timing(const dg_sens *s, double *forward_ms, double *gradient_ms, double *tangent_ms);
+/* The recorded dt sequence (n must equal the recorded step count). */
+int dg_sens_dts(const dg_sens *s, int n, double *dts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,12 @@ struct TlmArgs {
   double *out;    // [K][Np][4]: tangent of the RHS
 };
 
+// State epilogue of the adjoint RHS (element-wise, so out may alias x0 and
+// out2 may alias y0):
+//   out  = e0 x0 + scale L'(w)^T mu            (x0 may be null)
+//   out2 = f0 y0 + f1 x0 + f2 out              (when out2 is not null)
+// The reverse SSPRK3 step is three such launches with no separate
+// linear-combination passes.
 struct AdjArgs {
   int ncells;
   double scale;
@@ -25,7 +31,12 @@ struct AdjArgs {
   const double *rx, *ry, *sx, *sy, *nx, *ny, *fscale;
   const int *fmap, *fmask;
   double *gmq, *gpq;                    // scratch [K][NF][4]
-  double *acc;                          // += scale L'(w)^T mu
+  const double *x0;
+  double e0;
+  double *out;
+  const double *y0;
+  double f0, f1, f2;
+  double *out2;
   double *g_rx, *g_ry, *g_sx, *g_sy;    // += scale (dL/dgeo)^T mu  [K][Np]
   double *g_nx, *g_ny, *g_fs;           // [K][NF]
   double *g_bcu, *g_bcv;                // [K][NF] (boundary slots only)
@@ -40,5 +51,24 @@ hipError_t launch_lincomb(const Launch &L, size_t n, double a, const double *x, 
                           const double *y, double *z);
 // *dst = cfl / dt1
 hipError_t launch_record_dt(const Launch &L, const unsigned long long *dt1, double cfl, double *dst);
+
+// Reverse of init_grid for the vertex coordinates: for every cell and each
+// of its 6 vertex coordinates d, gv[6 c + d] = sum over the cell's factors
+// of cotangent * d(factor)/d(coordinate d) (forward mode, one dual
+// direction per thread).  g_J (cotangent of J at node 0) may be null.
+struct GeoCot {
+  const double *rx, *ry, *sx, *sy;  // [K][Np]
+  const double *nx, *ny, *fs;       // [K][NF]
+  const double *J;                  // [K]
+};
+hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_xy, const GeoCot &g,
+                          double *gv);
+// partial[0..kSumBlocks) = block sums of x[i * stride + offset], i < n, in a
+// fixed order; the caller adds the partials in index order.
+constexpr int kSumBlocks = 256;
+hipError_t launch_strided_sum(const Launch &L, size_t n, int stride, int offset, const double *x,
+                              double *partial);
+// dst[idx[i]] += val[i] (indices distinct)
+hipError_t launch_add_indexed(const Launch &L, int n, const int *idx, const double *val, double *dst);
 
 }  // namespace dg

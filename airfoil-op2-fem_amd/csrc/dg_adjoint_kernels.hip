@@ -5,18 +5,22 @@
 //
 // Tangent:  Ld = L'(w) wd + dL/dgeo geod + dL/dbc bcd, with the point
 //           functions evaluated in dual arithmetic.
-// Adjoint:  acc += scale * L'(w)^T mu, geometry and bc accumulators
-//           += scale * (dL/dgeo)^T mu, (dL/dbc)^T mu.  Gather-only in two
-//           passes: the face pass evaluates the reverse Roe flux once per
-//           face slot and stores the interior-side (gmq) and exterior-side
-//           (gpq) cotangents; the cell pass applies the transposed weak
+// Adjoint:  out = e0 x0 + scale * L'(w)^T mu (plus an optional second
+//           combination), geometry and bc accumulators += scale *
+//           (dL/dgeo)^T mu, (dL/dbc)^T mu.  Gather-only in two passes: the
+//           face pass evaluates the reverse Roe flux once per face slot and
+//           stores the interior-side (gmq) and exterior-side (gpq)
+//           cotangents; the cell pass applies the transposed weak
 //           derivatives, the reverse Euler fluxes, and gathers its own gmq and
-//           the neighbours' gpq of the slots that read its nodes.
+//           the neighbours' gpq of the slots that read its nodes.  Both passes
+//           give each workgroup whole cells, staging the cotangent mu, the
+//           metrics and the nodal fluxes of its cells in LDS.
 #include <hip/hip_runtime.h>
 
 #include "dg_adjoint.h"
 #include "dg_device.h"
 #include "dg_dual.h"
+#include "dg_geom.h"
 
 namespace dg {
 
@@ -258,9 +262,12 @@ __device__ void roe_vjp(double gam, const double mQ[4], const double eQ[4], doub
 
 // Reverse of boundary_t through its dual Jacobian (8 directions: the four
 // interior variables, the normal, the free-stream velocity).
-__device__ void boundary_vjp(const Phys &ph, int type, const double mQ[4], double nx, double ny,
-                             const double bqb[4], double bmQ[4], double &bnx, double &bny,
-                             double &bbcu, double &bbcv) {
+// Out of line: only boundary slots take it, and inlined it would raise the
+// register count of the whole face pass.
+__device__ __noinline__ void boundary_vjp(const Phys &ph, int type, const double mQ[4], double nx, double ny,
+                                          const double bqb[4], double bmQ[4], double &bnx, double &bny,
+                                          double &bbcu, double &bbcv) {
+#pragma unroll 1
   for (int dir = 0; dir < 8; dir++) {
     Dual q[4], qb[4];
     for (int k = 0; k < 4; k++) q[k] = Dual(mQ[k], dir == k ? 1.0 : 0.0);
@@ -338,21 +345,36 @@ __global__ __launch_bounds__(256, 2) void tlm_cell_kernel(TlmArgs a, Phys ph, co
   }
 }
 
+constexpr int kAdjThreads = 256;
+
+// One thread per face slot, whole cells per workgroup; the cells' mu and the
+// LIFT matrix are staged in LDS for psi = -LIFT^T mu.
 template <int N>
-__global__ __launch_bounds__(256, 2) void adj_face_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops) {
+__global__ __launch_bounds__(kAdjThreads, 2) void adj_face_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
-  const double *LIFT = ops + 2 * Np * Np;
-  const size_t n = (size_t)a.ncells * NF;
-  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
-       it += (size_t)gridDim.x * blockDim.x) {
-    const size_t c = it / NF;
-    const int m = (int)(it - c * NF);
-    const int f = m / Nfp, i = m - f * Nfp;
+  constexpr int CPB = kAdjThreads / NF;
+  __shared__ double sL[Np * NF];
+  __shared__ double sMu[4][CPB * Np];
+  for (int k = threadIdx.x; k < Np * NF; k += kAdjThreads) sL[k] = ops[2 * Np * Np + k];
+  const int t = threadIdx.x;
+  const int cl = t / NF, m = t - cl * NF;
+  const int f = m / Nfp, i = m - f * Nfp;
+  for (size_t base = (size_t)blockIdx.x * CPB; base < (size_t)a.ncells; base += (size_t)gridDim.x * CPB) {
+    __syncthreads();
+    const int nc = (int)min((size_t)CPB, (size_t)a.ncells - base);
+    for (int k = t; k < nc * Np; k += kAdjThreads) {
+      const double4 mu = *reinterpret_cast<const double4 *>(a.mu + (base * Np + k) * 4);
+      sMu[0][k] = mu.x; sMu[1][k] = mu.y; sMu[2][k] = mu.z; sMu[3][k] = mu.w;
+    }
+    __syncthreads();
+    if (cl >= nc) continue;
+    const size_t c = base + cl;
+    const size_t it = c * NF + m;
     // cotangent of the scaled flux at this slot: L = vol - LIFT flux
     double psi[4] = {0, 0, 0, 0};
     for (int j = 0; j < Np; j++) {
-      const double l = LIFT[j * NF + m];
-      for (int v = 0; v < 4; v++) psi[v] -= l * a.mu[(c * Np + j) * 4 + v];
+      const double l = sL[j * NF + m];
+      for (int v = 0; v < 4; v++) psi[v] -= l * sMu[v][cl * Np + j];
     }
     const int node = a.fmask[m];
     double mQ[4], pQ[4];
@@ -388,30 +410,53 @@ __global__ __launch_bounds__(256, 2) void adj_face_kernel(AdjArgs a, Phys ph, co
   }
 }
 
+// One thread per node, whole cells per workgroup; mu, the metrics and the
+// nodal Euler fluxes of the cells are staged in LDS, so each flux is
+// evaluated once per node.
 template <int N>
-__global__ __launch_bounds__(256, 2) void adj_cell_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops,
+__global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops,
                                 const int *__restrict__ node_slots) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
-  const double *Drw = ops, *Dsw = ops + Np * Np;
-  const size_t n = (size_t)a.ncells * Np;
-  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
-       it += (size_t)gridDim.x * blockDim.x) {
-    const size_t c = it / Np;
-    const int i = (int)(it - c * Np);
+  constexpr int CPB = kAdjThreads / Np, NT = CPB * Np;
+  __shared__ double sD[2 * Np * Np];
+  __shared__ double sMu[4][NT], sMet[4][NT], sFG[8][NT];
+  for (int k = threadIdx.x; k < 2 * Np * Np; k += kAdjThreads) sD[k] = ops[k];
+  const double *Drw = sD, *Dsw = sD + Np * Np;
+  const int t = threadIdx.x;
+  const int cl = t / Np, i = t - cl * Np;
+  for (size_t base = (size_t)blockIdx.x * CPB; base < (size_t)a.ncells; base += (size_t)gridDim.x * CPB) {
+    const int nc = (int)min((size_t)CPB, (size_t)a.ncells - base);
+    const bool act = cl < nc;
+    const size_t c = base + cl;
+    const size_t it = c * Np + i;
+    double wi[4];
+    __syncthreads();
+    if (act) {
+      const double4 w4 = *reinterpret_cast<const double4 *>(a.w + it * 4);
+      wi[0] = w4.x; wi[1] = w4.y; wi[2] = w4.z; wi[3] = w4.w;
+      const double4 mu = *reinterpret_cast<const double4 *>(a.mu + it * 4);
+      sMu[0][t] = mu.x; sMu[1][t] = mu.y; sMu[2][t] = mu.z; sMu[3][t] = mu.w;
+      sMet[0][t] = a.rx[it]; sMet[1][t] = a.sx[it]; sMet[2][t] = a.ry[it]; sMet[3][t] = a.sy[it];
+      double F[4], G[4];
+      euler_flux(ph.gam, wi, F, G);
+      for (int v = 0; v < 4; v++) { sFG[v][t] = F[v]; sFG[4 + v][t] = G[v]; }
+    }
+    __syncthreads();
+    if (!act) continue;
+    const int c0 = cl * Np;
     // transposed weak derivatives with the metric terms
     double phF[4] = {0, 0, 0, 0}, phG[4] = {0, 0, 0, 0};
     for (int j = 0; j < Np; j++) {
-      const size_t gj = c * Np + j;
       const double cr = Drw[j * Np + i], cs = Dsw[j * Np + i];
-      const double aF = cr * a.rx[gj] + cs * a.sx[gj], aG = cr * a.ry[gj] + cs * a.sy[gj];
+      const double aF = cr * sMet[0][c0 + j] + cs * sMet[1][c0 + j];
+      const double aG = cr * sMet[2][c0 + j] + cs * sMet[3][c0 + j];
       for (int v = 0; v < 4; v++) {
-        const double mu = a.mu[gj * 4 + v];
+        const double mu = sMu[v][c0 + j];
         phF[v] += aF * mu;
         phG[v] += aG * mu;
       }
     }
-    double wi[4], lam[4] = {0, 0, 0, 0};
-    for (int v = 0; v < 4; v++) wi[v] = a.w[it * 4 + v];
+    double lam[4] = {0, 0, 0, 0};
     euler_flux_vjp(ph.gam, wi, phF, phG, lam);
     // face contributions that read this node
     for (int s = 0; s < 2; s++) {
@@ -425,21 +470,27 @@ __global__ __launch_bounds__(256, 2) void adj_cell_kernel(AdjArgs a, Phys ph, co
         for (int v = 0; v < 4; v++) lam[v] += a.gpq[(nb * NF + mp) * 4 + v];
       }
     }
-    for (int v = 0; v < 4; v++) a.acc[it * 4 + v] += a.scale * lam[v];
+    double o[4];
+    for (int v = 0; v < 4; v++) o[v] = (a.x0 ? a.e0 * a.x0[it * 4 + v] : 0.0) + a.scale * lam[v];
+    *reinterpret_cast<double4 *>(a.out + it * 4) = make_double4(o[0], o[1], o[2], o[3]);
+    if (a.out2) {
+      double o2[4];
+      for (int v = 0; v < 4; v++)
+        o2[v] = a.f0 * a.y0[it * 4 + v] + (a.x0 ? a.f1 * a.x0[it * 4 + v] : 0.0) + a.f2 * o[v];
+      *reinterpret_cast<double4 *>(a.out2 + it * 4) = make_double4(o2[0], o2[1], o2[2], o2[3]);
+    }
     // metric cotangents: d/drx[i] = mu[i] . (Drw F)[i], etc.
     double dFdr[4] = {0, 0, 0, 0}, dFds[4] = {0, 0, 0, 0}, dGdr[4] = {0, 0, 0, 0}, dGds[4] = {0, 0, 0, 0};
     for (int k = 0; k < Np; k++) {
-      double q[4], F[4], G[4];
-      for (int v = 0; v < 4; v++) q[v] = a.w[(c * Np + k) * 4 + v];
-      euler_flux(ph.gam, q, F, G);
       const double ar = Drw[i * Np + k], as = Dsw[i * Np + k];
       for (int v = 0; v < 4; v++) {
-        dFdr[v] += ar * F[v]; dFds[v] += as * F[v]; dGdr[v] += ar * G[v]; dGds[v] += as * G[v];
+        const double F = sFG[v][c0 + k], G = sFG[4 + v][c0 + k];
+        dFdr[v] += ar * F; dFds[v] += as * F; dGdr[v] += ar * G; dGds[v] += as * G;
       }
     }
     double grx = 0, gsx = 0, gry = 0, gsy = 0;
     for (int v = 0; v < 4; v++) {
-      const double mu = a.mu[it * 4 + v];
+      const double mu = sMu[v][t];
       grx += mu * dFdr[v]; gsx += mu * dFds[v]; gry += mu * dGdr[v]; gsy += mu * dGds[v];
     }
     a.g_rx[it] += a.scale * grx;
@@ -460,6 +511,62 @@ __global__ void record_dt_kernel(const unsigned long long *dt1, double cfl, doub
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = cfl / __longlong_as_double((long long)*dt1);
 }
 
+template <int N>
+__global__ __launch_bounds__(128) void geo_vjp_kernel(int ncells, const double *__restrict__ ops,
+                                                      const int *__restrict__ fmask,
+                                                      const double *__restrict__ cell_xy, GeoCot g,
+                                                      double *gv) {
+  constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
+  const double *Dr = ops + 2 * Np * Np + Np * NF;
+  const double *Ds = Dr + Np * Np;
+  const double *r = Ds + Np * Np;
+  const double *s = r + Np;
+  const size_t n = (size_t)ncells * 6;
+  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
+       it += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = it / 6;
+    const int d = (int)(it - c * 6);
+    Dual v[6];
+    for (int k = 0; k < 6; k++) v[k] = Dual(cell_xy[6 * c + k], k == d ? 1.0 : 0.0);
+    Dual rx[Np], ry[Np], sx[Np], sy[Np], nx[NF], ny[NF], fs[NF], Jc;
+    init_grid_cell_t<N, Dual>(r, s, Dr, Ds, fmask, v, rx, ry, sx, sy, nx, ny, fs, Jc);
+    double acc = g.J ? g.J[c] * Jc.d : 0.0;
+    for (int i = 0; i < Np; i++) {
+      const size_t k = c * Np + i;
+      acc += g.rx[k] * rx[i].d + g.ry[k] * ry[i].d + g.sx[k] * sx[i].d + g.sy[k] * sy[i].d;
+    }
+    for (int m = 0; m < NF; m++) {
+      const size_t k = c * NF + m;
+      acc += g.nx[k] * nx[m].d + g.ny[k] * ny[m].d + g.fs[k] * fs[m].d;
+    }
+    gv[it] = acc;
+  }
+}
+
+// partial[b] = sum of x[i * stride + offset] over block b's contiguous
+// chunk of i, in a fixed order (deterministic for a fixed grid).
+__global__ __launch_bounds__(256) void strided_sum_kernel(size_t n, int stride, int offset,
+                                                          const double *__restrict__ x, double *partial) {
+  __shared__ double red[256];
+  const size_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const size_t i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double acc = 0.0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += 256) acc += x[i * stride + offset];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void add_indexed_kernel(int n, const int *__restrict__ idx, const double *__restrict__ val,
+                                   double *dst) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[idx[i]] += val[i];
+}
+
 static int grid_of(size_t n) {
   size_t g = (n + 255) / 256;
   if (g > 16384) g = 16384;
@@ -476,14 +583,20 @@ static hipError_t tlm_n(const Launch &L, const TlmArgs &a, double bcud, double b
   return hipGetLastError();
 }
 
+static int cell_blocks(size_t ncells, int cells_per_block) {
+  size_t g = (ncells + cells_per_block - 1) / cells_per_block;
+  if (g > 65536) g = 65536;
+  return g < 1 ? 1 : (int)g;
+}
+
 template <int N>
 static hipError_t adj_n(const Launch &L, const AdjArgs &a, const int *node_slots) {
-  const size_t nf = (size_t)a.ncells * Ord<N>::NF, nc = (size_t)a.ncells * Ord<N>::Np;
-  hipLaunchKernelGGL(adj_face_kernel<N>, dim3(grid_of(nf)), dim3(256), 0, L.stream, a, L.ph, L.ops);
+  hipLaunchKernelGGL(adj_face_kernel<N>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::NF)),
+                     dim3(kAdjThreads), 0, L.stream, a, L.ph, L.ops);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(adj_cell_kernel<N>, dim3(grid_of(nc)), dim3(256), 0, L.stream, a, L.ph, L.ops,
-                     node_slots);
+  hipLaunchKernelGGL(adj_cell_kernel<N>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::Np)),
+                     dim3(kAdjThreads), 0, L.stream, a, L.ph, L.ops, node_slots);
   return hipGetLastError();
 }
 
@@ -518,6 +631,34 @@ hipError_t launch_lincomb(const Launch &L, size_t n, double a, const double *x, 
 
 hipError_t launch_record_dt(const Launch &L, const unsigned long long *dt1, double cfl, double *dst) {
   hipLaunchKernelGGL(record_dt_kernel, dim3(1), dim3(64), 0, L.stream, dt1, cfl, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_xy, const GeoCot &g,
+                          double *gv) {
+  if (ncells == 0) return hipSuccess;
+  size_t nb = ((size_t)ncells * 6 + 127) / 128;
+  const int grid = nb > 65536 ? 65536 : (int)nb;
+  switch (L.N) {
+    case 1: hipLaunchKernelGGL(geo_vjp_kernel<1>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_xy, g, gv); break;
+    case 2: hipLaunchKernelGGL(geo_vjp_kernel<2>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_xy, g, gv); break;
+    case 3: hipLaunchKernelGGL(geo_vjp_kernel<3>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_xy, g, gv); break;
+    case 4: hipLaunchKernelGGL(geo_vjp_kernel<4>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_xy, g, gv); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_strided_sum(const Launch &L, size_t n, int stride, int offset, const double *x,
+                              double *partial) {
+  hipLaunchKernelGGL(strided_sum_kernel, dim3(kSumBlocks), dim3(256), 0, L.stream, n, stride, offset, x,
+                     partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_indexed(const Launch &L, int n, const int *idx, const double *val, double *dst) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(add_indexed_kernel, dim3((n + 255) / 256), dim3(256), 0, L.stream, n, idx, val, dst);
   return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@
 // arithmetic, one direction per vertex coordinate.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -34,6 +35,7 @@
 #include "dg_adjoint.h"
 #include "dg_ctx.h"
 #include "dg_dual.h"
+#include "dg_geom.h"
 #include "dg_internal.h"
 #include "dg_mesh.h"
 #include "dg_ops.h"
@@ -44,40 +46,17 @@ using dg::hipfail;
 namespace {
 
 // init_grid of one cell in generic arithmetic, as init_grid_kernel
-// (src/init_grid.h:1-106).
+// (src/init_grid.h:1-106), for the context's order.
 template <class T>
 void init_grid_cell(const dg_ctx *c, const T v[6], T *rx, T *ry, T *sx, T *sy, T *nx, T *ny, T *fs,
                     T &Jc) {
-  const int Np = c->Np, Nfp = c->Nfp, NF = c->NF;
-  std::vector<T> X(Np), Y(Np), XR(Np), XS(Np), YR(Np), YS(Np), J(Np);
-  for (int d = 0; d < 2; d++) {
-    std::vector<T> &o = d == 0 ? X : Y;
-    T a1 = 0.5 * v[2 + d], a2 = 0.5 * v[4 + d], a0 = -0.5 * v[d];
-    for (int i = 0; i < Np; i++) o[i] = a1 * (1.0 + c->r[i]) + a2 * (1.0 + c->s[i]) + a0 * (c->s[i] + c->r[i]);
-  }
-  for (int j = 0; j < Np; j++) {
-    T a = 0.0, b = 0.0, cc = 0.0, dd = 0.0;
-    for (int i = 0; i < Np; i++) {
-      a += c->Dr[j * Np + i] * X[i];
-      b += c->Ds[j * Np + i] * X[i];
-      cc += c->Dr[j * Np + i] * Y[i];
-      dd += c->Ds[j * Np + i] * Y[i];
-    }
-    XR[j] = a; XS[j] = b; YR[j] = cc; YS[j] = dd;
-  }
-  for (int i = 0; i < Np; i++) {
-    J[i] = -XS[i] * YR[i] + XR[i] * YS[i];
-    rx[i] = YS[i] / J[i]; sx[i] = -YR[i] / J[i]; ry[i] = -XS[i] / J[i]; sy[i] = XR[i] / J[i];
-  }
-  Jc = J[0];
-  for (int m = 0; m < NF; m++) {
-    const int f = c->FMASK[m], face = m / Nfp;
-    T a, b;
-    if (face == 0) { a = YR[f]; b = -XR[f]; }
-    else if (face == 1) { a = YS[f] - YR[f]; b = XR[f] - XS[f]; }
-    else { a = -YS[f]; b = XS[f]; }
-    T sJ = sqrt(a * a + b * b);
-    nx[m] = a / sJ; ny[m] = b / sJ; fs[m] = sJ / J[f];
+  const double *r = c->r.data(), *s = c->s.data(), *Dr = c->Dr.data(), *Ds = c->Ds.data();
+  const int *fm = c->FMASK.data();
+  switch (c->N) {
+    case 1: dg::init_grid_cell_t<1, T>(r, s, Dr, Ds, fm, v, rx, ry, sx, sy, nx, ny, fs, Jc); break;
+    case 2: dg::init_grid_cell_t<2, T>(r, s, Dr, Ds, fm, v, rx, ry, sx, sy, nx, ny, fs, Jc); break;
+    case 3: dg::init_grid_cell_t<3, T>(r, s, Dr, Ds, fm, v, rx, ry, sx, sy, nx, ny, fs, Jc); break;
+    default: dg::init_grid_cell_t<4, T>(r, s, Dr, Ds, fm, v, rx, ry, sx, sy, nx, ny, fs, Jc); break;
   }
 }
 
@@ -131,6 +110,10 @@ struct dg_sens {
   std::vector<int> h_wcell, h_wface;
   int *wcell, *wface;
   double *facew, *wforce, *wcp;
+  // geometry reverse: wall-slot indices and C_L's own cotangents there
+  // (nx, ny, fscale), J cotangents, and the vertex-coordinate cotangents
+  int *wslot;
+  double *wcot, *gJ, *gv;
   std::vector<double> h_facew;
   // recorded time steps
   std::vector<double> h_dts;
@@ -138,7 +121,11 @@ struct dg_sens {
   int nsteps = 0;
   double cfl = 1.0;
   double fwd_ms = 0.0, grad_ms = 0.0, tlm_ms = 0.0;
+  double grad_fwd_ms = 0.0, grad_rev_ms = 0.0, grad_host_ms = 0.0;
+  // checkpoints (state every ckpt steps) and the segment store: q, w1, w2 of
+  // each step of one segment, [3 * j + {0, 1, 2}]
   std::vector<double *> ckpt, seg;
+  int seg_len = 0;
 };
 
 namespace {
@@ -189,9 +176,11 @@ dg::StageArgs stage_args(dg_sens *s) {
   return a;
 }
 
-// One SSPRK3 step from qin (in place) with w1 -> wA, w2 -> wB; frozen dt
+// One SSPRK3 step of q with the stage states w1 -> w1out, w2 -> w2out and the
+// new state -> qout (qout may be q; null skips the third stage); frozen dt
 // dts[n] unless recording (then dt = cfl / dt1 and dts[n] is recorded).
-hipError_t step(dg_sens *s, double *qin, int n, bool record) {
+hipError_t step3(dg_sens *s, const double *q, double *w1out, double *w2out, double *qout, int n,
+                 bool record) {
   dg::Launch L = launch_a(s);
   dg::StageArgs a = stage_args(s);
   const int p = n & 1;
@@ -201,29 +190,29 @@ hipError_t step(dg_sens *s, double *qin, int n, bool record) {
   if (record) e = dg::launch_record_dt(L, a.dt1_cur, s->cfl, s->dts + n);
   else a.dt_fixed = s->dts + n;
   if (e != hipSuccess) return e;
-  a.q = qin;
-  a.w = qin; a.out = s->wA;
+  a.q = q;
+  a.w = q; a.out = w1out;
   if ((e = dg::launch_rhs(L, a, true, dg::EPI_STAGE0)) != hipSuccess) return e;
-  a.w = s->wA; a.out = s->wB;
+  a.w = w1out; a.out = w2out;
   if ((e = dg::launch_rhs(L, a, true, dg::EPI_STAGE1)) != hipSuccess) return e;
-  a.w = s->wB; a.out = qin;
+  if (!qout) return hipSuccess;
+  a.w = w2out; a.out = qout;
   return dg::launch_rhs(L, a, true, dg::EPI_STAGE2);
+}
+
+// One SSPRK3 step of qin in place (stage states in wA, wB).
+hipError_t step(dg_sens *s, double *qin, int n, bool record) {
+  return step3(s, qin, s->wA, s->wB, qin, n, record);
 }
 
 // w1 -> wA and w2 -> wB of step n from q (q unchanged)
 hipError_t stages01(dg_sens *s, const double *q, int n) {
-  dg::Launch L = launch_a(s);
-  dg::StageArgs a = stage_args(s);
-  a.dt1_cur = s->dt1 + (n & 1);
-  a.dt1_next = s->dt1 + ((n & 1) ^ 1);
-  a.dt_fixed = s->dts + n;
-  a.q = q;
-  a.w = q; a.out = s->wA;
-  hipError_t e = dg::launch_rhs(L, a, true, dg::EPI_STAGE0);
-  if (e != hipSuccess) return e;
-  a.w = s->wA; a.out = s->wB;
-  return dg::launch_rhs(L, a, true, dg::EPI_STAGE1);
+  return step3(s, q, s->wA, s->wB, nullptr, n, false);
 }
+
+double *seg_q(dg_sens *s, int j) { return s->seg[3 * j]; }
+double *seg_w1(dg_sens *s, int j) { return s->seg[3 * j + 1]; }
+double *seg_w2(dg_sens *s, int j) { return s->seg[3 * j + 2]; }
 
 int set_ic(dg_sens *s, double *q) {
   HIPCHECK(s->ctx, dg::launch_set_ic(launch_a(s), s->K, q, s->wA));
@@ -265,11 +254,21 @@ int run_forward(dg_sens *s, bool record, int ckpt_every, double *cl) {
     HIPCHECK(c, hipMemsetAsync(s->dt1, 0, 2 * sizeof(unsigned long long), s->stream));
     HIPCHECK(c, dg::launch_calc_dt(launch_a(s), s->K, s->q, s->fs, s->dt1));
   }
+  // With checkpoints, the steps of the last segment keep q, w1, w2 in the
+  // segment store, so the reverse sweep need not recompute that segment.
+  const size_t bytes = sizeof(double) * (size_t)s->K * s->Np * 4;
+  const int last0 = ckpt_every > 0 ? ((s->nsteps - 1) / ckpt_every) * ckpt_every : s->nsteps;
   for (int n = 0; n < s->nsteps; n++) {
     if (ckpt_every > 0 && n % ckpt_every == 0)
-      HIPCHECK(c, hipMemcpyAsync(s->ckpt[n / ckpt_every], s->q, sizeof(double) * (size_t)s->K * s->Np * 4,
-                                 hipMemcpyDeviceToDevice, s->stream));
-    HIPCHECK(c, step(s, s->q, n, record));
+      HIPCHECK(c, hipMemcpyAsync(s->ckpt[n / ckpt_every], s->q, bytes, hipMemcpyDeviceToDevice, s->stream));
+    if (n < last0) {
+      HIPCHECK(c, step(s, s->q, n, record));
+    } else {
+      const int j = n - last0;
+      if (j == 0) HIPCHECK(c, hipMemcpyAsync(seg_q(s, 0), s->q, bytes, hipMemcpyDeviceToDevice, s->stream));
+      double *qout = n + 1 < s->nsteps ? seg_q(s, j + 1) : s->q;
+      HIPCHECK(c, step3(s, seg_q(s, j), seg_w1(s, j), seg_w2(s, j), qout, n, record));
+    }
   }
   double fx, fy;
   int rc = forces(s, s->q, &fx, &fy);
@@ -295,34 +294,6 @@ int wall_data(dg_sens *s, const double *q, WallData &d) {
   HIPCHECK(c, hipMemcpy(d.fs.data(), s->fs, sizeof(double) * d.fs.size(), hipMemcpyDeviceToHost));
   HIPCHECK(c, hipMemcpy(d.J.data(), s->J, sizeof(double) * d.J.size(), hipMemcpyDeviceToHost));
   return DG_OK;
-}
-
-// Per-cell Jacobian of the init_grid factors with respect to the 6 vertex
-// coordinates; out[d] holds the derivative arrays for direction d.
-struct CellGeoJac {
-  std::vector<double> rx, ry, sx, sy, nx, ny, fs, J;  // [6][Np] / [6][NF] / [6]
-};
-
-void cell_geo_jac(const dg_sens *s, const double *v, CellGeoJac &jac) {
-  const dg_ctx *c = s->ctx;
-  const int Np = s->Np, NF = s->NF;
-  jac.rx.assign(6 * Np, 0.0); jac.ry = jac.rx; jac.sx = jac.rx; jac.sy = jac.rx;
-  jac.nx.assign(6 * NF, 0.0); jac.ny = jac.nx; jac.fs = jac.nx; jac.J.assign(6, 0.0);
-  std::vector<dg::Dual> rx(Np), ry(Np), sx(Np), sy(Np), nx(NF), ny(NF), fs(NF);
-  for (int d = 0; d < 6; d++) {
-    dg::Dual vd[6];
-    for (int k = 0; k < 6; k++) vd[k] = dg::Dual(v[k], k == d ? 1.0 : 0.0);
-    dg::Dual Jc;
-    init_grid_cell<dg::Dual>(c, vd, rx.data(), ry.data(), sx.data(), sy.data(), nx.data(), ny.data(), fs.data(), Jc);
-    for (int i = 0; i < Np; i++) {
-      jac.rx[d * Np + i] = rx[i].d; jac.ry[d * Np + i] = ry[i].d;
-      jac.sx[d * Np + i] = sx[i].d; jac.sy[d * Np + i] = sy[i].d;
-    }
-    for (int m = 0; m < NF; m++) {
-      jac.nx[d * NF + m] = nx[m].d; jac.ny[d * NF + m] = ny[m].d; jac.fs[d * NF + m] = fs[m].d;
-    }
-    jac.J[d] = Jc.d;
-  }
 }
 
 // d(node y)/d(bump i) for every node (x does not move)
@@ -411,6 +382,10 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   s->wcell = b.alloc<int>(s->nwall); s->wface = b.alloc<int>(s->nwall);
   s->facew = b.alloc<double>(s->Nfp); s->wforce = b.alloc<double>(2 * (size_t)s->nwall);
   s->wcp = b.alloc<double>((size_t)s->nwall * s->Nfp);
+  s->wslot = b.alloc<int>((size_t)s->nwall * s->Nfp);
+  s->wcot = b.alloc<double>(3 * (size_t)s->nwall * s->Nfp);
+  s->gJ = b.alloc<double>(K);
+  s->gv = b.alloc<double>(std::max<size_t>(6 * K, 5 * dg::kSumBlocks));  // also the partial sums
   e = b.err;
   dg::Operators o;
   const char *oe = nullptr;
@@ -540,7 +515,7 @@ int dg_sens_tangent(dg_sens *s, double dalpha, const double *dbumps, double *cl,
   t.fmap = s->fmap; t.fmask = c->d_fmask; t.fluxd = s->fluxd; t.out = s->Ld;
   for (int n = 0; n < s->nsteps; n++) {
     const double dt = s->h_dts[n];
-    HIPCHECK(c, stages01(s, s->q, n));
+    HIPCHECK(c, step3(s, s->q, s->wA, s->wB, s->qw, n, false));
     t.w = s->q; t.wd = s->qd;
     HIPCHECK(c, dg::launch_rhs_tlm(L, t, bcud, bcvd));
     HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->qd, dt, s->Ld, s->w1d));
@@ -552,11 +527,15 @@ int dg_sens_tangent(dg_sens *s, double dalpha, const double *dbumps, double *cl,
     HIPCHECK(c, dg::launch_rhs_tlm(L, t, bcud, bcvd));
     HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0 / 3.0, s->qd, 2.0 / 3.0, s->w2d, s->tmp));
     HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->tmp, 2.0 / 3.0 * dt, s->Ld, s->qd));
-    HIPCHECK(c, step(s, s->q, n, false));
+    std::swap(s->q, s->qw);
   }
-  // C_L and its tangent on the host
+  // C_L (device forces, as every forward evaluation) and its tangent (host,
+  // dual arithmetic)
+  double fx0, fy0;
+  int rc = forces(s, s->q, &fx0, &fy0);
+  if (rc != DG_OK) return rc;
   WallData wd;
-  int rc = wall_data(s, s->q, wd);
+  rc = wall_data(s, s->q, wd);
   if (rc != DG_OK) return rc;
   std::vector<double> qdh(ST);
   HIPCHECK(c, hipMemcpy(qdh.data(), s->qd, sizeof(double) * ST, hipMemcpyDeviceToHost));
@@ -582,7 +561,7 @@ int dg_sens_tangent(dg_sens *s, double dalpha, const double *dbumps, double *cl,
   const double al = s->alpha * rad, qd0 = qdyn_of(s);
   dg::Dual A(al, rad * dalpha);
   dg::Dual CL = (Fy * dg::cos(A) - Fx * dg::sin(A)) / qd0;
-  if (cl) *cl = CL.v;
+  if (cl) *cl = lift_of(s, fx0, fy0);
   *dcl = CL.d;
   s->tlm_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return DG_OK;
@@ -600,15 +579,19 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
     HIPCHECK(c, hipMalloc(&p, sizeof(double) * ST));
     s->ckpt.push_back(p);
   }
-  while ((int)s->seg.size() < ckpt_every) {
+  while ((int)s->seg.size() < 3 * ckpt_every) {
     double *p = nullptr;
     HIPCHECK(c, hipMalloc(&p, sizeof(double) * ST));
     s->seg.push_back(p);
   }
-  auto t0 = std::chrono::steady_clock::now();
+  using clk = std::chrono::steady_clock;
+  auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+  auto t0 = clk::now();
   double clv = 0.0;
   int rc = run_forward(s, false, ckpt_every, &clv);
   if (rc != DG_OK) return rc;
+  s->grad_fwd_ms = ms_since(t0);
+  auto th = clk::now();
   // cotangent of C_L with respect to q_N, the wall geometry and alpha
   WallData wd;
   rc = wall_data(s, s->q, wd);
@@ -631,8 +614,14 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
     }
     Fx += bx; Fy += by;
   }
+  // cotangents of C_L: state at the wall nodes (sparse, merged per node
+  // entry), nx / ny / fscale on the wall slots, J of the wall cells
   const double kx = -std::sin(al) / qd0, ky = std::cos(al) / qd0;
-  std::vector<double> lamN(ST, 0.0), hnx(K * NF, 0.0), hny(K * NF, 0.0), hfs(K * NF, 0.0), hJ(K, 0.0);
+  const size_t nws = (size_t)s->nwall * s->Nfp;
+  std::vector<std::pair<int, double>> lamN;
+  lamN.reserve(4 * nws);
+  std::vector<int> wslot(nws);
+  std::vector<double> wcot(3 * nws), hJ(K, 0.0);
   for (int w = 0; w < s->nwall; w++) {
     const size_t k = s->h_wcell[w];
     const int f = s->h_wface[w];
@@ -646,23 +635,48 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
       const double wq = s->h_facew[i];
       const double kn = kx * nxv + ky * nyv;
       const double dp = wq * fsv * Jv * kn;
-      lamN[4 * node] += dp * g1 * 0.5 * (u * u + v * v);
-      lamN[4 * node + 1] += dp * (-g1 * u);
-      lamN[4 * node + 2] += dp * (-g1 * v);
-      lamN[4 * node + 3] += dp * g1;
-      hnx[k * NF + m] += wq * fsv * Jv * p * kx;
-      hny[k * NF + m] += wq * fsv * Jv * p * ky;
-      hfs[k * NF + m] += wq * Jv * p * kn;
+      lamN.emplace_back((int)(4 * node), dp * g1 * 0.5 * (u * u + v * v));
+      lamN.emplace_back((int)(4 * node + 1), dp * (-g1 * u));
+      lamN.emplace_back((int)(4 * node + 2), dp * (-g1 * v));
+      lamN.emplace_back((int)(4 * node + 3), dp * g1);
+      const size_t ws = (size_t)w * s->Nfp + i;
+      wslot[ws] = (int)(k * NF + m);
+      wcot[ws] = wq * fsv * Jv * p * kx;
+      wcot[nws + ws] = wq * fsv * Jv * p * ky;
+      wcot[2 * nws + ws] = wq * Jv * p * kn;
       hJ[k] += wq * fsv * p * kn;
     }
   }
+  // merge entries of nodes shared by two wall faces of one cell (in order)
+  std::stable_sort(lamN.begin(), lamN.end(),
+                   [](const std::pair<int, double> &x, const std::pair<int, double> &y) { return x.first < y.first; });
+  std::vector<int> lidx;
+  std::vector<double> lval;
+  for (const auto &e : lamN) {
+    if (!lidx.empty() && lidx.back() == e.first) lval.back() += e.second;
+    else { lidx.push_back(e.first); lval.push_back(e.second); }
+  }
   double dalpha = (-Fy * std::sin(al) - Fx * std::cos(al)) / qd0 * rad;
-  HIPCHECK(c, hipMemcpy(s->lam, lamN.data(), sizeof(double) * ST, hipMemcpyHostToDevice));
+  dg::Launch L = launch_a(s);
+  HIPCHECK(c, hipMemsetAsync(s->lam, 0, sizeof(double) * ST, s->stream));
+  HIPCHECK(c, hipMemcpyAsync(s->gmq, lidx.data(), sizeof(int) * lidx.size(), hipMemcpyHostToDevice, s->stream));
+  HIPCHECK(c, hipMemcpyAsync(s->gpq, lval.data(), sizeof(double) * lval.size(), hipMemcpyHostToDevice, s->stream));
+  HIPCHECK(c, dg::launch_add_indexed(L, (int)lidx.size(), reinterpret_cast<const int *>(s->gmq), s->gpq, s->lam));
   double *acc[9] = {s->g_rx, s->g_ry, s->g_sx, s->g_sy, s->g_nx, s->g_ny, s->g_fs, s->g_bcu, s->g_bcv};
   for (int a = 0; a < 9; a++)
     HIPCHECK(c, hipMemsetAsync(acc[a], 0, sizeof(double) * (a < 4 ? K * Np : K * NF), s->stream));
-  // reverse sweep over checkpoint segments
-  dg::Launch L = launch_a(s);
+  HIPCHECK(c, hipMemcpyAsync(s->wslot, wslot.data(), sizeof(int) * nws, hipMemcpyHostToDevice, s->stream));
+  HIPCHECK(c, hipMemcpyAsync(s->wcot, wcot.data(), sizeof(double) * 3 * nws, hipMemcpyHostToDevice, s->stream));
+  HIPCHECK(c, hipMemcpyAsync(s->gJ, hJ.data(), sizeof(double) * K, hipMemcpyHostToDevice, s->stream));
+  HIPCHECK(c, hipStreamSynchronize(s->stream));  // host vectors leave scope
+  s->grad_host_ms = ms_since(th);
+  // reverse sweep over checkpoint segments; each segment but the last is
+  // recomputed from its checkpoint into the segment store (q, w1, w2 of
+  // every step), then reversed step by step:
+  //   A   = 2/3 lam' + 2/3 dt L'(w2)^T lam'
+  //   B   = 1/4 A + 1/4 dt L'(w1)^T A,     T = 1/3 lam' + 3/4 A + B
+  //   lam = T + dt L'(q)^T B
+  auto tr = clk::now();
   dg::AdjArgs a{};
   a.ncells = s->K;
   a.rx = s->rx; a.ry = s->ry; a.sx = s->sx; a.sy = s->sy; a.nx = s->nx; a.ny = s->ny; a.fscale = s->fs;
@@ -671,90 +685,69 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   a.g_nx = s->g_nx; a.g_ny = s->g_ny; a.g_fs = s->g_fs; a.g_bcu = s->g_bcu; a.g_bcv = s->g_bcv;
   for (int sg = nck - 1; sg >= 0; sg--) {
     const int n0 = sg * ckpt_every, n1 = std::min(s->nsteps, n0 + ckpt_every);
-    HIPCHECK(c, hipMemcpyAsync(s->seg[0], s->ckpt[sg], sizeof(double) * ST, hipMemcpyDeviceToDevice, s->stream));
-    for (int n = n0; n < n1 - 1; n++) {
-      HIPCHECK(c, hipMemcpyAsync(s->qw, s->seg[n - n0], sizeof(double) * ST, hipMemcpyDeviceToDevice, s->stream));
-      HIPCHECK(c, step(s, s->qw, n, false));
-      HIPCHECK(c, hipMemcpyAsync(s->seg[n - n0 + 1], s->qw, sizeof(double) * ST, hipMemcpyDeviceToDevice, s->stream));
+    if (sg != nck - 1) {
+      HIPCHECK(c, hipMemcpyAsync(seg_q(s, 0), s->ckpt[sg], sizeof(double) * ST, hipMemcpyDeviceToDevice, s->stream));
+      for (int n = n0; n < n1; n++) {
+        const int j = n - n0;
+        HIPCHECK(c, step3(s, seg_q(s, j), seg_w1(s, j), seg_w2(s, j), n + 1 < n1 ? seg_q(s, j + 1) : nullptr,
+                          n, false));
+      }
     }
     for (int n = n1 - 1; n >= n0; n--) {
       const double dt = s->h_dts[n];
-      const double *qn = s->seg[n - n0];
-      HIPCHECK(c, stages01(s, qn, n));  // w1 -> wA, w2 -> wB
-      a.scale = dt;
-      // q' = 1/3 q + 2/3 (w2 + dt L(w2))
-      HIPCHECK(c, dg::launch_lincomb(L, ST, 2.0 / 3.0, s->lam, 0.0, nullptr, s->M));
-      HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->M, 0.0, nullptr, s->A));
-      a.w = s->wB; a.mu = s->M; a.acc = s->A;
+      const int j = n - n0;
+      a.w = seg_w2(s, j); a.mu = s->lam;
+      a.x0 = s->lam; a.e0 = 2.0 / 3.0; a.scale = 2.0 / 3.0 * dt; a.out = s->A; a.out2 = nullptr;
       HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
-      HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0 / 3.0, s->lam, 0.75, s->A, s->lamq));
-      // w2 = 3/4 q + 1/4 (w1 + dt L(w1))
-      HIPCHECK(c, dg::launch_lincomb(L, ST, 0.25, s->A, 0.0, nullptr, s->M));
-      HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->M, 0.0, nullptr, s->B));
-      a.w = s->wA; a.mu = s->M; a.acc = s->B;
+      a.w = seg_w1(s, j); a.mu = s->A;
+      a.x0 = s->A; a.e0 = 0.25; a.scale = 0.25 * dt; a.out = s->B;
+      a.out2 = s->lam; a.y0 = s->lam; a.f0 = 1.0 / 3.0; a.f1 = 0.75; a.f2 = 1.0;
       HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
-      // w1 = q + dt L(q)
-      HIPCHECK(c, dg::launch_lincomb(L, ST, 1.0, s->lamq, 1.0, s->B, s->lamq));
-      a.w = qn; a.mu = s->B; a.acc = s->lamq;
+      a.w = seg_q(s, j); a.mu = s->B;
+      a.x0 = s->lam; a.e0 = 1.0; a.scale = dt; a.out = s->lam; a.out2 = nullptr;
       HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
-      std::swap(s->lam, s->lamq);
     }
   }
-  // initial state: q0 = free stream(alpha)
-  std::vector<double> lam0(ST);
   HIPCHECK(c, hipStreamSynchronize(s->stream));
-  HIPCHECK(c, hipMemcpy(lam0.data(), s->lam, sizeof(double) * ST, hipMemcpyDeviceToHost));
-  const dg::Phys ph = L.ph;
-  const double dbcu = -ph.bc_v * rad, dbcv = ph.bc_u * rad;  // per degree
+  s->grad_rev_ms = ms_since(tr);
+  th = clk::now();
+  // alpha through the initial state q0 = free stream(alpha) (sums of the
+  // momentum and energy components of lam_0) and through the free-stream
+  // velocity the boundary sees (sums of the bc cotangents); fixed-order
+  // device partial sums, added on the host in index order
   {
-    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    for (size_t i = 0; i < K * Np; i++) { s1 += lam0[4 * i + 1]; s2 += lam0[4 * i + 2]; s3 += lam0[4 * i + 3]; }
-    dalpha += ph.bc_r * (s1 * dbcu + s2 * dbcv) + s3 * ph.bc_r * (ph.bc_u * dbcu + ph.bc_v * dbcv);
+    const int nbk = dg::kSumBlocks;
+    HIPCHECK(c, dg::launch_strided_sum(L, K * Np, 4, 1, s->lam, s->gv));
+    HIPCHECK(c, dg::launch_strided_sum(L, K * Np, 4, 2, s->lam, s->gv + nbk));
+    HIPCHECK(c, dg::launch_strided_sum(L, K * Np, 4, 3, s->lam, s->gv + 2 * nbk));
+    HIPCHECK(c, dg::launch_strided_sum(L, K * NF, 1, 0, s->g_bcu, s->gv + 3 * nbk));
+    HIPCHECK(c, dg::launch_strided_sum(L, K * NF, 1, 0, s->g_bcv, s->gv + 4 * nbk));
+    std::vector<double> part(5 * nbk);
+    HIPCHECK(c, hipMemcpyAsync(part.data(), s->gv, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(c, hipStreamSynchronize(s->stream));
+    double sum[5] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < 5; q++)
+      for (int b = 0; b < nbk; b++) sum[q] += part[q * nbk + b];
+    const dg::Phys ph = L.ph;
+    const double dbcu = -ph.bc_v * rad, dbcv = ph.bc_u * rad;  // per degree
+    dalpha += ph.bc_r * (sum[0] * dbcu + sum[1] * dbcv) + sum[2] * ph.bc_r * (ph.bc_u * dbcu + ph.bc_v * dbcv);
+    dalpha += sum[3] * dbcu + sum[4] * dbcv;
   }
-  // free-stream velocity seen by the boundary
-  std::vector<double> gbu(K * NF), gbv(K * NF);
-  HIPCHECK(c, hipMemcpy(gbu.data(), s->g_bcu, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
-  HIPCHECK(c, hipMemcpy(gbv.data(), s->g_bcv, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
-  {
-    double su = 0.0, sv = 0.0;
-    for (size_t i = 0; i < K * NF; i++) { su += gbu[i]; sv += gbv[i]; }
-    dalpha += su * dbcu + sv * dbcv;
-  }
-  // geometry: device cotangents + those of C_L, chained to the bumps
+  // geometry: device cotangents + those of C_L, through the reverse of
+  // init_grid (device, per cell and vertex coordinate) to the node y
+  // coordinates (host, in cell order) and on to the bumps
   if (dcl_db && nb > 0) {
-    std::vector<double> grx(K * Np), gry(K * Np), gsx(K * Np), gsy(K * Np), gnx(K * NF), gny(K * NF), gfs(K * NF);
-    HIPCHECK(c, hipMemcpy(grx.data(), s->g_rx, sizeof(double) * K * Np, hipMemcpyDeviceToHost));
-    HIPCHECK(c, hipMemcpy(gry.data(), s->g_ry, sizeof(double) * K * Np, hipMemcpyDeviceToHost));
-    HIPCHECK(c, hipMemcpy(gsx.data(), s->g_sx, sizeof(double) * K * Np, hipMemcpyDeviceToHost));
-    HIPCHECK(c, hipMemcpy(gsy.data(), s->g_sy, sizeof(double) * K * Np, hipMemcpyDeviceToHost));
-    HIPCHECK(c, hipMemcpy(gnx.data(), s->g_nx, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
-    HIPCHECK(c, hipMemcpy(gny.data(), s->g_ny, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
-    HIPCHECK(c, hipMemcpy(gfs.data(), s->g_fs, sizeof(double) * K * NF, hipMemcpyDeviceToHost));
+    HIPCHECK(c, dg::launch_add_indexed(L, (int)nws, s->wslot, s->wcot, s->g_nx));
+    HIPCHECK(c, dg::launch_add_indexed(L, (int)nws, s->wslot, s->wcot + nws, s->g_ny));
+    HIPCHECK(c, dg::launch_add_indexed(L, (int)nws, s->wslot, s->wcot + 2 * nws, s->g_fs));
+    const dg::GeoCot gc{s->g_rx, s->g_ry, s->g_sx, s->g_sy, s->g_nx, s->g_ny, s->g_fs, s->gJ};
+    HIPCHECK(c, dg::launch_geo_vjp(L, s->K, s->cell_xy, gc, s->gv));
+    std::vector<double> gv(6 * K);
+    HIPCHECK(c, hipMemcpyAsync(gv.data(), s->gv, sizeof(double) * 6 * K, hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(c, hipStreamSynchronize(s->stream));
     std::vector<double> gnode_y(s->mesh.nnodes, 0.0);
-    CellGeoJac jac;
-    for (size_t k = 0; k < K; k++) {
-      double gv[6] = {0, 0, 0, 0, 0, 0};
-      bool any = hJ[k] != 0.0;
-      for (size_t i = 0; i < Np && !any; i++)
-        any = grx[k * Np + i] != 0.0 || gry[k * Np + i] != 0.0 || gsx[k * Np + i] != 0.0 || gsy[k * Np + i] != 0.0;
-      for (size_t m = 0; m < NF && !any; m++)
-        any = gnx[k * NF + m] + hnx[k * NF + m] != 0.0 || gny[k * NF + m] + hny[k * NF + m] != 0.0 ||
-              gfs[k * NF + m] + hfs[k * NF + m] != 0.0;
-      if (!any) continue;
-      cell_geo_jac(s, &s->mesh.cell_xy[6 * k], jac);
-      for (int d = 0; d < 6; d++) {
-        double acc_d = hJ[k] * jac.J[d];
-        for (size_t i = 0; i < Np; i++)
-          acc_d += grx[k * Np + i] * jac.rx[d * Np + i] + gry[k * Np + i] * jac.ry[d * Np + i] +
-                   gsx[k * Np + i] * jac.sx[d * Np + i] + gsy[k * Np + i] * jac.sy[d * Np + i];
-        for (size_t m = 0; m < NF; m++)
-          acc_d += (gnx[k * NF + m] + hnx[k * NF + m]) * jac.nx[d * NF + m] +
-                   (gny[k * NF + m] + hny[k * NF + m]) * jac.ny[d * NF + m] +
-                   (gfs[k * NF + m] + hfs[k * NF + m]) * jac.fs[d * NF + m];
-        gv[d] = acc_d;
-      }
-      for (int t = 0; t < 3; t++) gnode_y[s->mesh.cells[3 * k + t]] += gv[2 * t + 1];
-    }
+    for (size_t k = 0; k < K; k++)
+      for (int t = 0; t < 3; t++) gnode_y[s->mesh.cells[3 * k + t]] += gv[6 * k + 2 * t + 1];
     for (int i = 0; i < nb; i++) {
       double g = 0.0;
       for (int n = 0; n < s->mesh.nnodes; n++)
@@ -764,7 +757,8 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   }
   if (cl) *cl = clv;
   if (dcl_dalpha) *dcl_dalpha = dalpha;
-  s->grad_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  s->grad_host_ms += ms_since(th);
+  s->grad_ms = ms_since(t0);
   return DG_OK;
 }
 
@@ -805,12 +799,15 @@ int dg_rhs_adjoint(dg_ctx *c, const double *w, const double *mu, double scale, c
     HIPCHECK(c, hipMalloc(&c->d_node_slots, sizeof(int) * slots.size()));
     HIPCHECK(c, hipMemcpy(c->d_node_slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice));
   }
+  if (((uintptr_t)w | (uintptr_t)mu | (uintptr_t)acc) & 31u)
+    return fail(c, DG_ERR_ARG, "dg_rhs_adjoint: w, mu and acc must be 32-byte aligned");
   dg::AdjArgs a{};
   a.ncells = c->ncells; a.scale = scale; a.w = w; a.mu = mu;
   a.rx = rx; a.ry = ry; a.sx = sx; a.sy = sy; a.nx = nx; a.ny = ny; a.fscale = fscale;
   a.fmap = c->d_fmap; a.fmask = c->d_fmask;
   a.gmq = scratch; a.gpq = scratch + (size_t)c->ncells * c->NF * 4;
-  a.acc = acc; a.g_rx = grx; a.g_ry = gry; a.g_sx = gsx; a.g_sy = gsy;
+  a.x0 = acc; a.e0 = 1.0; a.out = acc;
+  a.g_rx = grx; a.g_ry = gry; a.g_sx = gsx; a.g_sy = gsy;
   a.g_nx = gnx; a.g_ny = gny; a.g_fs = gfs; a.g_bcu = gbcu; a.g_bcv = gbcv;
   HIPCHECK(c, dg::launch_rhs_adj(dg::launch_of(c), a, c->d_node_slots));
   return DG_OK;
@@ -821,6 +818,14 @@ int dg_sens_timing(const dg_sens *s, double *forward_ms, double *gradient_ms, do
   if (forward_ms) *forward_ms = s->fwd_ms;
   if (gradient_ms) *gradient_ms = s->grad_ms;
   if (tangent_ms) *tangent_ms = s->tlm_ms;
+  return DG_OK;
+}
+
+int dg_sens_gradient_breakdown(const dg_sens *s, double *forward_ms, double *reverse_ms, double *host_ms) {
+  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_sens_gradient_breakdown: null argument");
+  if (forward_ms) *forward_ms = s->grad_fwd_ms;
+  if (reverse_ms) *reverse_ms = s->grad_rev_ms;
+  if (host_ms) *host_ms = s->grad_host_ms;
   return DG_OK;
 }
 

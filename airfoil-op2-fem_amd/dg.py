@@ -450,7 +450,10 @@ class Sensitivity:
     def timing(self):
         f, g, t = c_double(), c_double(), c_double()
         self._chk(lib.dg_sens_timing(self._h, ctypes.byref(f), ctypes.byref(g), ctypes.byref(t)))
-        return dict(forward_ms=f.value, gradient_ms=g.value, tangent_ms=t.value)
+        out = dict(forward_ms=f.value, gradient_ms=g.value, tangent_ms=t.value)
+        self._chk(lib.dg_sens_gradient_breakdown(self._h, ctypes.byref(f), ctypes.byref(g), ctypes.byref(t)))
+        out.update(gradient_forward_ms=f.value, gradient_reverse_ms=g.value, gradient_host_ms=t.value)
+        return out
 
     def dts(self):
         d = np.zeros(self.nsteps)

@@ -319,6 +319,11 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
                      double *dcl_dbumps);
 /* Wall-clock times (ms) of the last record, gradient and tangent calls. */
 int dg_sens_timing(const dg_sens *s, double *forward_ms, double *gradient_ms, double *tangent_ms);
+/* Parts of the last gradient call (ms): the checkpointed forward run, the
+ * reverse sweep (segment recomputation included) and the host work (C_L
+ * cotangent, free-stream and geometry chains). */
+int dg_sens_gradient_breakdown(const dg_sens *s, double *forward_ms, double *reverse_ms,
+                               double *host_ms);
 /* The recorded dt sequence (n must equal the recorded step count). */
 int dg_sens_dts(const dg_sens *s, int n, double *dts);
 

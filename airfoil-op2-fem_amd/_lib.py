@@ -85,6 +85,7 @@ _SIGS = {
     "dg_sens_destroy": ([c_void_p], None),
     "dg_sens_record": ([c_void_p, c_void_p, c_int, c_double, P(c_double)], c_int),
     "dg_sens_eval": ([c_void_p, c_double, c_void_p, P(c_double)], c_int),
+    "dg_sens_set_alpha": ([c_void_p, c_double], c_int),
     "dg_sens_tangent": ([c_void_p, c_double, c_void_p, P(c_double), P(c_double)], c_int),
     "dg_sens_gradient": ([c_void_p, c_int, P(c_double), P(c_double), c_void_p], c_int),
     "dg_sens_timing": ([c_void_p, P(c_double), P(c_double), P(c_double)], c_int),

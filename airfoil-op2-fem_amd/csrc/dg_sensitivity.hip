@@ -443,6 +443,12 @@ int dg_sens_record(dg_sens *s, const double *bumps, int nsteps, double cfl, doub
   return DG_OK;
 }
 
+int dg_sens_set_alpha(dg_sens *s, double alpha_deg) {
+  if (!s || !std::isfinite(alpha_deg)) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_sens_set_alpha: bad argument");
+  s->alpha = alpha_deg;
+  return DG_OK;
+}
+
 int dg_sens_eval(dg_sens *s, double alpha_deg, const double *bumps, double *cl) {
   if (!s || !cl || s->nsteps < 1) return fail(s ? s->ctx : nullptr, DG_ERR_STATE, "dg_sens_eval: record first");
   const double a0 = s->alpha;

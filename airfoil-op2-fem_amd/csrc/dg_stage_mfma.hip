@@ -119,13 +119,20 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
     }
     __syncthreads();
 
-    // Euler fluxes per (cell, node), written in the A-operand layout
-    for (int it = lane; it < ne * Np; it += 64) {
-      const int e = it / Np, k = it - e * Np;
-      const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)(e0 + e) * Np + k) * 4);
-      const double2 q01 = src[0], q23 = src[1];
-      double qv[4] = {q01.x, q01.y, q23.x, q23.y}, F[4], Gv[4];
-      euler_flux(ph.gam, qv, F, Gv);
+    // Euler fluxes per (cell, node), written in the A-operand layout.  The
+    // padding columns k >= Np are written with zeros on every tile: the face
+    // phase of the previous tile left its values there, and a zero B entry
+    // does not cancel a non-finite one (a failed case must not reach the
+    // next tile, which may belong to another case).
+    for (int it = lane; it < ne * 4 * KP; it += 64) {
+      const int e = it / (4 * KP), k = it - e * (4 * KP);
+      double F[4] = {0.0, 0.0, 0.0, 0.0}, Gv[4] = {0.0, 0.0, 0.0, 0.0};
+      if (k < Np) {
+        const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)(e0 + e) * Np + k) * 4);
+        const double2 q01 = src[0], q23 = src[1];
+        double qv[4] = {q01.x, q01.y, q23.x, q23.y};
+        euler_flux(ph.gam, qv, F, Gv);
+      }
       // A row = cell + 4*var, A column = source node k
       const int off = ((e >> 2) * KP + (k >> 2)) * 64 + 16 * (k & 3) + (e & 3);
 #pragma unroll
@@ -199,6 +206,15 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
       const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
 #pragma unroll
       for (int v = 0; v < 4; v++) sL[off + 4 * v] = fl[v];
+    }
+    if (NF < 4 * KF) {  // zero padding columns (orders whose 3(N+1) is not a multiple of 4)
+      constexpr int PADF = 4 * KF - NF;
+      for (int it = lane; it < ne * PADF; it += 64) {
+        const int e = it / PADF, m = NF + (it - e * PADF);
+        const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
+#pragma unroll
+        for (int v = 0; v < 4; v++) sL[off + 4 * v] = 0.0;
+      }
     }
     __syncthreads();
 

@@ -428,6 +428,9 @@ class Sensitivity:
         self.nsteps = int(nsteps)
         return cl.value
 
+    def set_alpha(self, alpha):
+        self._chk(lib.dg_sens_set_alpha(self._h, float(alpha)))
+
     def eval(self, alpha, bumps=None):
         a, p = self._b(bumps)
         cl = c_double()

@@ -307,6 +307,9 @@ void dg_sens_destroy(dg_sens *s);
  * dt = cfl/dt1 and records the dt sequence, which every later evaluation of
  * this object replays (dt frozen).  Returns C_L. */
 int dg_sens_record(dg_sens *s, const double *bumps, int nsteps, double cfl, double *cl);
+/* Sets the angle of attack (degrees) of later record / tangent / gradient
+ * calls; the recorded dt sequence is kept until the next record. */
+int dg_sens_set_alpha(dg_sens *s, double alpha_deg);
 /* C_L with the recorded dt sequence at another angle / shape (for finite
  * differences); the recorded point is restored afterwards. */
 int dg_sens_eval(dg_sens *s, double alpha_deg, const double *bumps, double *cl);

@@ -92,6 +92,9 @@ _SIGS = {
     "dg_sens_dts": ([c_void_p, c_int, c_void_p], c_int),
     "dg_sens_gradient_breakdown": ([c_void_p, P(c_double), P(c_double), P(c_double)], c_int),
     "dg_solver_profile": ([c_void_p, c_int], c_int),
+    "dg_solver_set_launch": ([c_void_p, c_int, c_int, c_int], c_int),
+    "dg_solver_launch_config": ([c_void_p, P(c_int), P(c_int), P(c_int)], c_int),
+    "dg_solver_autotune": ([c_void_p, c_int, c_double, c_void_p, P(c_int)], c_int),
     "dg_solver_kernel_time": ([c_void_p, c_char_p, P(c_double), P(c_int)], c_int),
 }
 

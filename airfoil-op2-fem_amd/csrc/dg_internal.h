@@ -26,6 +26,8 @@ struct Launch {
   int cus;             // compute units of the device
   int max_blocks;      // optional cap on persistent grids (0: none)
   int variant;         // stage kernel: 0 MFMA (wave tiles), 1 LDS/VALU (workgroup tiles)
+  int wpb = 1;         // matrix-core kernel: waves (16-cell tiles) per workgroup, 1 / 2 / 4
+  int walk = 0;        // matrix-core kernel: 0 XCD-chunked block walk, 1 round-robin walk
 };
 
 // Arguments of the stage kernels.

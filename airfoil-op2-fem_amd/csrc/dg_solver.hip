@@ -62,6 +62,8 @@ struct dg_solver {
   // two-step graph
   hipGraphExec_t gexec = nullptr;
   double gcfl = -1.0;
+  // launch shape of the matrix-core stage kernel (autotuned or set)
+  int wpb = 1, walk = 0, max_blocks = 0;
 };
 
 namespace {
@@ -69,6 +71,9 @@ namespace {
 dg::Launch launch_s(const dg_solver *s) {
   dg::Launch L = dg::launch_of(s->ctx);
   L.stream = s->stream;
+  L.wpb = s->wpb;
+  L.walk = s->walk;
+  if (s->max_blocks > 0) L.max_blocks = s->max_blocks;
   return L;
 }
 
@@ -641,6 +646,99 @@ int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int 
   }
   if (avg_ms) *avg_ms = it->second.first / it->second.second;
   if (count) *count = it->second.second;
+  return DG_OK;
+}
+
+int dg_solver_set_launch(dg_solver *s, int wpb, int walk, int max_blocks) {
+  if (!s || (wpb != 1 && wpb != 2 && wpb != 4) || (walk != 0 && walk != 1) || max_blocks < 0)
+    return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_set_launch: wpb in {1,2,4}, walk in {0,1}, max_blocks >= 0");
+  drop_graph(s);
+  s->wpb = wpb; s->walk = walk; s->max_blocks = max_blocks;
+  return DG_OK;
+}
+
+int dg_solver_launch_config(const dg_solver *s, int *wpb, int *walk, int *max_blocks) {
+  if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_launch_config: null solver");
+  if (wpb) *wpb = s->wpb;
+  if (walk) *walk = s->walk;
+  if (max_blocks) *max_blocks = s->max_blocks;
+  return DG_OK;
+}
+
+// Times trial_steps fused steps for every launch shape (waves per workgroup
+// 1/2/4 x XCD-chunked or round-robin walk x 4/8/12 resident waves per CU)
+// on the solver's own cases, restores the state, time and dt it started
+// from, and keeps the fastest shape.  Every shape runs the same per-cell
+// arithmetic, so the results do not depend on the choice.
+int dg_solver_autotune(dg_solver *s, int trial_steps, double cfl, double *table, int *ntable) {
+  if (!s || trial_steps < 1 || !(cfl > 0.0)) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_autotune: bad argument");
+  dg_ctx *c = s->ctx;
+  if (c->variant != 0) return fail(c, DG_ERR_ARG, "dg_solver_autotune: tunes the matrix-core stage kernel (variant 0)");
+  const size_t nstate = (size_t)s->K * c->Np * 4;
+  if (!s->dt_valid) {
+    HIPCHECK(c, compute_dt1(s, (int)(s->step & 1)));
+    s->dt_valid = true;
+  }
+  drop_graph(s);
+  double *q0 = nullptr, *t0 = nullptr;
+  unsigned long long *d0 = nullptr;
+  HIPCHECK(c, hipMalloc(&q0, sizeof(double) * (nstate ? nstate : 1)));
+  hipError_t e = hipMalloc(&t0, sizeof(double) * s->ncases);
+  if (e == hipSuccess) e = hipMalloc(&d0, 2 * sizeof(unsigned long long) * s->ncases);
+  if (e == hipSuccess) e = hipMemcpyAsync(q0, s->q, sizeof(double) * nstate, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(t0, s->t_dev, sizeof(double) * s->ncases, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d0, s->dt1, 2 * sizeof(unsigned long long) * s->ncases, hipMemcpyDeviceToDevice, s->stream);
+  const long long step0 = s->step;
+  const int w0 = s->wpb, k0 = s->walk, b0 = s->max_blocks;
+  const int wpbs[3] = {1, 2, 4}, waves[3] = {4, 8, 12};
+  double best = 1e300;
+  int bw = w0, bk = k0, bb = b0, n = 0;
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&ea);
+  if (e == hipSuccess) e = hipEventCreate(&eb);
+  const int cus = c->cus > 0 ? c->cus : 256;
+  for (int iw = 0; iw < 3 && e == hipSuccess; iw++)
+    for (int walk = 0; walk < 2 && e == hipSuccess; walk++)
+      for (int iv = 0; iv < 3 && e == hipSuccess; iv++) {
+        s->wpb = wpbs[iw];
+        s->walk = walk;
+        s->max_blocks = cus * waves[iv] / wpbs[iw];
+        e = fused_step(s, (int)(s->step & 1), cfl);  // warm
+        s->step++;
+        if (e == hipSuccess) e = hipEventRecord(ea, s->stream);
+        for (int k = 0; k < trial_steps && e == hipSuccess; k++) {
+          e = fused_step(s, (int)(s->step & 1), cfl);
+          s->step++;
+        }
+        if (e == hipSuccess) e = hipEventRecord(eb, s->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(eb);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
+        const double per = ms / trial_steps;
+        if (table) {
+          table[4 * n] = s->wpb; table[4 * n + 1] = walk; table[4 * n + 2] = waves[iv]; table[4 * n + 3] = per;
+        }
+        n++;
+        if (e == hipSuccess && per < best) { best = per; bw = s->wpb; bk = walk; bb = s->max_blocks; }
+      }
+  // restore the starting state, time, dt slots and step parity
+  if (e == hipSuccess) e = hipMemcpyAsync(s->q, q0, sizeof(double) * nstate, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->w, q0, sizeof(double) * nstate, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->t_dev, t0, sizeof(double) * s->ncases, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->dt1, d0, 2 * sizeof(unsigned long long) * s->ncases, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  s->step = step0;
+  if (ea) (void)hipEventDestroy(ea);
+  if (eb) (void)hipEventDestroy(eb);
+  (void)hipFree(q0); (void)hipFree(t0); (void)hipFree(d0);
+  if (e != hipSuccess) {
+    s->wpb = w0; s->walk = k0; s->max_blocks = b0;
+    return hipfail(c, e, "dg_solver_autotune");
+  }
+  s->wpb = bw; s->walk = bk; s->max_blocks = bb;
+  if (ntable) *ntable = n;
   return DG_OK;
 }
 

@@ -36,24 +36,35 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+template <int N>
+constexpr int mfma_lds_doubles() {
+  return 2 * 4 * ((Ord<N>::Np + 3) / 4) * 64;  // per wave: F and G tiles [4][KP][64]
+}
+
+// A workgroup is 1, 2 or 4 waves (16, 32 or 64 cells per workgroup visit);
+// each wave owns one 16-cell tile of the workgroup's block of tiles and its
+// own LDS tiles.  The walk over blocks of tiles is uniform across the
+// workgroup, so every wave reaches every barrier; waves whose tile lies past
+// the end only skip the work.
 template <int N, bool GATHER, int EPI>
-__global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const double *__restrict__ ops,
-                                                      const int *__restrict__ fmask, Phys ph0,
-                                                      unsigned long long facenodes) {
+__global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const double *__restrict__ ops,
+                                                       const int *__restrict__ fmask, Phys ph0,
+                                                       unsigned long long facenodes, int walk) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   constexpr int KP = (Np + 3) / 4;  // K-steps over source nodes
   constexpr int KF = (NF + 3) / 4;  // K-steps over face nodes
   constexpr int TE = kMfmaTile, G = 4;
   static_assert(Np <= 16, "one 16-row output tile per cell group");
   static_assert(KF <= 2 * KP, "face tile must fit in the volume tiles it reuses");
-  // [0]: F tiles, [1]: G tiles, each [G][KP][64]; the face-flux tiles
-  // [G][KF][64] reuse the same memory after the volume products.
-  __shared__ double sA[2 * G * KP * 64];
+  // per wave: [0]: F tiles, [1]: G tiles, each [G][KP][64]; the face-flux
+  // tiles [G][KF][64] reuse the same memory after the volume products.
+  extern __shared__ double lds_dyn[];
   __shared__ int sFm[NF];
 
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  double *sA = lds_dyn + wave * mfma_lds_doubles<N>();
   for (int i = lane; i < 2 * G * KP * 64; i += 64) sA[i] = 0.0;
-  if (lane < NF) sFm[lane] = fmask[lane];
+  if (threadIdx.x < NF) sFm[threadIdx.x] = fmask[threadIdx.x];
 
   // B operands: lane l holds B[k = 4*ks + l/16][j = l%16]
   const int jj = lane & 15, kk = lane >> 4;
@@ -73,9 +84,9 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
   }
 
   const bool multi = a.ncases > 0;
-  if (EPI == EPI_STAGE0 && blockIdx.x == 0)
+  if (EPI == EPI_STAGE0 && blockIdx.x == 0 && wave == 0)
     for (int c = lane; c < a.ncases; c += 64) a.dt1_next[c] = 0ull;
-  if (EPI == EPI_STAGE2 && blockIdx.x == 0)
+  if (EPI == EPI_STAGE2 && blockIdx.x == 0 && wave == 0)
     for (int c = lane; c < a.ncases; c += 64)
       a.t_dev[c] += a.dt_fixed ? a.dt_fixed[c] : a.cfl / __longlong_as_double((long long)a.dt1_cur[c]);
   unsigned long long cfl_bits = 0ull;
@@ -85,13 +96,18 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
   double *sF = sA, *sG = sA + G * KP * 64, *sL = sA;
 
   const int ntiles = multi ? a.tile_off[a.ncases] : (a.ncells + TE - 1) / TE;
+  const int nblk = (ntiles + wpb - 1) / wpb;  // blocks of wpb tiles
   for (int it_tile = 0;; it_tile++) {
-    const int tile = xcd_tile(it_tile, ntiles);
-    if (tile >= ntiles) break;
-    int e0, ne, cs = 0;
+    const int blk = walk == 0 ? xcd_tile(it_tile, nblk) : (int)blockIdx.x + it_tile * (int)gridDim.x;
+    if (blk >= nblk) break;  // uniform across the workgroup
+    const int tile = blk * wpb + wave;
+    const bool active = tile < ntiles;
+    int e0 = 0, ne = 0, cs = 0;
     Phys ph = ph0;
     double dt = 0.0;
-    if (multi) {
+    if (!active) {
+      // past the last tile: no work, but every barrier below is reached
+    } else if (multi) {
       int lo = 0, hi = a.ncases - 1;  // case owning this tile
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -107,7 +123,7 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
       e0 = tile * TE;
       ne = min(TE, a.ncells - e0);
     }
-    if (EPI == EPI_STAGE2 && cs != cfl_case) {
+    if (EPI == EPI_STAGE2 && active && cs != cfl_case) {
       // flush the running CFL max of the previous case (wave-uniform branch)
       if (cfl_case >= 0) {
         unsigned long long r = cfl_bits;
@@ -275,25 +291,36 @@ __global__ __launch_bounds__(64, 3) void rhs_mfma_kernel(StageArgs a, const doub
   }
 }
 
+// Waves per workgroup actually used for order N: 1, 2 or 4, within 48 KB of
+// LDS per workgroup.
+template <int N>
+static int mfma_wpb(int want) {
+  int w = want >= 4 ? 4 : (want >= 2 ? 2 : 1);
+  while (w > 1 && (size_t)w * mfma_lds_doubles<N>() * sizeof(double) > 48 * 1024) w /= 2;
+  return w;
+}
+
 template <int N, bool GATHER, int EPI>
 static hipError_t launch_t(const Launch &L, const StageArgs &a) {
-  static int per_cu = -1;
-  if (per_cu < 0) {
+  static int per_cu[5] = {-1, -1, -1, -1, -1};  // resident workgroups per CU, by waves per workgroup
+  const int wpb = mfma_wpb<N>(L.wpb);
+  const size_t lds = (size_t)wpb * mfma_lds_doubles<N>() * sizeof(double);
+  if (per_cu[wpb] < 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rhs_mfma_kernel<N, GATHER, EPI>, 64, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rhs_mfma_kernel<N, GATHER, EPI>, 64 * wpb, lds) !=
             hipSuccess ||
         nb < 1)
-      nb = 8;
-    per_cu = nb;
+      nb = 8 / wpb > 0 ? 8 / wpb : 1;
+    per_cu[wpb] = nb;
   }
-  const int ntiles = a.ncases > 0 ? -1 : (a.ncells + 15) / 16;
-  int grid = L.cus > 0 ? L.cus * per_cu : 2048;
+  const int nblk = a.ncases > 0 ? -1 : ((a.ncells + 15) / 16 + wpb - 1) / wpb;
+  int grid = L.cus > 0 ? L.cus * per_cu[wpb] : 2048 / wpb;
   if (L.max_blocks > 0 && grid > L.max_blocks) grid = L.max_blocks;
-  if (ntiles >= 0 && grid > ntiles) grid = ntiles;
+  if (nblk >= 0 && grid > nblk) grid = nblk;
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) return hipSuccess;
-  hipLaunchKernelGGL((rhs_mfma_kernel<N, GATHER, EPI>), dim3(grid), dim3(64), 0, L.stream, a, L.ops,
-                     L.fmask, L.ph, (unsigned long long)L.facenodes);
+  hipLaunchKernelGGL((rhs_mfma_kernel<N, GATHER, EPI>), dim3(grid), dim3(64 * wpb), lds, L.stream, a, L.ops,
+                     L.fmask, L.ph, (unsigned long long)L.facenodes, L.walk);
   return hipGetLastError();
 }
 

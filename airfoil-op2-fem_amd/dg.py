@@ -7,6 +7,7 @@ replaces the CGNS loader (src/load_mesh.cpp) with generated or VTK meshes.
 ``Solver`` is the native SSPRK3 driver of src/airfoil.cpp:204-334.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -154,6 +155,16 @@ class Context:
     def set_variant(self, variant, max_blocks=0):
         """Stage kernel: 0 matrix-core (default), 1 vector-ALU."""
         self._chk(lib.dg_set_variant(self._h, int(variant), int(max_blocks)))
+        self._variant = int(variant)
+
+    @property
+    def variant(self):
+        """Stage kernel in use (the DG_STAGE_VARIANT environment variable
+        selects it at context creation)."""
+        v = getattr(self, "_variant", None)
+        if v is None:
+            v = 1 if os.environ.get("DG_STAGE_VARIANT") == "1" else 0
+        return v
 
     def _dim(self, name):
         if name in self._DBL1:
@@ -385,6 +396,25 @@ class Solver:
 
     def profile(self, enable=True):
         self._chk(lib.dg_solver_profile(self._h, 1 if enable else 0))
+
+    def set_launch(self, wpb=1, walk=0, max_blocks=0):
+        """Launch shape of the matrix-core stage kernel (results unchanged)."""
+        self._chk(lib.dg_solver_set_launch(self._h, int(wpb), int(walk), int(max_blocks)))
+
+    def launch_config(self):
+        w, k, b = c_int(), c_int(), c_int()
+        self._chk(lib.dg_solver_launch_config(self._h, ctypes.byref(w), ctypes.byref(k), ctypes.byref(b)))
+        return dict(wpb=w.value, walk=k.value, max_blocks=b.value)
+
+    def autotune(self, trial_steps=3, cfl=1.0):
+        """Times every launch shape on this solver's cases (state restored
+        afterwards), keeps the fastest; returns the table of
+        (wpb, walk, waves_per_cu, ms_per_step) rows."""
+        tab = np.zeros((32, 4))
+        n = c_int()
+        self._chk(lib.dg_solver_autotune(self._h, int(trial_steps), float(cfl), _np_ptr(tab), ctypes.byref(n)))
+        return [dict(wpb=int(r[0]), walk=int(r[1]), waves_per_cu=int(r[2]), ms_per_step=float(r[3]))
+                for r in tab[:n.value]]
 
     def kernel_time(self, name):
         ms, n = c_double(), c_int()

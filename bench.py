@@ -188,6 +188,9 @@ def batch_bench(args):
     batch.run(2, cfl=args.cfl, graph=True)  # warm (graph capture), then restart from the free stream
     del batch
     batch = dg.BatchSolver(ctx, meshes, [s[2] for s in specs], [s[3] for s in specs])
+    batch.autotune(trial_steps=3, cfl=args.cfl)
+    launch_batch = batch.launch_config()
+    print(f"[bench] autotuned batch launch: {launch_batch}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch.run(args.batch_steps, cfl=args.cfl, graph=True)
@@ -205,6 +208,7 @@ def batch_bench(args):
             s = dg.Solver(c1, m)
             s.run(1, cfl=args.cfl, graph=graph)
             s = dg.Solver(c1, m)
+            s.autotune(trial_steps=2, cfl=args.cfl)
             torch.cuda.synchronize()
             a = time.perf_counter()
             s.run(args.batch_steps, cfl=args.cfl, graph=graph)
@@ -227,6 +231,7 @@ def batch_bench(args):
                                              f"{args.batch_steps} SSPRK3 steps, CFL {args.cfl}, one persistent "
                                              "launch per stage, hipGraph-captured steps"},
                       "batched_s": t_batch, "sequential_s": seq, "nonfinite_cases": bad,
+                      "batch_launch": launch_batch,
                       "batched_dof_updates_per_s_per_stage": dof_updates / t_batch,
                       "sequential_dof_updates_per_s_per_stage": dof_updates / t_seq,
                       "bit_identical_all_cases": identical}))
@@ -335,6 +340,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run plain device copies of GIB GiB (counter calibration)")
     args = ap.parse_args()
@@ -374,6 +380,12 @@ def main():
     print(f"[bench] mesh {n}x{n}: {K} cells, {dof} DoF, set-up {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
 
+    tune = None
+    if not args.no_autotune and ctx.variant == 0:
+        table = solver.autotune(trial_steps=3, cfl=args.cfl)
+        tune = dict(solver.launch_config(), table_ms_per_step={
+            f"wpb{r['wpb']}_walk{r['walk']}_w{r['waves_per_cu']}": round(r["ms_per_step"], 4) for r in table})
+        print(f"[bench] autotuned launch: {solver.launch_config()}", file=sys.stderr, flush=True)
     solver.run(args.warmup, cfl=args.cfl)
     torch.cuda.synchronize()
     q_start = solver.get_q() if not args.no_cpu_baseline else None
@@ -440,7 +452,7 @@ def main():
            "config": {"workload": f"isentropic vortex, {n}x{n} jittered periodic grid "
                                   f"({K} triangles), N={N}, SSPRK3 CFL {args.cfl}, fused stage kernels",
                       "cells": K, "order": N, "dof": dof, "parallelism": "single GPU"},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, "launch": tune}
     print(json.dumps(out))
 
 

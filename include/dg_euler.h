@@ -266,6 +266,20 @@ int dg_solver_wall_cp(dg_solver *s, int *nwall, double *x, double *cp);
 int dg_solver_profile(dg_solver *s, int enable);
 int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int *count);
 
+/* Launch shape of the matrix-core stage kernel (no reference counterpart):
+ * wpb = waves (16-cell tiles) per workgroup, 1, 2 or 4 (16 / 32 / 64
+ * elements per workgroup); walk = 0 XCD-chunked (the workgroups of one XCD
+ * walk one contiguous range of tiles), 1 round-robin; max_blocks caps the
+ * persistent grid (0: occupancy-limited).  The shape never changes results. */
+int dg_solver_set_launch(dg_solver *s, int wpb, int walk, int max_blocks);
+int dg_solver_launch_config(const dg_solver *s, int *wpb, int *walk, int *max_blocks);
+/* Autotuner: times trial_steps fused steps (CFL cfl) of the solver's own
+ * cases for each of 18 shapes (wpb 1/2/4 x walk 0/1 x 4/8/12 resident waves
+ * per CU), restores the state, time and dt it started from and keeps the
+ * fastest.  table (may be NULL) receives [18][4] rows (wpb, walk,
+ * waves_per_cu, ms_per_step); *ntable the row count. */
+int dg_solver_autotune(dg_solver *s, int trial_steps, double cfl, double *table, int *ntable);
+
 /* ------------------------------------------------------------------ */
 /* Tangent-linear and adjoint RHS (no reference counterpart)          */
 /* ------------------------------------------------------------------ */

@@ -62,6 +62,61 @@ def test_failed_case_does_not_reach_other_cases(dg):
         assert np.array_equal(batch.case_q(k), s.get_q()), k
 
 
+def _vortex(dg, n=24):
+    mesh = dg.periodic_grid(n, n, 0.0, -5.0, 10.0, 10.0, seed=0, jitter=0.1)
+    ctx = dg.Context(3)
+    ctx.set_freestream(GAM, 0.5, 0.0, 1.0, 1.0)
+    s = dg.Solver(ctx, mesh)
+    s.set_vortex(0.5 * math.sqrt(GAM), 5.0, 5.0, 0.0, 10.0)
+    return s
+
+
+def test_launch_shapes_and_autotune_keep_results(dg):
+    """Every launch shape of the matrix-core stage kernel (waves per
+    workgroup, walk order, grid cap) gives the same bits, and the autotuner
+    restores the state it tried its shapes from."""
+    import os
+    if os.environ.get("DG_STAGE_VARIANT") == "1":
+        pytest.skip("launch shapes belong to the matrix-core stage kernel")
+    ref = _vortex(dg)
+    ref.run(5, cfl=0.5)
+    q_ref, t_ref = ref.get_q(), ref.time()
+    for wpb, walk, cap in ((1, 1, 0), (2, 0, 0), (2, 1, 40), (4, 0, 0), (4, 1, 8), (1, 0, 3)):
+        s = _vortex(dg)
+        s.set_launch(wpb, walk, cap)
+        s.run(5, cfl=0.5, graph=(wpb == 4))
+        assert np.array_equal(s.get_q(), q_ref), (wpb, walk, cap)
+        assert s.time() == t_ref
+    s = _vortex(dg)
+    q0 = s.get_q()
+    tab = s.autotune(trial_steps=2, cfl=0.5)
+    assert len(tab) == 18 and all(r["ms_per_step"] > 0 for r in tab)
+    assert np.array_equal(s.get_q(), q0) and s.time()[0] == 0.0
+    best = min(tab, key=lambda r: r["ms_per_step"])
+    cfg = s.launch_config()
+    assert (cfg["wpb"], cfg["walk"]) == (best["wpb"], best["walk"])
+    s.run(5, cfl=0.5)
+    assert np.array_equal(s.get_q(), q_ref)
+
+
+def test_batch_launch_shapes_bit_identical(dg):
+    import os
+    if os.environ.get("DG_STAGE_VARIANT") == "1":
+        pytest.skip("batches run the matrix-core stage kernel only")
+    specs = [(24, 6, 0.35, -2.0), (40, 10, 0.5, 4.0), (32, 8, 0.45, 7.5)]
+    meshes = [dg.naca0012_ogrid(nc, nr, 1.15, 15.0) for nc, nr, _, _ in specs]
+    out = []
+    for wpb, walk in ((1, 0), (4, 1), (2, 0)):
+        ctx = dg.Context(3)
+        ctx.set_freestream(GAM, 0.4, 0.0, 1.0, 1.0)
+        b = dg.BatchSolver(ctx, meshes, [s[2] for s in specs], [s[3] for s in specs])
+        b.set_launch(wpb, walk, 0)
+        b.run(7, cfl=1.0)
+        out.append([b.case_q(k) for k in range(3)])
+    for k in range(3):
+        assert np.array_equal(out[0][k], out[1][k]) and np.array_equal(out[0][k], out[2][k]), k
+
+
 def test_graph_replay_matches_direct_launches(dg):
     def vortex():
         mesh = dg.periodic_grid(24, 24, 0.0, -5.0, 10.0, 10.0, seed=0, jitter=0.1)

@@ -96,6 +96,27 @@ def test_abi_exports_every_declared_symbol(dg):
     assert not missing, missing
 
 
+def test_op2_interface_exports_every_declared_function(dg):
+    """Every non-template function of include/op2_dg.h (C++ linkage) is
+    defined in the library, and the OP2 driver built against it links."""
+    import shutil
+    import subprocess
+    hdr = open(os.path.join(ROOT, "include", "op2_dg.h")).read()
+    names = set(re.findall(r"^(?:void|op_set|op_map|op_dat|op_arg|int)\s+(op_\w+)\s*\(", hdr, re.M))
+    templates = set(re.findall(r"template <class T>\s*\n\s*\w+\s+(op_\w+)\s*\(", hdr))
+    assert templates == {"op_decl_dat", "op_decl_const2", "op_arg_gbl", "op_fetch_data"}
+    names = sorted(names - templates)
+    assert "op_par_loop_euler_rhs" in names and len(names) >= 20
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", "--defined-only", "-C", dg.lib_path], capture_output=True, text=True).stdout
+    defined = set(re.findall(r"\s(op_\w+)\(", out))
+    missing = [n for n in names if n not in defined]
+    assert not missing, missing
+    assert os.path.exists(os.path.join(ROOT, "tests", "op2", "airfoil_op2_driver"))
+
+
 def test_bad_order_rejected_without_device(dg):
     with pytest.raises(dg.DGError):
         dg.Context(order=0)

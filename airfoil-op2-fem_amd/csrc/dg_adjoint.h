@@ -40,6 +40,8 @@ struct AdjArgs {
   double *g_rx, *g_ry, *g_sx, *g_sy;    // += scale (dL/dgeo)^T mu  [K][Np]
   double *g_nx, *g_ny, *g_fs;           // [K][NF]
   double *g_bcu, *g_bcv;                // [K][NF] (boundary slots only)
+  int nbedges;                          // boundary edges (cell, local face)
+  const int *b2c, *bnum;
 };
 
 // Ld = L'(w) wd + dL/dgeo geod + dL/dbc (bcud, bcvd)
@@ -68,6 +70,11 @@ hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_xy, co
 constexpr int kSumBlocks = 256;
 hipError_t launch_strided_sum(const Launch &L, size_t n, int stride, int offset, const double *x,
                               double *partial);
+// Wall-face data of the lift functional, compact: [nwall][Nfp][7] (state at
+// the face node, nx, ny, fscale) then [nwall] J.
+hipError_t launch_wall_gather(const Launch &L, int nwall, const int *wcell, const int *wface, const double *q,
+                              const double *nx, const double *ny, const double *fs, const double *J,
+                              double *out);
 // dst[idx[i]] += val[i] (indices distinct)
 hipError_t launch_add_indexed(const Launch &L, int n, const int *idx, const double *val, double *dst);
 

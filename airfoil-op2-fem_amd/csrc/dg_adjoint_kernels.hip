@@ -130,7 +130,7 @@ __device__ __forceinline__ void boundary_t(const Phys &ph, int type, T bcu, T bc
 __device__ __forceinline__ void euler_flux_vjp(double gam, const double q[4], const double bF[4],
                                                const double bG[4], double bq[4]) {
   const double g1 = gam - 1.0;
-  const double rho = q[0], u = q[1] / rho, v = q[2] / rho;
+  const double rho = q[0], ir = rcp_nr(rho), u = q[1] * ir, v = q[2] * ir;
   const double p = g1 * (q[3] - 0.5 * (q[1] * u + q[2] * v));
   double bu = bF[1] * q[1] + bF[2] * q[2] + bF[3] * (q[3] + p);
   double bv = bG[1] * q[1] + bG[2] * q[2] + bG[3] * (q[3] + p);
@@ -145,9 +145,9 @@ __device__ __forceinline__ void euler_flux_vjp(double gam, const double q[4], co
   b2 += -0.5 * g1 * v * bp;
   bv += -0.5 * g1 * q[2] * bp;
   // u = q1/rho, v = q2/rho
-  b1 += bu / rho;
-  b2 += bv / rho;
-  const double b0 = -(bu * u + bv * v) / rho;
+  b1 += bu * ir;
+  b2 += bv * ir;
+  const double b0 = -(bu * u + bv * v) * ir;
   bq[0] += b0; bq[1] += b1; bq[2] += b2; bq[3] += b3;
 }
 
@@ -160,24 +160,29 @@ __device__ void roe_vjp(double gam, const double mQ[4], const double eQ[4], doub
   // forward sweep
   const double mR0 = mQ[0], mR1 = nx * mQ[1] + ny * mQ[2], mR2 = -ny * mQ[1] + nx * mQ[2], mR3 = mQ[3];
   const double pR0 = eQ[0], pR1 = nx * eQ[1] + ny * eQ[2], pR2 = -ny * eQ[1] + nx * eQ[2], pR3 = eQ[3];
-  const double mU = mR1 / mR0, mV = mR2 / mR0;
+  // reciprocals refined from the hardware estimates (dg_device.h), shared
+  // by the forward recomputation and every quotient of the reverse sweep
+  const double imR0 = rcp_nr(mR0), ipR0 = rcp_nr(pR0);
+  const double mU = mR1 * imR0, mV = mR2 * imR0;
   const double mP = g1 * (mR3 - 0.5 * (mR1 * mU + mR2 * mV));
-  const double pU = pR1 / pR0, pV = pR2 / pR0;
+  const double pU = pR1 * ipR0, pV = pR2 * ipR0;
   const double pP = g1 * (pR3 - 0.5 * (pR1 * pU + pR2 * pV));
   const double fm1 = mR1 * mU + mP, fm2 = mR2 * mU, fm3 = mU * (mR3 + mP);
   const double fp1 = pR1 * pU + pP, fp2 = pR2 * pU, fp3 = pU * (pR3 + pP);
-  const double mH = (mR3 + mP) / mR0, pH = (pR3 + pP) / pR0;
-  const double ms = sqrt(mR0), ps = sqrt(pR0);
-  const double rho = ms * ps, den = ms + ps;
-  const double u = (ms * mU + ps * pU) / den, v = (ms * mV + ps * pV) / den;
-  const double H = (ms * mH + ps * pH) / den;
+  const double mH = (mR3 + mP) * imR0, pH = (pR3 + pP) * ipR0;
+  const double ms = sqrt_nr(mR0), ps = sqrt_nr(pR0);
+  const double ims = rcp_nr(ms), ips = rcp_nr(ps);
+  const double rho = ms * ps, den = ms + ps, iden = rcp_nr(den);
+  const double u = (ms * mU + ps * pU) * iden, v = (ms * mV + ps * pV) * iden;
+  const double H = (ms * mH + ps * pH) * iden;
   const double c2 = g1 * (H - 0.5 * (u * u + v * v));
-  const double c = sqrt(c2);
+  const double c = sqrt_nr(c2);
+  const double ic = rcp_nr(c), ic2 = ic * ic, ic4 = ic2 * ic2;
   const double dU = pU - mU, dP = pP - mP, dV = pV - mV, dR = pR0 - mR0;
-  const double W1 = -0.5 * rho * dU / c + 0.5 * dP / c2;
-  const double W2 = dR - dP / c2;
+  const double W1 = -0.5 * rho * dU * ic + 0.5 * dP * ic2;
+  const double W2 = dR - dP * ic2;
   const double W3 = rho * dV;
-  const double W4 = 0.5 * rho * dU / c + 0.5 * dP / c2;
+  const double W4 = 0.5 * rho * dU * ic + 0.5 * dP * ic2;
   const double a1 = fabs(u - c), a2 = fabs(u), a4 = fabs(u + c);
   const double X1 = a1 * W1, X2 = a2 * W2, X3 = a2 * W3, X4 = a4 * W4;
   const double f0 = (mR1 + pR1) / 2.0 - (X1 + X2 + X4) / 2.0;
@@ -216,28 +221,29 @@ __device__ void roe_vjp(double gam, const double mQ[4], const double eQ[4], doub
   bc += -ba1 * s1 + ba4 * s4;
   // W terms
   double brho = 0, bdU = 0, bdP = 0, bdV = 0, bdR = 0, bc2 = 0;
-  brho += bW1 * (-0.5 * dU / c); bdU += bW1 * (-0.5 * rho / c);
-  bc += bW1 * (0.5 * rho * dU / (c * c)); bdP += bW1 * (0.5 / c2); bc2 += bW1 * (-0.5 * dP / (c2 * c2));
-  bdR += bW2; bdP += -bW2 / c2; bc2 += bW2 * dP / (c2 * c2);
+  brho += bW1 * (-0.5 * dU * ic); bdU += bW1 * (-0.5 * rho * ic);
+  bc += bW1 * (0.5 * rho * dU * ic2); bdP += bW1 * (0.5 * ic2); bc2 += bW1 * (-0.5 * dP * ic4);
+  bdR += bW2; bdP += -bW2 * ic2; bc2 += bW2 * dP * ic4;
   brho += bW3 * dV; bdV += bW3 * rho;
-  brho += bW4 * (0.5 * dU / c); bdU += bW4 * (0.5 * rho / c);
-  bc += bW4 * (-0.5 * rho * dU / (c * c)); bdP += bW4 * (0.5 / c2); bc2 += bW4 * (-0.5 * dP / (c2 * c2));
+  brho += bW4 * (0.5 * dU * ic); bdU += bW4 * (0.5 * rho * ic);
+  bc += bW4 * (-0.5 * rho * dU * ic2); bdP += bW4 * (0.5 * ic2); bc2 += bW4 * (-0.5 * dP * ic4);
   double bpU = bdU, bmU = -bdU, bpP = bdP, bmP = -bdP, bpV = bdV, bmV = -bdV;
   bpR0 += bdR; bmR0 -= bdR;
   // c = sqrt(c2); c2 = g1 (H - (u^2+v^2)/2)
-  bc2 += bc / (2.0 * c);
+  bc2 += bc * 0.5 * ic;
   bH += g1 * bc2; bu += -g1 * u * bc2; bv += -g1 * v * bc2;
   // Roe averages
-  double bms = bu * (mU - u) / den + bv * (mV - v) / den + bH * (mH - H) / den;
-  double bps = bu * (pU - u) / den + bv * (pV - v) / den + bH * (pH - H) / den;
-  bmU += bu * ms / den; bpU += bu * ps / den;
-  bmV += bv * ms / den; bpV += bv * ps / den;
-  double bmH = bH * ms / den, bpH = bH * ps / den;
+  double bms = (bu * (mU - u) + bv * (mV - v) + bH * (mH - H)) * iden;
+  double bps = (bu * (pU - u) + bv * (pV - v) + bH * (pH - H)) * iden;
+  const double wm = ms * iden, wp = ps * iden;
+  bmU += bu * wm; bpU += bu * wp;
+  bmV += bv * wm; bpV += bv * wp;
+  double bmH = bH * wm, bpH = bH * wp;
   bms += brho * ps; bps += brho * ms;
-  bmR0 += bms / (2.0 * ms); bpR0 += bps / (2.0 * ps);
+  bmR0 += bms * 0.5 * ims; bpR0 += bps * 0.5 * ips;
   // mH = (mR3 + mP)/mR0
-  bmR3 += bmH / mR0; bmP += bmH / mR0; bmR0 += -bmH * mH / mR0;
-  bpR3 += bpH / pR0; bpP += bpH / pR0; bpR0 += -bpH * pH / pR0;
+  bmR3 += bmH * imR0; bmP += bmH * imR0; bmR0 += -bmH * mH * imR0;
+  bpR3 += bpH * ipR0; bpP += bpH * ipR0; bpR0 += -bpH * pH * ipR0;
   // fluxes in the rotated frame
   bmR1 += bfm1 * mU; bmU += bfm1 * mR1 + bfm2 * mR2 + bfm3 * (mR3 + mP); bmP += bfm1 + bfm3 * mU;
   bmR2 += bfm2 * mU; bmR3 += bfm3 * mU;
@@ -249,8 +255,8 @@ __device__ void roe_vjp(double gam, const double mQ[4], const double eQ[4], doub
   bpR3 += g1 * bpP; bpR1 += -0.5 * g1 * pU * bpP; bpU += -0.5 * g1 * pR1 * bpP;
   bpR2 += -0.5 * g1 * pV * bpP; bpV += -0.5 * g1 * pR2 * bpP;
   // velocities
-  bmR1 += bmU / mR0; bmR2 += bmV / mR0; bmR0 += -(bmU * mU + bmV * mV) / mR0;
-  bpR1 += bpU / pR0; bpR2 += bpV / pR0; bpR0 += -(bpU * pU + bpV * pV) / pR0;
+  bmR1 += bmU * imR0; bmR2 += bmV * imR0; bmR0 += -(bmU * mU + bmV * mV) * imR0;
+  bpR1 += bpU * ipR0; bpR2 += bpV * ipR0; bpR0 += -(bpU * pU + bpV * pV) * ipR0;
   // rotations
   bmQ[0] += bmR0; bmQ[3] += bmR3;
   bmQ[1] += nx * bmR1 - ny * bmR2; bmQ[2] += ny * bmR1 + nx * bmR2;
@@ -258,30 +264,6 @@ __device__ void roe_vjp(double gam, const double mQ[4], const double eQ[4], doub
   beQ[1] += nx * bpR1 - ny * bpR2; beQ[2] += ny * bpR1 + nx * bpR2;
   bnx += bmR1 * mQ[1] + bmR2 * mQ[2] + bpR1 * eQ[1] + bpR2 * eQ[2];
   bny += bmR1 * mQ[2] - bmR2 * mQ[1] + bpR1 * eQ[2] - bpR2 * eQ[1];
-}
-
-// Reverse of boundary_t through its dual Jacobian (8 directions: the four
-// interior variables, the normal, the free-stream velocity).
-// Out of line: only boundary slots take it, and inlined it would raise the
-// register count of the whole face pass.
-__device__ __noinline__ void boundary_vjp(const Phys &ph, int type, const double mQ[4], double nx, double ny,
-                                          const double bqb[4], double bmQ[4], double &bnx, double &bny,
-                                          double &bbcu, double &bbcv) {
-#pragma unroll 1
-  for (int dir = 0; dir < 8; dir++) {
-    Dual q[4], qb[4];
-    for (int k = 0; k < 4; k++) q[k] = Dual(mQ[k], dir == k ? 1.0 : 0.0);
-    Dual dnx(nx, dir == 4 ? 1.0 : 0.0), dny(ny, dir == 5 ? 1.0 : 0.0);
-    Dual du(ph.bc_u, dir == 6 ? 1.0 : 0.0), dv(ph.bc_v, dir == 7 ? 1.0 : 0.0);
-    boundary_t<Dual>(ph, type, du, dv, q, dnx, dny, qb);
-    double s = 0.0;
-    for (int k = 0; k < 4; k++) s += bqb[k] * qb[k].d;
-    if (dir < 4) bmQ[dir] += s;
-    else if (dir == 4) bnx += s;
-    else if (dir == 5) bny += s;
-    else if (dir == 6) bbcu += s;
-    else bbcv += s;
-  }
 }
 
 // ------------------------------------------------------------------------
@@ -386,20 +368,13 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_face_kernel(AdjArgs a, Phy
       const int nn = a.fmask[(code & 3) * Nfp + (Nfp - 1 - i)];
       for (int v = 0; v < 4; v++) pQ[v] = a.w[(nb * Np + nn) * 4 + v];
     } else {
-      Dual q[4], qb[4];
-      for (int v = 0; v < 4; v++) q[v] = Dual(mQ[v]);
-      boundary_t<Dual>(ph, -code - 1, Dual(ph.bc_u), Dual(ph.bc_v), q, Dual(nx), Dual(ny), qb);
-      for (int v = 0; v < 4; v++) pQ[v] = qb[v].v;
+      boundary_state(ph, -code - 1, mQ, nx, ny, pQ);  // the forward's exterior state
     }
     double bm[4] = {0, 0, 0, 0}, bp[4] = {0, 0, 0, 0}, bnx = 0, bny = 0, bfs = 0;
     roe_vjp(ph.gam, mQ, pQ, nx, ny, fs, psi, bm, bp, bnx, bny, bfs);
-    if (code < 0) {
-      double bbu = 0, bbv = 0;
-      boundary_vjp(ph, -code - 1, mQ, nx, ny, bp, bm, bnx, bny, bbu, bbv);
-      a.g_bcu[it] += a.scale * bbu;
-      a.g_bcv[it] += a.scale * bbv;
-      for (int v = 0; v < 4; v++) bp[v] = 0.0;
-    }
+    // On a boundary slot bp is the cotangent of the exterior (boundary)
+    // state; it stays in gpq (no neighbour reads a boundary slot's gpq) for
+    // adj_boundary_kernel, which reverses the boundary state into gmq.
     for (int v = 0; v < 4; v++) {
       a.gmq[it * 4 + v] = bm[v];
       a.gpq[it * 4 + v] = bp[v];
@@ -407,6 +382,40 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_face_kernel(AdjArgs a, Phy
     a.g_nx[it] += a.scale * bnx;
     a.g_ny[it] += a.scale * bny;
     a.g_fs[it] += a.scale * bfs;
+  }
+}
+
+// One thread per (boundary face node, input direction): reverse of the
+// boundary state through one column of its dual Jacobian (directions: the 4
+// interior variables, nx, ny, bc_u, bc_v), dotted with the exterior-state
+// cotangent the face pass left in gpq.  Each direction feeds a different
+// accumulator (the slot's interior cotangent, the normal, the free-stream
+// velocity), so no two threads write the same value.  Runs between the face
+// and cell passes.
+template <int N>
+__global__ __launch_bounds__(256) void adj_boundary_kernel(AdjArgs a, Phys ph, const int *__restrict__ fmask) {
+  constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
+  const int n = a.nbedges * Nfp * 8;
+  for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
+    const int dir = it & 7, bi = it >> 3;
+    const int b = bi / Nfp, i = bi - b * Nfp;
+    const size_t c = a.b2c[b];
+    const int f = a.bnum[b], m = f * Nfp + i;
+    const int code = a.fmap[c * 3 + f];
+    if (code >= 0) continue;  // not a boundary face (never for a valid mesh)
+    const size_t slot = c * NF + m;
+    Dual q[4], qb[4];
+    for (int v = 0; v < 4; v++) q[v] = Dual(a.w[(c * Np + fmask[m]) * 4 + v], dir == v ? 1.0 : 0.0);
+    const Dual dnx(a.nx[slot], dir == 4 ? 1.0 : 0.0), dny(a.ny[slot], dir == 5 ? 1.0 : 0.0);
+    const Dual du(ph.bc_u, dir == 6 ? 1.0 : 0.0), dv(ph.bc_v, dir == 7 ? 1.0 : 0.0);
+    boundary_t<Dual>(ph, -code - 1, du, dv, q, dnx, dny, qb);
+    double s = 0.0;
+    for (int v = 0; v < 4; v++) s += a.gpq[slot * 4 + v] * qb[v].d;
+    if (dir < 4) a.gmq[slot * 4 + dir] += s;
+    else if (dir == 4) a.g_nx[slot] += a.scale * s;
+    else if (dir == 5) a.g_ny[slot] += a.scale * s;
+    else if (dir == 6) a.g_bcu[slot] += a.scale * s;
+    else a.g_bcv[slot] += a.scale * s;
   }
 }
 
@@ -543,6 +552,27 @@ __global__ __launch_bounds__(128) void geo_vjp_kernel(int ncells, const double *
   }
 }
 
+// Compact copy of what the lift functional reads on the wall faces: per
+// (wall face w, face node i) the state at the node and nx, ny, fscale of the
+// slot, [nwall][Nfp][7]; then J of each wall cell, [nwall].
+__global__ void wall_gather_kernel(int nwall, int Np, int Nfp, const int *__restrict__ wcell,
+                                   const int *__restrict__ wface, const int *__restrict__ fmask,
+                                   const double *__restrict__ q, const double *__restrict__ nx,
+                                   const double *__restrict__ ny, const double *__restrict__ fs,
+                                   const double *__restrict__ J, double *out) {
+  const int NF = 3 * Nfp, n = nwall * Nfp;
+  for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
+    const int w = it / Nfp, i = it - w * Nfp;
+    const size_t k = wcell[w];
+    const int m = wface[w] * Nfp + i;
+    const size_t node = k * Np + fmask[m];
+    double *o = out + (size_t)it * 7;
+    for (int v = 0; v < 4; v++) o[v] = q[node * 4 + v];
+    o[4] = nx[k * NF + m]; o[5] = ny[k * NF + m]; o[6] = fs[k * NF + m];
+    if (i == 0) out[(size_t)n * 7 + w] = J[k];
+  }
+}
+
 // partial[b] = sum of x[i * stride + offset] over block b's contiguous
 // chunk of i, in a fixed order (deterministic for a fixed grid).
 __global__ __launch_bounds__(256) void strided_sum_kernel(size_t n, int stride, int offset,
@@ -595,6 +625,11 @@ static hipError_t adj_n(const Launch &L, const AdjArgs &a, const int *node_slots
                      dim3(kAdjThreads), 0, L.stream, a, L.ph, L.ops);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (a.nbedges > 0) {
+    const int nb = (a.nbedges * Ord<N>::Nfp * 8 + 255) / 256;
+    hipLaunchKernelGGL(adj_boundary_kernel<N>, dim3(nb), dim3(256), 0, L.stream, a, L.ph, L.fmask);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(adj_cell_kernel<N>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::Np)),
                      dim3(kAdjThreads), 0, L.stream, a, L.ph, L.ops, node_slots);
   return hipGetLastError();
@@ -653,6 +688,16 @@ hipError_t launch_strided_sum(const Launch &L, size_t n, int stride, int offset,
                               double *partial) {
   hipLaunchKernelGGL(strided_sum_kernel, dim3(kSumBlocks), dim3(256), 0, L.stream, n, stride, offset, x,
                      partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_wall_gather(const Launch &L, int nwall, const int *wcell, const int *wface, const double *q,
+                              const double *nx, const double *ny, const double *fs, const double *J,
+                              double *out) {
+  if (nwall == 0) return hipSuccess;
+  const int n = nwall * L.Nfp;
+  hipLaunchKernelGGL(wall_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, L.stream, nwall, L.Np, L.Nfp,
+                     wcell, wface, L.fmask, q, nx, ny, fs, J, out);
   return hipGetLastError();
 }
 

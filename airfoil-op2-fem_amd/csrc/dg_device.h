@@ -185,6 +185,15 @@ __device__ __forceinline__ unsigned long long max_bits(unsigned long long a, uns
   return a > b ? a : b;
 }
 
+// *dst = max(*dst, v) for a running maximum that only grows: a coherent
+// load first, and the atomic only when v is larger.  Thousands of waves
+// finishing together otherwise serialise on one address; a stale load only
+// costs an unneeded atomic.
+__device__ __forceinline__ void atomic_max_if_larger(unsigned long long *dst, unsigned long long v) {
+  const unsigned long long cur = __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (v > cur) atomicMax(dst, v);
+}
+
 // Max over a workgroup of non-negative doubles held as bit patterns (a NaN
 // pattern compares above +inf, so it propagates).
 template <int NT>

@@ -57,6 +57,10 @@ struct StageArgs {
   // dt_fixed[case] instead of cfl / dt1 (sensitivity runs replay a recorded
   // dt sequence).
   const double *dt_fixed;
+  // Host-side count of 16-cell tiles of a case table (0: unknown); lets the
+  // launcher size the persistent grid so that every wave gets the same
+  // number of tiles.
+  int ntiles_hint;
 };
 
 // Cells per tile of the matrix-core stage kernel (one wave per tile).

@@ -187,7 +187,7 @@ __global__ __launch_bounds__(NT) void rhs_kernel(StageArgs a, const double *__re
   }
   if (EPI == EPI_STAGE2) {
     unsigned long long r = block_max_bits<NT>(cfl_bits, sRed);
-    if (tid == 0) atomicMax(a.dt1_next, r);
+    if (tid == 0) atomic_max_if_larger(a.dt1_next, r);
     if (bad) atomicOr(a.flag, 1u);
     if (blockIdx.x == 0 && tid == 0) *a.t_dev += dt;
   }
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void calc_dt_kernel(int ncells, int N, int Np,
     }
   }
   unsigned long long r = block_max_bits<256>(best, red);
-  if (threadIdx.x == 0) atomicMax(out, r);
+  if (threadIdx.x == 0) atomic_max_if_larger(out, r);
 }
 
 // src/init_grid.h: one lane per cell.

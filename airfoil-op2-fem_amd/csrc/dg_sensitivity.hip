@@ -114,6 +114,8 @@ struct dg_sens {
   // (nx, ny, fscale), J cotangents, and the vertex-coordinate cotangents
   int *wslot;
   double *wcot, *gJ, *gv;
+  int *d_b2c, *d_bnum;  // boundary edges (cell, local face)
+  double *wbuf;         // compact wall data of the lift functional
   std::vector<double> h_facew;
   // recorded time steps
   std::vector<double> h_dts;
@@ -173,6 +175,7 @@ dg::StageArgs stage_args(dg_sens *s) {
   a.cfl = s->cfl; a.t_dev = s->tdev; a.flag = s->flag;
   a.ncells = s->K; a.ncases = 1; a.cell_off = s->cell_off; a.tile_off = s->tile_off;
   a.phys = s->phys_d;
+  a.ntiles_hint = (s->K + dg::kMfmaTile - 1) / dg::kMfmaTile;
   return a;
 }
 
@@ -277,22 +280,32 @@ int run_forward(dg_sens *s, bool record, int ckpt_every, double *cl) {
   return DG_OK;
 }
 
-// Tangent of C_L (host, dual arithmetic) and its cotangent: q at the wall
-// cells, geometry of the wall slots and of J.
+// What C_L reads on the wall, gathered on the device into a compact copy:
+// per (wall face w, face node i) the state at the node and nx, ny, fscale of
+// the slot; J per wall face's cell.  The tangent of C_L (host, dual
+// arithmetic) and its cotangent are formed from it.
 struct WallData {
-  std::vector<double> q, nx, ny, fs, J;
+  int nfp = 0;
+  size_t n = 0;             // wall face nodes
+  std::vector<double> buf;  // [n][7] then [nwall] J
+  double q(int w, int i, int v) const { return buf[((size_t)w * nfp + i) * 7 + v]; }
+  double nx(int w, int i) const { return buf[((size_t)w * nfp + i) * 7 + 4]; }
+  double ny(int w, int i) const { return buf[((size_t)w * nfp + i) * 7 + 5]; }
+  double fs(int w, int i) const { return buf[((size_t)w * nfp + i) * 7 + 6]; }
+  double J(int w) const { return buf[n * 7 + w]; }
 };
 
 int wall_data(dg_sens *s, const double *q, WallData &d) {
   dg_ctx *c = s->ctx;
-  const size_t K = s->K, Np = s->Np, NF = s->NF;
-  d.q.resize(K * Np * 4); d.nx.resize(K * NF); d.ny.resize(K * NF); d.fs.resize(K * NF); d.J.resize(K);
+  d.nfp = s->Nfp;
+  d.n = (size_t)s->nwall * s->Nfp;
+  d.buf.assign(d.n * 7 + s->nwall, 0.0);
+  HIPCHECK(c, dg::launch_wall_gather(launch_a(s), s->nwall, s->wcell, s->wface, q, s->nx, s->ny, s->fs, s->J,
+                                     s->wbuf));
+  if (!d.buf.empty())
+    HIPCHECK(c, hipMemcpyAsync(d.buf.data(), s->wbuf, sizeof(double) * d.buf.size(), hipMemcpyDeviceToHost,
+                               s->stream));
   HIPCHECK(c, hipStreamSynchronize(s->stream));
-  HIPCHECK(c, hipMemcpy(d.q.data(), q, sizeof(double) * d.q.size(), hipMemcpyDeviceToHost));
-  HIPCHECK(c, hipMemcpy(d.nx.data(), s->nx, sizeof(double) * d.nx.size(), hipMemcpyDeviceToHost));
-  HIPCHECK(c, hipMemcpy(d.ny.data(), s->ny, sizeof(double) * d.ny.size(), hipMemcpyDeviceToHost));
-  HIPCHECK(c, hipMemcpy(d.fs.data(), s->fs, sizeof(double) * d.fs.size(), hipMemcpyDeviceToHost));
-  HIPCHECK(c, hipMemcpy(d.J.data(), s->J, sizeof(double) * d.J.size(), hipMemcpyDeviceToHost));
   return DG_OK;
 }
 
@@ -386,6 +399,9 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   s->wcot = b.alloc<double>(3 * (size_t)s->nwall * s->Nfp);
   s->gJ = b.alloc<double>(K);
   s->gv = b.alloc<double>(std::max<size_t>(6 * K, 5 * dg::kSumBlocks));  // also the partial sums
+  s->wbuf = b.alloc<double>((size_t)s->nwall * (7 * s->Nfp + 1));
+  s->d_b2c = b.alloc<int>(M.nbedges);
+  s->d_bnum = b.alloc<int>(M.nbedges);
   e = b.err;
   dg::Operators o;
   const char *oe = nullptr;
@@ -411,6 +427,8 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   up(s->wcell, s->h_wcell.data(), sizeof(int) * s->nwall);
   up(s->wface, s->h_wface.data(), sizeof(int) * s->nwall);
   up(s->facew, s->h_facew.data(), sizeof(double) * s->Nfp);
+  up(s->d_b2c, M.bedge2cells.data(), sizeof(int) * M.nbedges);
+  up(s->d_bnum, M.bedgeNum.data(), sizeof(int) * M.nbedges);
   if (e != hipSuccess) {
     dg_sens_destroy(s);
     return hipfail(c, e, "dg_sens_create: allocation");
@@ -555,12 +573,12 @@ int dg_sens_tangent(dg_sens *s, double dalpha, const double *dbumps, double *cl,
       const int m = f * s->Nfp + i;
       const size_t node = k * Np + c->FMASK[m];
       dg::Dual qq[4];
-      for (int v = 0; v < 4; v++) qq[v] = dg::Dual(wd.q[4 * node + v], qdh[4 * node + v]);
+      for (int v = 0; v < 4; v++) qq[v] = dg::Dual(wd.q(w, i, v), qdh[4 * node + v]);
       dg::Dual rho = qq[0], u = qq[1] / rho, vv = qq[2] / rho;
       dg::Dual p = g1 * (qq[3] - 0.5 * rho * (u * u + vv * vv));
-      dg::Dual sJ = dg::Dual(wd.fs[k * NF + m], fsd[k * NF + m]) * dg::Dual(wd.J[k], Jd[k]);
-      bx += s->h_facew[i] * sJ * p * dg::Dual(wd.nx[k * NF + m], nxd[k * NF + m]);
-      by += s->h_facew[i] * sJ * p * dg::Dual(wd.ny[k * NF + m], nyd[k * NF + m]);
+      dg::Dual sJ = dg::Dual(wd.fs(w, i), fsd[k * NF + m]) * dg::Dual(wd.J(w), Jd[k]);
+      bx += s->h_facew[i] * sJ * p * dg::Dual(wd.nx(w, i), nxd[k * NF + m]);
+      by += s->h_facew[i] * sJ * p * dg::Dual(wd.ny(w, i), nyd[k * NF + m]);
     }
     Fx += bx; Fy += by;
   }
@@ -609,14 +627,13 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
     const size_t k = s->h_wcell[w];
     const int f = s->h_wface[w];
     double bx = 0.0, by = 0.0;
+    (void)k; (void)f;
     for (int i = 0; i < s->Nfp; i++) {
-      const int m = f * s->Nfp + i;
-      const size_t node = k * Np + c->FMASK[m];
-      const double rho = wd.q[4 * node], u = wd.q[4 * node + 1] / rho, v = wd.q[4 * node + 2] / rho;
-      const double p = g1 * (wd.q[4 * node + 3] - 0.5 * rho * (u * u + v * v));
-      const double sJ = wd.fs[k * NF + m] * wd.J[k];
-      bx += s->h_facew[i] * sJ * p * wd.nx[k * NF + m];
-      by += s->h_facew[i] * sJ * p * wd.ny[k * NF + m];
+      const double rho = wd.q(w, i, 0), u = wd.q(w, i, 1) / rho, v = wd.q(w, i, 2) / rho;
+      const double p = g1 * (wd.q(w, i, 3) - 0.5 * rho * (u * u + v * v));
+      const double sJ = wd.fs(w, i) * wd.J(w);
+      bx += s->h_facew[i] * sJ * p * wd.nx(w, i);
+      by += s->h_facew[i] * sJ * p * wd.ny(w, i);
     }
     Fx += bx; Fy += by;
   }
@@ -634,10 +651,10 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
     for (int i = 0; i < s->Nfp; i++) {
       const int m = f * s->Nfp + i;
       const size_t node = k * Np + c->FMASK[m];
-      const double q0 = wd.q[4 * node], q1 = wd.q[4 * node + 1], q2 = wd.q[4 * node + 2], q3 = wd.q[4 * node + 3];
+      const double q0 = wd.q(w, i, 0), q1 = wd.q(w, i, 1), q2 = wd.q(w, i, 2), q3 = wd.q(w, i, 3);
       const double u = q1 / q0, v = q2 / q0;
       const double p = g1 * (q3 - 0.5 * q0 * (u * u + v * v));
-      const double nxv = wd.nx[k * NF + m], nyv = wd.ny[k * NF + m], fsv = wd.fs[k * NF + m], Jv = wd.J[k];
+      const double nxv = wd.nx(w, i), nyv = wd.ny(w, i), fsv = wd.fs(w, i), Jv = wd.J(w);
       const double wq = s->h_facew[i];
       const double kn = kx * nxv + ky * nyv;
       const double dp = wq * fsv * Jv * kn;
@@ -689,6 +706,7 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   a.fmap = s->fmap; a.fmask = c->d_fmask; a.gmq = s->gmq; a.gpq = s->gpq;
   a.g_rx = s->g_rx; a.g_ry = s->g_ry; a.g_sx = s->g_sx; a.g_sy = s->g_sy;
   a.g_nx = s->g_nx; a.g_ny = s->g_ny; a.g_fs = s->g_fs; a.g_bcu = s->g_bcu; a.g_bcv = s->g_bcv;
+  a.nbedges = s->mesh.nbedges; a.b2c = s->d_b2c; a.bnum = s->d_bnum;
   for (int sg = nck - 1; sg >= 0; sg--) {
     const int n0 = sg * ckpt_every, n1 = std::min(s->nsteps, n0 + ckpt_every);
     if (sg != nck - 1) {
@@ -813,6 +831,7 @@ int dg_rhs_adjoint(dg_ctx *c, const double *w, const double *mu, double scale, c
   a.fmap = c->d_fmap; a.fmask = c->d_fmask;
   a.gmq = scratch; a.gpq = scratch + (size_t)c->ncells * c->NF * 4;
   a.x0 = acc; a.e0 = 1.0; a.out = acc;
+  a.nbedges = c->nbedges; a.b2c = c->d_b2c; a.bnum = c->d_bnum;
   a.g_rx = grx; a.g_ry = gry; a.g_sx = gsx; a.g_sy = gsy;
   a.g_nx = gnx; a.g_ny = gny; a.g_fs = gfs; a.g_bcu = gbcu; a.g_bcv = gbcv;
   HIPCHECK(c, dg::launch_rhs_adj(dg::launch_of(c), a, c->d_node_slots));

@@ -159,6 +159,7 @@ dg::StageArgs stage_args(dg_solver *s, double cfl) {
   a.cell_off = s->d_cell_off;
   a.tile_off = s->d_tile_off;
   a.phys = s->d_phys;
+  a.ntiles_hint = s->tile_off.back();
   return a;
 }
 

@@ -95,6 +95,8 @@ hipError_t launch_set_vortex(const Launch &L, int ncells, const double *x,
                              double y0, double period_x, double *q, double *w);
 hipError_t launch_zero(const Launch &L, size_t n, double *p);
 hipError_t launch_copy(const Launch &L, size_t n, const double *src, double *dst);
+// out[i][0..4) = q node record rec[i] (zeros where rec[i] < 0)
+hipError_t launch_gather_records(const Launch &L, int n, const long long *rec, const double *q, double *out);
 // Pressure forces on the listed wall faces: out[2*b] = (fx, fy) of face b;
 // cp[b*Nfp + i] = (p - p_inf) / qdyn at its nodes.
 hipError_t launch_wall_forces(const Launch &L, int nwall, const int *wall_cell,

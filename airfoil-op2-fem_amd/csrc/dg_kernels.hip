@@ -579,4 +579,20 @@ hipError_t launch_copy(const Launch &L, size_t n, const double *src, double *dst
   return hipGetLastError();
 }
 
+// out[i] = the 4-double node record rec[i] of q (zeros for rec[i] < 0).
+__global__ void gather_records_kernel(int n, const long long *__restrict__ rec, const double *__restrict__ q,
+                                      double *out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const long long r = rec[i];
+    for (int v = 0; v < 4; v++) out[4 * (size_t)i + v] = r >= 0 ? q[4 * r + v] : 0.0;
+  }
+}
+
+hipError_t launch_gather_records(const Launch &L, int n, const long long *rec, const double *q, double *out) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_records_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, L.stream, n, rec, q,
+                     out);
+  return hipGetLastError();
+}
+
 }  // namespace dg

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
@@ -64,6 +65,11 @@ struct dg_solver {
   double gcfl = -1.0;
   // launch shape of the matrix-core stage kernel (autotuned or set)
   int wpb = 1, walk = 0, max_blocks = 0;
+  // vertices and cells of every case (host), for the vertex export
+  std::vector<int> node_off;         // [ncases + 1]
+  std::vector<double> h_coords;      // [nodes][2], cases concatenated
+  std::vector<int> h_cells;          // [K][3], node indices local to the case
+  std::vector<std::vector<long long>> vrec;  // per case: q record of each vertex
 };
 
 namespace {
@@ -216,6 +222,7 @@ int dg_solver_create_batch(dg_ctx *c, int ncases, const dg_mesh *const *meshes, 
   s->cell_off.push_back(0);
   s->tile_off.push_back(0);
   s->wall_off.push_back(0);
+  s->node_off.push_back(0);
   long long ktot = 0;
   for (int k = 0; k < ncases; k++) {
     if (!meshes[k]) { delete s; return fail(c, DG_ERR_ARG, "dg_solver_create_batch: null mesh"); }
@@ -242,6 +249,9 @@ int dg_solver_create_batch(dg_ctx *c, int ncases, const dg_mesh *const *meshes, 
       }
     }
     cell_xy.insert(cell_xy.end(), M.cell_xy.begin(), M.cell_xy.end());
+    s->h_coords.insert(s->h_coords.end(), M.coords.begin(), M.coords.end());
+    s->h_cells.insert(s->h_cells.end(), M.cells.begin(), M.cells.end());
+    s->node_off.push_back(s->node_off.back() + M.nnodes);
     s->cell_off.push_back((int)ktot);
     s->tile_off.push_back(s->tile_off.back() + (M.ncells + TE - 1) / TE);
     s->wall_off.push_back((int)s->h_wcell.size());
@@ -647,6 +657,102 @@ int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int 
   }
   if (avg_ms) *avg_ms = it->second.first / it->second.second;
   if (count) *count = it->second.second;
+  return DG_OK;
+}
+
+// Vertex values of case k as src/save_solution.cpp:13-47 forms them: each
+// vertex takes the state at the matching corner node (0, N, Np-1 for cell
+// vertices 0, 1, 2) of the last cell, in cell order, that contains it;
+// density, velocity and pressure from that state; vertices of no cell stay 0.
+static int vertex_fields(dg_solver *s, int k, std::vector<double> &rho, std::vector<double> &vx,
+                         std::vector<double> &vy, std::vector<double> &p) {
+  dg_ctx *c = s->ctx;
+  const int nn = s->node_off[k + 1] - s->node_off[k];
+  const int c0 = s->cell_off[k], kc = s->cell_off[k + 1] - s->cell_off[k];
+  if ((int)s->vrec.size() < s->ncases) s->vrec.resize(s->ncases);
+  std::vector<long long> &rec = s->vrec[k];
+  if ((int)rec.size() != nn) {
+    rec.assign(nn, -1);
+    const int corner[3] = {0, c->N, c->Np - 1};
+    for (int e = 0; e < kc; e++)
+      for (int t = 0; t < 3; t++)
+        rec[s->h_cells[3 * ((size_t)c0 + e) + t]] = ((long long)c0 + e) * c->Np + corner[t];
+  }
+  rho.assign(nn, 0.0); vx.assign(nn, 0.0); vy.assign(nn, 0.0); p.assign(nn, 0.0);
+  if (nn == 0) return DG_OK;
+  long long *d_rec = nullptr;
+  double *d_out = nullptr;
+  std::vector<double> out(4 * (size_t)nn);
+  hipError_t e = hipMalloc(&d_rec, sizeof(long long) * nn);
+  if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * out.size());
+  if (e == hipSuccess) e = hipMemcpyAsync(d_rec, rec.data(), sizeof(long long) * nn, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = dg::launch_gather_records(launch_s(s), nn, d_rec, s->q, d_out);
+  if (e == hipSuccess) e = hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(d_rec);
+  (void)hipFree(d_out);
+  if (e != hipSuccess) return hipfail(c, e, "dg_solver_vertex_fields");
+  const double gam = s->phys[k].gam;
+  for (int i = 0; i < nn; i++) {
+    if (rec[i] < 0) continue;
+    const double *q = &out[4 * (size_t)i];
+    rho[i] = q[0];
+    vx[i] = q[1] / rho[i];
+    vy[i] = q[2] / rho[i];
+    p[i] = (gam - 1) * (q[3] - 0.5 * (q[1] * vx[i] + q[2] * vy[i]));
+  }
+  return DG_OK;
+}
+
+int dg_solver_vertex_fields(dg_solver *s, int k, int *nnodes, double *rho, double *vx, double *vy, double *p) {
+  size_t off, n;
+  int rc = case_range(s, k, &off, &n, "dg_solver_vertex_fields");
+  if (rc != DG_OK) return rc;
+  const int nn = s->node_off[k + 1] - s->node_off[k];
+  if (nnodes) *nnodes = nn;
+  if (!rho && !vx && !vy && !p) return DG_OK;
+  std::vector<double> a, b, cc, d;
+  rc = vertex_fields(s, k, a, b, cc, d);
+  if (rc != DG_OK) return rc;
+  if (rho) std::memcpy(rho, a.data(), sizeof(double) * nn);
+  if (vx) std::memcpy(vx, b.data(), sizeof(double) * nn);
+  if (vy) std::memcpy(vy, cc.data(), sizeof(double) * nn);
+  if (p) std::memcpy(p, d.data(), sizeof(double) * nn);
+  return DG_OK;
+}
+
+int dg_solver_write_vtk(dg_solver *s, int k, const char *path) {
+  size_t off, n;
+  int rc = case_range(s, k, &off, &n, "dg_solver_write_vtk");
+  if (rc != DG_OK) return rc;
+  if (!path) return fail(s->ctx, DG_ERR_ARG, "dg_solver_write_vtk: null path");
+  std::vector<double> rho, vx, vy, p;
+  rc = vertex_fields(s, k, rho, vx, vy, p);
+  if (rc != DG_OK) return rc;
+  FILE *f = std::fopen(path, "w");
+  if (!f) return fail(s->ctx, DG_ERR_ARG, std::string("dg_solver_write_vtk: cannot open ") + path);
+  const int nn = s->node_off[k + 1] - s->node_off[k];
+  const int c0 = s->cell_off[k], kc = s->cell_off[k + 1] - s->cell_off[k];
+  const double *xy = &s->h_coords[2 * (size_t)s->node_off[k]];
+  std::fprintf(f, "# vtk DataFile Version 3.0\nDG Euler solution (vertex values)\nASCII\nDATASET UNSTRUCTURED_GRID\n");
+  std::fprintf(f, "POINTS %d double\n", nn);
+  for (int i = 0; i < nn; i++) std::fprintf(f, "%.17g %.17g 0\n", xy[2 * i], xy[2 * i + 1]);
+  std::fprintf(f, "CELLS %d %d\n", kc, 4 * kc);
+  for (int e = 0; e < kc; e++) {
+    const int *cl = &s->h_cells[3 * ((size_t)c0 + e)];
+    std::fprintf(f, "3 %d %d %d\n", cl[0], cl[1], cl[2]);
+  }
+  std::fprintf(f, "CELL_TYPES %d\n", kc);
+  for (int e = 0; e < kc; e++) std::fprintf(f, "5\n");
+  std::fprintf(f, "POINT_DATA %d\n", nn);
+  const char *names[4] = {"Density", "Pressure", "VelocityX", "VelocityY"};
+  const std::vector<double> *vals[4] = {&rho, &p, &vx, &vy};
+  for (int a = 0; a < 4; a++) {
+    std::fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", names[a]);
+    for (int i = 0; i < nn; i++) std::fprintf(f, "%.17g\n", (*vals[a])[i]);
+  }
+  const bool ok = std::fclose(f) == 0;
+  if (!ok) return fail(s->ctx, DG_ERR_ARG, std::string("dg_solver_write_vtk: write failed: ") + path);
   return DG_OK;
 }
 

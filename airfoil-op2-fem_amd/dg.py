@@ -397,6 +397,21 @@ class Solver:
     def profile(self, enable=True):
         self._chk(lib.dg_solver_profile(self._h, 1 if enable else 0))
 
+    def vertex_fields(self, case=0):
+        """Vertex values as src/save_solution.cpp forms them: dict of
+        Density, Pressure, VelocityX, VelocityY arrays [nnodes]."""
+        n = c_int()
+        self._chk(lib.dg_solver_vertex_fields(self._h, int(case), ctypes.byref(n), None, None, None, None))
+        out = {k: np.zeros(n.value) for k in ("Density", "VelocityX", "VelocityY", "Pressure")}
+        self._chk(lib.dg_solver_vertex_fields(self._h, int(case), ctypes.byref(n), _np_ptr(out["Density"]),
+                                              _np_ptr(out["VelocityX"]), _np_ptr(out["VelocityY"]),
+                                              _np_ptr(out["Pressure"])))
+        return out
+
+    def write_vtk(self, path, case=0):
+        """Mesh and vertex fields as a legacy-VTK ASCII file."""
+        self._chk(lib.dg_solver_write_vtk(self._h, int(case), str(path).encode()))
+
     def set_launch(self, wpb=1, walk=0, max_blocks=0):
         """Launch shape of the matrix-core stage kernel (results unchanged)."""
         self._chk(lib.dg_solver_set_launch(self._h, int(wpb), int(walk), int(max_blocks)))

@@ -57,16 +57,27 @@ def load_traffic(kernel):
         return None
 
 
-def summarize_trace(db):
-    """Per-kernel durations from a rocprofv3 --kernel-trace --stats database."""
+def summarize_trace(db, last=0):
+    """Per-kernel durations from a rocprofv3 --kernel-trace --stats database;
+    with last > 0 also the average over each kernel's last `last` dispatches
+    (the launches after the autotuner's trials of other shapes)."""
+    import collections
     import sqlite3
     cur = sqlite3.connect(db).cursor()
     rows = cur.execute("select name, total_calls, total_duration, average, percentage "
                        "from top_kernels order by total_duration desc").fetchall()
-    print("| kernel | calls | total (us) | average (us) | % |")
-    print("|---|---|---|---|---|")
+    tail = {}
+    if last > 0:
+        per = collections.defaultdict(list)
+        for name, start, end in cur.execute("select name, start, end from kernels order by start"):
+            per[name].append((end - start) / 1e3)
+        tail = {k: sum(v[-last:]) / len(v[-last:]) for k, v in per.items()}
+    hdr = f" last {last} average (us) |" if last > 0 else ""
+    print("| kernel | calls | total (us) | average (us) |" + hdr + " % |")
+    print("|---|---|---|---|" + ("---|" if last > 0 else "") + "---|")
     for name, calls, tot, avg, pct in rows:
-        print(f"| `{name.split('(')[0]}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.2f} |")
+        extra = f" {tail[name]:.2f} |" if name in tail else (" |" if last > 0 else "")
+        print(f"| `{name.split('(')[0]}` | {calls} | {tot:.1f} | {avg:.2f} |{extra} {pct:.2f} |")
 
 
 def summarize_pmc(fetch_db, write_db, cfetch_db, cwrite_db, copy_bytes):
@@ -327,7 +338,7 @@ def main():
         ap.add_argument("--order", type=int, default=3)
         return adjoint_bench(ap.parse_args())
     if len(sys.argv) > 2 and sys.argv[1] == "--summarize-trace":
-        return summarize_trace(sys.argv[2])
+        return summarize_trace(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 0)
     if len(sys.argv) > 6 and sys.argv[1] == "--summarize-pmc":
         return summarize_pmc(*sys.argv[2:6], float(sys.argv[6]))
     ap = argparse.ArgumentParser()

@@ -260,6 +260,15 @@ int dg_solver_forces(dg_solver *s, double chord, double *cl, double *cd, double 
  * each wall face, in wall-face order, with the nodes' x coordinate.  With
  * x and cp NULL only *nwall is returned. */
 int dg_solver_wall_cp(dg_solver *s, int *nwall, double *x, double *cp);
+/* Replaces save_solution (src/save_solution.cpp:13-47) without CGNS: vertex
+ * values of case k (0 for a single-case solver), each vertex taking the
+ * state at the corner node (0, N, Np-1) of the last cell containing it;
+ * *nnodes receives the vertex count (arrays may be NULL to query it).
+ * dg_solver_write_vtk writes the case's mesh and the four fields (Density,
+ * Pressure, VelocityX, VelocityY) as a legacy-VTK ASCII file. */
+int dg_solver_vertex_fields(dg_solver *s, int k, int *nnodes, double *rho, double *vx, double *vy,
+                            double *p);
+int dg_solver_write_vtk(dg_solver *s, int k, const char *path);
 /* Kernel timing with HIP events on the context stream: enable, then read the
  * average duration (ms) and launch count per kernel name ("rhs_stage0",
  * "rhs_stage1", "rhs_stage2", "calc_dt", "euler_rhs", ...). */

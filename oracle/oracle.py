@@ -300,6 +300,25 @@ class Oracle:
         return err
 
 
+def vertex_fields(cells, q, gam, nnodes, order):
+    """src/save_solution.cpp:13-47 for order N: every cell, in order, writes
+    its corner nodes (0, N, Np-1 for its vertices 0, 1, 2) into the vertex
+    arrays, so the last cell containing a vertex wins; vertices of no cell
+    stay 0.  Returns (rho, velX, velY, p)."""
+    Np = (order + 1) * (order + 2) // 2
+    q = np.asarray(q, dtype=np.float64).reshape(-1, Np, 4)
+    rho, vx, vy, p = (np.zeros(nnodes) for _ in range(4))
+    corner = (0, order, Np - 1)
+    for i, cl in enumerate(np.asarray(cells).reshape(-1, 3)):
+        for t in range(3):
+            n, qq = int(cl[t]), q[i, corner[t]]
+            rho[n] = qq[0]
+            vx[n] = qq[1] / rho[n]
+            vy[n] = qq[2] / rho[n]
+            p[n] = (gam - 1) * (qq[3] - 0.5 * (qq[1] * vx[n] + qq[2] * vy[n]))
+    return rho, vx, vy, p
+
+
 def build_edges(cells):
     """tools/vtk2CGNS.cpp:112-199 edge contract (C restatement)."""
     cells = np.ascontiguousarray(cells, dtype=np.int32)

@@ -281,3 +281,29 @@ def test_reference_tables_order4(dg, orc, dev):
                   geo["ny"], out)
     ref, _ = o.rhs_all(case, w)
     assert max(rel_per_var(out.cpu().numpy(), ref)) <= 1e-12
+
+
+def test_vertex_export_matches_oracle(dg, orc, tmp_path):
+    """Vertex values (save_solution semantics) of a solver state and the
+    legacy-VTK file carrying them."""
+    mesh = dg.naca0012_ogrid(24, 6, 1.15, 8.0)
+    ctx = dg.Context(3)
+    ctx.set_freestream(GAM, 0.45, 2.0, 1.0, 1.0)
+    s = dg.Solver(ctx, mesh)
+    s.run(4, cfl=1.0)
+    v = s.vertex_fields()
+    rho, vx, vy, p = orc.vertex_fields(mesh.cells, s.get_q(), GAM, mesh.nnodes, 3)
+    for got, ref in ((v["Density"], rho), (v["VelocityX"], vx), (v["VelocityY"], vy), (v["Pressure"], p)):
+        assert np.array_equal(got, ref)
+    path = tmp_path / "sol.vtk"
+    s.write_vtk(path)
+    tok = open(path).read().split()
+    i = tok.index("POINTS")
+    assert int(tok[i + 1]) == mesh.nnodes
+    pts = np.array(tok[i + 3:i + 3 + 3 * mesh.nnodes], dtype=float).reshape(-1, 3)
+    assert np.array_equal(pts[:, :2], mesh.coords)
+    j = tok.index("CELLS")
+    assert int(tok[j + 1]) == mesh.ncells
+    k = tok.index("Density")
+    dens = np.array(tok[k + 5:k + 5 + mesh.nnodes], dtype=float)
+    assert np.array_equal(dens, rho)

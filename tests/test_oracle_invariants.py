@@ -308,3 +308,24 @@ def test_wall_force_of_uniform_pressure_vanishes():
         assert np.abs(f).max() < 1e-13
         # and a pressure p = x on the surface integrates to -(enclosed area) in x
         # (divergence theorem with the outward normal of the fluid pointing into the body)
+
+
+def test_vertex_fields_last_cell_wins():
+    """save_solution's vertex arrays (src/save_solution.cpp:23-47): cell i
+    writes its vertices 0, 1, 2 from nodes 0, N, Np-1, later cells overwrite
+    shared vertices, untouched vertices stay 0.  Hand case, N=3 (Np=10)."""
+    N, Np, gam = 3, 10, 1.4
+    cells = np.array([[0, 1, 2], [2, 1, 3]])
+    q = np.zeros((2, Np, 4))
+    for c in range(2):
+        for n in range(Np):
+            q[c, n] = [1.0 + c + 0.1 * n, 0.5 + n, -0.25 * c, 3.0 + c + n]
+    rho, vx, vy, p = oracle.vertex_fields(cells, q, gam, 5, N)
+    # vertex -> (cell, node) that must survive
+    expect = {0: (0, 0), 1: (1, N), 2: (1, 0), 3: (1, Np - 1)}
+    for v, (c, n) in expect.items():
+        r, mx, my, e = q[c, n]
+        assert rho[v] == r
+        assert vx[v] == mx / r and vy[v] == my / r
+        assert p[v] == (gam - 1) * (e - 0.5 * (mx * (mx / r) + my * (my / r)))
+    assert rho[4] == 0.0 and p[4] == 0.0

@@ -93,6 +93,7 @@ _SIGS = {
     "dg_sens_gradient_breakdown": ([c_void_p, P(c_double), P(c_double), P(c_double)], c_int),
     "dg_solver_profile": ([c_void_p, c_int], c_int),
     "dg_solver_set_launch": ([c_void_p, c_int, c_int, c_int], c_int),
+    "dg_solver_set_geometry": ([c_void_p, c_int], c_int),
     "dg_solver_vertex_fields": ([c_void_p, c_int, P(c_int), c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "dg_solver_write_vtk": ([c_void_p, c_int, c_char_p], c_int),
     "dg_solver_launch_config": ([c_void_p, P(c_int), P(c_int), P(c_int)], c_int),

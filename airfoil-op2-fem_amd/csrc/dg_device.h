@@ -46,6 +46,23 @@ __device__ __forceinline__ double sqrt_nr(double x) {
   return __builtin_fma(d, h, g);
 }
 
+// Outward unit normal and fscale of face f of a straight-sided cell from its
+// metric factors (rx, ry, sx, sy).  src/init_grid.h:80-104 forms the face
+// vectors (yr, -xr), (ys - yr, xr - xs), (-ys, xs) and divides by their
+// length sJ, with fscale = sJ / J; since rx = ys/J, ry = -xs/J, sx = -yr/J,
+// sy = xr/J, those vectors are J (-sx, -sy), J (rx + sx, ry + sy) and
+// J (-rx, -ry), so for J > 0 the unit normal is the normalised bracket and
+// fscale is the bracket's length.
+__device__ __forceinline__ void cell_face_geometry(double rx, double ry, double sx, double sy, int f,
+                                                   double &nx, double &ny, double &fs) {
+  const double a = f == 0 ? -sx : (f == 1 ? rx + sx : -rx);
+  const double b = f == 0 ? -sy : (f == 1 ? ry + sy : -ry);
+  fs = sqrt_nr(a * a + b * b);
+  const double i = rcp_nr(fs);
+  nx = a * i;
+  ny = b * i;
+}
+
 // ------------------------------------------------------------------------
 // Point-wise physics (src/euler_rhs.h:7-23, 120-220; src/get_bedge_q.h)
 // ------------------------------------------------------------------------

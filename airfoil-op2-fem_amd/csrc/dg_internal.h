@@ -36,6 +36,11 @@ struct StageArgs {
   double *ext;            // exteriorQ (drop-in euler_rhs only)
   const double *rx, *ry, *sx, *sy;   // [K][Np]
   const double *fscale, *nx, *ny;    // [K][NF]
+  // Per-cell metric factors of straight-sided cells, [K][4] = (rx, ry, sx,
+  // sy), or null.  When set, the matrix-core kernel reads these instead of
+  // the per-node factors and forms each face's normal and fscale from them
+  // (see cell_face_geometry); the per-node arrays are then not read.
+  const double *cmet;
   const int *fmap;        // [K][3] face map
   double *out;            // qRHS, or the next state
   const double *q;        // SSPRK3 base state (stages 1, 2)
@@ -89,6 +94,9 @@ hipError_t launch_init_grid(const Launch &L, int ncells, const double *cell_xy,
                             double *xr, double *yr, double *xs, double *ys,
                             double *rx, double *ry, double *sx, double *sy,
                             double *nx, double *ny, double *fscale, double *J);
+// cmet[c] = (rx, ry, sx, sy) at node 0 of cell c, [K][4]
+hipError_t launch_cell_metrics(const Launch &L, int ncells, const double *rx, const double *ry,
+                               const double *sx, const double *sy, double *cmet);
 hipError_t launch_set_ic(const Launch &L, int ncells, double *q, double *w);
 hipError_t launch_set_vortex(const Launch &L, int ncells, const double *x,
                              const double *y, double uinf, double beta, double x0,

@@ -548,6 +548,29 @@ hipError_t launch_init_grid(const Launch &L, int ncells, const double *cell_xy, 
   return hipGetLastError();
 }
 
+// Per-cell metric record (rx, ry, sx, sy) of straight-sided cells: the
+// factors init_grid wrote at the cell's node 0 (equal at every node up to
+// rounding).
+__global__ void cell_metrics_kernel(int ncells, int Np, const double *__restrict__ rx,
+                                    const double *__restrict__ ry, const double *__restrict__ sx,
+                                    const double *__restrict__ sy, double *cmet) {
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < (size_t)ncells;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const size_t n = c * Np;
+    double2 *o = reinterpret_cast<double2 *>(cmet + 4 * c);
+    o[0] = make_double2(rx[n], ry[n]);
+    o[1] = make_double2(sx[n], sy[n]);
+  }
+}
+
+hipError_t launch_cell_metrics(const Launch &L, int ncells, const double *rx, const double *ry,
+                               const double *sx, const double *sy, double *cmet) {
+  if (ncells == 0) return hipSuccess;
+  hipLaunchKernelGGL(cell_metrics_kernel, dim3(grid_for(ncells, 256, 8192)), dim3(256), 0, L.stream,
+                     ncells, L.Np, rx, ry, sx, sy, cmet);
+  return hipGetLastError();
+}
+
 hipError_t launch_set_ic(const Launch &L, int ncells, double *q, double *w) {
   size_t n = (size_t)ncells * L.Np;
   if (n == 0) return hipSuccess;

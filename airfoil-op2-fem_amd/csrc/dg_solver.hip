@@ -65,6 +65,9 @@ struct dg_solver {
   double gcfl = -1.0;
   // launch shape of the matrix-core stage kernel (autotuned or set)
   int wpb = 1, walk = 0, max_blocks = 0;
+  // stage kernels read the per-cell metric record (DG_GEOM_CELL) or the
+  // per-node factors, normals and fscale (DG_GEOM_NODE)
+  bool cell_geometry = false;
   // vertices and cells of every case (host), for the vertex export
   std::vector<int> node_off;         // [ncases + 1]
   std::vector<double> h_coords;      // [nodes][2], cases concatenated
@@ -159,6 +162,7 @@ dg::StageArgs stage_args(dg_solver *s, double cfl) {
   dg::StageArgs a{};
   a.rx = dat(s, "rx"); a.ry = dat(s, "ry"); a.sx = dat(s, "sx"); a.sy = dat(s, "sy");
   a.fscale = dat(s, "fscale"); a.nx = dat(s, "nx"); a.ny = dat(s, "ny");
+  a.cmet = s->cell_geometry ? dat(s, "cmet") : nullptr;
   a.fmap = s->d_fmap; a.q = s->q; a.cfl = cfl; a.t_dev = s->t_dev; a.flag = s->flag;
   a.ncells = s->K;
   a.ncases = s->ncases;
@@ -280,7 +284,7 @@ int dg_solver_create_batch(dg_ctx *c, int ncases, const dg_mesh *const *meshes, 
       {"xr", K * Np},      {"yr", K * Np},           {"xs", K * Np},      {"ys", K * Np},
       {"rx", K * Np},      {"ry", K * Np},           {"sx", K * Np},      {"sy", K * Np},
       {"nx", K * NF},      {"ny", K * NF},           {"fscale", K * NF},  {"nodeX", K * 3},
-      {"nodeY", K * 3},    {"J", K},                 {"cell_xy", K * 6},
+      {"nodeY", K * 3},    {"J", K},                 {"cell_xy", K * 6},  {"cmet", K * 4},
   };
   for (auto &l : layout) {
     if (e != hipSuccess) break;
@@ -328,6 +332,8 @@ int dg_solver_create_batch(dg_ctx *c, int ncases, const dg_mesh *const *meshes, 
                            dat(s, "x"), dat(s, "y"), dat(s, "xr"), dat(s, "yr"), dat(s, "xs"),
                            dat(s, "ys"), dat(s, "rx"), dat(s, "ry"), dat(s, "sx"), dat(s, "sy"),
                            dat(s, "nx"), dat(s, "ny"), dat(s, "fscale"), dat(s, "J"));
+  if (e == hipSuccess)
+    e = dg::launch_cell_metrics(L, s->K, dat(s, "rx"), dat(s, "ry"), dat(s, "sx"), dat(s, "sy"), dat(s, "cmet"));
   for (int k = 0; k < ncases && e == hipSuccess; k++) {
     dg::Launch Lk = L;
     Lk.ph = s->phys[k];
@@ -761,6 +767,14 @@ int dg_solver_set_launch(dg_solver *s, int wpb, int walk, int max_blocks) {
     return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_set_launch: wpb in {1,2,4}, walk in {0,1}, max_blocks >= 0");
   drop_graph(s);
   s->wpb = wpb; s->walk = walk; s->max_blocks = max_blocks;
+  return DG_OK;
+}
+
+int dg_solver_set_geometry(dg_solver *s, int mode) {
+  if (!s || (mode != DG_GEOM_CELL && mode != DG_GEOM_NODE))
+    return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_set_geometry: mode is DG_GEOM_CELL or DG_GEOM_NODE");
+  drop_graph(s);
+  s->cell_geometry = mode == DG_GEOM_CELL;
   return DG_OK;
 }
 

@@ -2,8 +2,7 @@
 //
 // Each wave (64 lanes) owns tiles of 16 consecutive cells, split into 4
 // groups of 4 cells; a workgroup is 1, 2 or 4 such waves.  For one group the
-// volume term is
-// four 16x16 products on v_mfma_f64_16x16x4f64,
+// volume term is four 16x16 products on v_mfma_f64_16x16x4f64,
 //
 //     D[cell + 4*var][node j] = sum_k  A[cell + 4*var][k] * B[k][j],
 //
@@ -14,16 +13,33 @@
 // LDS already in the operand layout, so every MFMA reads one contiguous
 // 512-byte row.  The FP64 accumulator layout is row-interleaved: lane l holds
 // D[l/16 + 4*r][l%16], r = 0..3.  Rows are therefore ordered (cell + 4*var),
-// so lane l holds cell l/16 of the group, node l%16, variables r = 0..3: the metric
-// terms are then lane-local, the LIFT product (A = Roe fluxes, B = -LIFT^T)
-// accumulates onto the same registers, and the epilogue stores each lane's
-// 4 variables of one node as one 32-byte record (the op_dat layout), so
-// stores are coalesced.
+// so lane l holds cell l/16 of the group, node l%16, variables r = 0..3: the
+// metric terms are then lane-local, the LIFT product (A = Roe fluxes,
+// B = -LIFT^T) lands in the same registers, and the epilogue stores each
+// lane's 4 variables of one node as one 32-byte record (the op_dat layout),
+// so stores are coalesced.
 //
 // The Euler fluxes (one lane per (cell, node)) and the Roe fluxes (one lane
 // per (cell, face node), neighbour traces gathered from the neighbour's
 // state record through the face map) run on the vector ALUs; 16 cells per
-// tile make the Roe phase exactly three full passes of the wave at N=3.
+// tile make each of the two phases exactly three full passes of the wave at
+// N=3.  A tile runs: Roe fluxes -> LIFT products (the accumulators start
+// from the surface term) -> Euler fluxes -> volume products -> epilogue.
+// Within each flux phase a lane issues the loads of all its items before it
+// computes any of them (the Roe phase in two rounds: face-map code, geometry
+// and own trace, then the neighbour traces), so a tile waits on memory three
+// times instead of once per item and round.  Running the surface term first
+// keeps the accumulators out of the Roe phase, the phase with the most live
+// values, which is what lets the loads of all items be in flight within the
+// register budget of three waves per SIMD.
+//
+// Compact geometry (CG): on straight-sided cells the metric factors are the
+// same at every node, so the kernel reads one (rx, ry, sx, sy) record per
+// cell, forms the contravariant fluxes rx F + ry G and sx F + sy G at each
+// node, and the volume term is Drw (rx F + ry G) + Dsw (sx F + sy G): two
+// products per group instead of four, one accumulator, and 32 bytes of
+// geometry per cell instead of the 608 of the per-node factors, normals and
+// fscale (which it derives per face, cell_face_geometry).
 #include <hip/hip_runtime.h>
 
 #include "dg_device.h"
@@ -47,7 +63,7 @@ constexpr int mfma_lds_doubles() {
 // own LDS tiles.  The walk over blocks of tiles is uniform across the
 // workgroup, so every wave reaches every barrier; waves whose tile lies past
 // the end only skip the work.
-template <int N, bool GATHER, int EPI>
+template <int N, bool GATHER, int EPI, bool CG>
 __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const double *__restrict__ ops,
                                                        const int *__restrict__ fmask, Phys ph0,
                                                        unsigned long long facenodes, int walk) {
@@ -55,10 +71,12 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   constexpr int KP = (Np + 3) / 4;  // K-steps over source nodes
   constexpr int KF = (NF + 3) / 4;  // K-steps over face nodes
   constexpr int TE = kMfmaTile, G = 4;
+  constexpr int NIT = (TE * 4 * KP + 63) / 64;  // Euler-flux items per lane
+  constexpr int NRT = (TE * NF + 63) / 64;      // Roe-flux items per lane
   static_assert(Np <= 16, "one 16-row output tile per cell group");
   static_assert(KF <= 2 * KP, "face tile must fit in the volume tiles it reuses");
   // per wave: [0]: F tiles, [1]: G tiles, each [G][KP][64]; the face-flux
-  // tiles [G][KF][64] reuse the same memory after the volume products.
+  // tiles [G][KF][64] use the same memory before the Euler fluxes.
   extern __shared__ double lds_dyn[];
   __shared__ int sFm[NF];
 
@@ -95,34 +113,54 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   bool bad = false;
   const double n1sq = (double)((N + 1) * (N + 1));
   double *sF = sA, *sG = sA + G * KP * 64, *sL = sA;
+  const int ce = lane >> 4;  // cell within a group owned by this lane's outputs
 
   const int ntiles = multi ? a.tile_off[a.ncases] : (a.ncells + TE - 1) / TE;
   const int nblk = (ntiles + wpb - 1) / wpb;  // blocks of wpb tiles
-  for (int it_tile = 0;; it_tile++) {
-    const int blk = walk == 0 ? xcd_tile(it_tile, nblk) : (int)blockIdx.x + it_tile * (int)gridDim.x;
-    if (blk >= nblk) break;  // uniform across the workgroup
+  // The it_t-th tile of this wave: 0 = past the walk (uniform across the
+  // workgroup), 1 = a tile of cells [e0, e0 + ne) of case cs, 2 = past the
+  // last tile (no work, but every barrier is reached).
+  auto locate = [&](int it_t, int &e0_, int &ne_, int &cs_) -> int {
+    const int blk = walk == 0 ? xcd_tile(it_t, nblk) : (int)blockIdx.x + it_t * (int)gridDim.x;
+    e0_ = 0; ne_ = 0; cs_ = 0;
+    if (blk >= nblk) return 0;
     const int tile = blk * wpb + wave;
-    const bool active = tile < ntiles;
-    int e0 = 0, ne = 0, cs = 0;
-    Phys ph = ph0;
-    double dt = 0.0;
-    if (!active) {
-      // past the last tile: no work, but every barrier below is reached
-    } else if (multi) {
+    if (tile >= ntiles) return 2;
+    if (multi) {
       int lo = 0, hi = a.ncases - 1;  // case owning this tile
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (a.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
       }
-      cs = lo;
-      e0 = a.cell_off[cs] + (tile - a.tile_off[cs]) * TE;
-      ne = min(TE, a.cell_off[cs + 1] - e0);
+      cs_ = lo;
+      e0_ = a.cell_off[cs_] + (tile - a.tile_off[cs_]) * TE;
+      ne_ = min(TE, a.cell_off[cs_ + 1] - e0_);
+    } else {
+      e0_ = tile * TE;
+      ne_ = min(TE, a.ncells - e0_);
+    }
+    return 1;
+  };
+  // Face-map codes of a tile's Roe items (lane + 64 j -> (cell, face node)).
+  auto load_codes = [&](int e0_, int ne_, int *c) {
+#pragma unroll
+    for (int j = 0; j < NRT; j++) {
+      const int it = lane + 64 * j;
+      c[j] = (GATHER && it < ne_ * NF) ? a.fmap[(size_t)(e0_ + it / NF) * 3 + (it % NF) / Nfp] : 0;
+    }
+  };
+  int e0, ne, cs;
+  int st = locate(0, e0, ne, cs);
+  int code[NRT];
+  load_codes(e0, ne, code);
+  for (int it_tile = 0; st != 0; it_tile++) {
+    const bool active = st == 1;
+    Phys ph = ph0;
+    double dt = 0.0;
+    if (active && multi) {
       ph = a.phys[cs];
       if (EPI != EPI_RHS)
         dt = a.dt_fixed ? a.dt_fixed[cs] : a.cfl / __longlong_as_double((long long)a.dt1_cur[cs]);
-    } else {
-      e0 = tile * TE;
-      ne = min(TE, a.ncells - e0);
     }
     if (EPI == EPI_STAGE2 && active && cs != cfl_case) {
       // flush the running CFL max of the previous case (wave-uniform branch)
@@ -134,39 +172,133 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       cfl_case = cs;
       cfl_bits = 0ull;
     }
-    __syncthreads();
+    __syncthreads();  // the previous tile's volume products are done with the LDS tiles
 
-    // Euler fluxes per (cell, node), written in the A-operand layout.  The
-    // padding columns k >= Np are written with zeros on every tile: the face
-    // phase of the previous tile left its values there, and a zero B entry
-    // does not cancel a non-finite one (a failed case must not reach the
-    // next tile, which may belong to another case).
-    for (int it = lane; it < ne * 4 * KP; it += 64) {
-      const int e = it / (4 * KP), k = it - e * (4 * KP);
-      double F[4] = {0.0, 0.0, 0.0, 0.0}, Gv[4] = {0.0, 0.0, 0.0, 0.0};
-      if (k < Np) {
-        const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)(e0 + e) * Np + k) * 4);
-        const double2 q01 = src[0], q23 = src[1];
-        double qv[4] = {q01.x, q01.y, q23.x, q23.y};
-        euler_flux(ph.gam, qv, F, Gv);
-      }
-      // A row = cell + 4*var, A column = source node k
-      const int off = ((e >> 2) * KP + (k >> 2)) * 64 + 16 * (k & 3) + (e & 3);
+    // ---- Roe fluxes per (cell, face node), written in the A-operand layout
+    {
+      const int nrt = ne * NF;
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        sF[off + 4 * v] = F[v];
-        sG[off + 4 * v] = Gv[v];
+      for (int j = 0; j < NRT; j++) {
+        const int it = lane + 64 * j;
+        if (it >= nrt) continue;
+        const int e = it / NF, m = it - e * NF;
+        const int i = m - (m / Nfp) * Nfp;
+        const size_t cell = (size_t)(e0 + e);
+        const size_t gm = cell * NF + m;
+        double nxv, nyv, fs;
+        if constexpr (CG) {
+          const double2 *g = reinterpret_cast<const double2 *>(a.cmet + cell * 4);
+          const double2 rxy = g[0], sxy = g[1];
+          cell_face_geometry(rxy.x, rxy.y, sxy.x, sxy.y, m / Nfp, nxv, nyv, fs);
+        } else {
+          nxv = a.nx[gm]; nyv = a.ny[gm]; fs = a.fscale[gm];
+        }
+        const double2 *ms = reinterpret_cast<const double2 *>(a.w + (cell * Np + sFm[m]) * 4);
+        const double2 m01 = ms[0], m23 = ms[1];
+        double mQ[4] = {m01.x, m01.y, m23.x, m23.y}, pQ[4];
+        if (!GATHER) {
+          double2 *src = reinterpret_cast<double2 *>(a.ext + gm * 4);
+          const double2 p01 = src[0], p23 = src[1];
+          pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
+          src[0] = make_double2(0.0, 0.0);  // OP_RW: euler_rhs zeroes exteriorQ
+          src[1] = make_double2(0.0, 0.0);
+        } else if (code[j] >= 0) {
+          const int nb = code[j] >> 2, nf = code[j] & 3;
+          const int node = sFm[nf * Nfp + (Nfp - 1 - i)];
+          const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)nb * Np + node) * 4);
+          const double2 p01 = src[0], p23 = src[1];
+          pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
+        } else {
+          boundary_state(ph, -code[j] - 1, mQ, nxv, nyv, pQ);
+        }
+        double fl[4];
+        roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
+        const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
+#pragma unroll
+        for (int v = 0; v < 4; v++) sL[off + 4 * v] = fl[v];
+      }
+      if (NF < 4 * KF) {  // zero padding columns (orders whose 3(N+1) is not a multiple of 4)
+        constexpr int PADF = 4 * KF - NF;
+        for (int it = lane; it < ne * PADF; it += 64) {
+          const int e = it / PADF, m = NF + (it - e * PADF);
+          const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
+#pragma unroll
+          for (int v = 0; v < 4; v++) sL[off + 4 * v] = 0.0;
+        }
       }
     }
     __syncthreads();
 
-    // volume products and metric terms, one 4-cell group at a time
-    const int ce = lane >> 4;  // cell within the group owned by this lane's outputs
+    // ---- LIFT products: the accumulators start from the surface term
     dbl4 acc[G];
 #pragma unroll
     for (int g = 0; g < G; g++) {
       acc[g] = dbl4{0.0, 0.0, 0.0, 0.0};
-      if (4 * g < ne) {
+      if (4 * g >= ne) continue;
+#pragma unroll
+      for (int ks = 0; ks < KF; ks++) acc[g] = mfma16x16x4(sL[(g * KF + ks) * 64 + lane], bL[ks], acc[g]);
+    }
+    __syncthreads();  // the Euler fluxes overwrite the face-flux tiles
+
+    // ---- Euler fluxes per (cell, node), written in the A-operand layout.
+    // The padding columns k >= Np are written with zeros on every tile: the
+    // face phase left its values there, and a zero B entry does not cancel a
+    // non-finite one (a failed case must not reach the next tile, which may
+    // belong to another case).
+    {
+      const int nit = ne * 4 * KP;
+#pragma unroll
+      for (int j = 0; j < NIT; j++) {
+        const int it = lane + 64 * j;
+        if (it >= nit) continue;
+        const int e = it / (4 * KP), k = it - e * (4 * KP);
+        double F[4] = {0.0, 0.0, 0.0, 0.0}, Gv[4] = {0.0, 0.0, 0.0, 0.0};
+        if (k < Np) {
+          const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)(e0 + e) * Np + k) * 4);
+          const double2 q01 = src[0], q23 = src[1];
+          double qv[4] = {q01.x, q01.y, q23.x, q23.y};
+          euler_flux(ph.gam, qv, F, Gv);
+          if constexpr (CG) {  // contravariant fluxes
+            const double2 *mt = reinterpret_cast<const double2 *>(a.cmet + (size_t)(e0 + e) * 4);
+            const double2 rxy = mt[0], sxy = mt[1];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+              const double f = F[v], g = Gv[v];
+              F[v] = rxy.x * f + rxy.y * g;
+              Gv[v] = sxy.x * f + sxy.y * g;
+            }
+          }
+        }
+        // A row = cell + 4*var, A column = source node k
+        const int off = ((e >> 2) * KP + (k >> 2)) * 64 + 16 * (k & 3) + (e & 3);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          sF[off + 4 * v] = F[v];
+          sG[off + 4 * v] = Gv[v];
+        }
+      }
+    }
+    __syncthreads();
+
+    // the next tile's face-map codes load while this tile's volume products run
+    int e0n, nen, csn;
+    const int stn = locate(it_tile + 1, e0n, nen, csn);
+    int ncode[NRT];
+    load_codes(e0n, nen, ncode);
+
+    // ---- volume products, metric terms and the epilogue, one 4-cell group at a time
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (4 * g >= ne) continue;
+      dbl4 c = acc[g];
+      const int e = 4 * g + ce;
+      if constexpr (CG) {
+#pragma unroll
+        for (int ks = 0; ks < KP; ks++) {
+          c = mfma16x16x4(sF[(g * KP + ks) * 64 + lane], bR[ks], c);
+          c = mfma16x16x4(sG[(g * KP + ks) * 64 + lane], bS[ks], c);
+        }
+      } else {
         dbl4 d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0}, d3 = {0, 0, 0, 0}, d4 = {0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < KP; ks++) {
@@ -177,72 +309,14 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
           d3 = mfma16x16x4(aG, bR[ks], d3);
           d4 = mfma16x16x4(aG, bS[ks], d4);
         }
-        const int e = 4 * g + ce;
         double rxv = 0.0, sxv = 0.0, ryv = 0.0, syv = 0.0;
         if (e < ne && jj < Np) {
           const size_t gn = (size_t)(e0 + e) * Np + jj;
           rxv = a.rx[gn]; sxv = a.sx[gn]; ryv = a.ry[gn]; syv = a.sy[gn];
         }
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-          acc[g][r] = (rxv * d1[r] + sxv * d2[r]) + (ryv * d3[r] + syv * d4[r]);
+        for (int r = 0; r < 4; r++) c[r] += (rxv * d1[r] + sxv * d2[r]) + (ryv * d3[r] + syv * d4[r]);
       }
-    }
-    __syncthreads();  // the face fluxes overwrite the F tiles
-
-    // Roe fluxes per (cell, face node), written in the A-operand layout
-    for (int it = lane; it < ne * NF; it += 64) {
-      const int e = it / NF, m = it - e * NF;
-      const int f = m / Nfp, i = m - f * Nfp;
-      const size_t cell = (size_t)(e0 + e);
-      const size_t gm = cell * NF + m;
-      const double nxv = a.nx[gm], nyv = a.ny[gm], fs = a.fscale[gm];
-      const double2 *ms = reinterpret_cast<const double2 *>(a.w + (cell * Np + sFm[m]) * 4);
-      const double2 m01 = ms[0], m23 = ms[1];
-      double mQ[4] = {m01.x, m01.y, m23.x, m23.y}, pQ[4];
-      if (GATHER) {
-        const int code = a.fmap[cell * 3 + f];
-        if (code >= 0) {
-          const int nb = code >> 2, nf = code & 3;
-          const int node = sFm[nf * Nfp + (Nfp - 1 - i)];
-          const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)nb * Np + node) * 4);
-          const double2 p01 = src[0], p23 = src[1];
-          pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
-        } else {
-          boundary_state(ph, -code - 1, mQ, nxv, nyv, pQ);
-        }
-      } else {
-        double2 *src = reinterpret_cast<double2 *>(a.ext + gm * 4);
-        const double2 p01 = src[0], p23 = src[1];
-        pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
-        src[0] = make_double2(0.0, 0.0);  // OP_RW: euler_rhs zeroes exteriorQ
-        src[1] = make_double2(0.0, 0.0);
-      }
-      double fl[4];
-      roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
-      const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
-#pragma unroll
-      for (int v = 0; v < 4; v++) sL[off + 4 * v] = fl[v];
-    }
-    if (NF < 4 * KF) {  // zero padding columns (orders whose 3(N+1) is not a multiple of 4)
-      constexpr int PADF = 4 * KF - NF;
-      for (int it = lane; it < ne * PADF; it += 64) {
-        const int e = it / PADF, m = NF + (it - e * PADF);
-        const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
-#pragma unroll
-        for (int v = 0; v < 4; v++) sL[off + 4 * v] = 0.0;
-      }
-    }
-    __syncthreads();
-
-    // LIFT products onto the volume terms, then the epilogue
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-      if (4 * g >= ne) continue;
-      dbl4 c = acc[g];
-#pragma unroll
-      for (int ks = 0; ks < KF; ks++) c = mfma16x16x4(sL[(g * KF + ks) * 64 + lane], bL[ks], c);
-      const int e = 4 * g + ce;
       if (e >= ne || jj >= Np) continue;
       const size_t gn = (size_t)(e0 + e) * Np + jj;
       double o[4];
@@ -274,13 +348,27 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       dst[1] = make_double2(o[2], o[3]);
       if (EPI == EPI_STAGE2) {
         if ((facenodes >> jj) & 1ull) {
-          const double cv = cfl_node(ph.gam, n1sq, o, a.fscale[(size_t)(e0 + e) * NF + jj]) + 0.0;
+          // src/calc_dt.h:11 reads fscale at face-node index FMASK[i], i.e.
+          // at this node's index jj read as a face-node index (face jj / Nfp)
+          double fsj;
+          if constexpr (CG) {
+            const double2 *mt = reinterpret_cast<const double2 *>(a.cmet + (size_t)(e0 + e) * 4);
+            const double2 rxy = mt[0], sxy = mt[1];
+            double nxj, nyj;
+            cell_face_geometry(rxy.x, rxy.y, sxy.x, sxy.y, jj / Nfp, nxj, nyj, fsj);
+          } else {
+            fsj = a.fscale[(size_t)(e0 + e) * NF + jj];
+          }
+          const double cv = cfl_node(ph.gam, n1sq, o, fsj) + 0.0;
           cfl_bits = max_bits(cfl_bits, (unsigned long long)__double_as_longlong(cv));
         }
         bad |= !(isfinite(o[0]) && isfinite(o[1]) && isfinite(o[2]) && isfinite(o[3])) ||
                !(o[0] > 0.0);
       }
     }
+    e0 = e0n; ne = nen; cs = csn; st = stn;
+#pragma unroll
+    for (int j = 0; j < NRT; j++) code[j] = ncode[j];
   }
   if (EPI == EPI_STAGE2) {
     if (cfl_case >= 0) {
@@ -301,14 +389,14 @@ static int mfma_wpb(int want) {
   return w;
 }
 
-template <int N, bool GATHER, int EPI>
+template <int N, bool GATHER, int EPI, bool CG>
 static hipError_t launch_t(const Launch &L, const StageArgs &a) {
   static int per_cu[5] = {-1, -1, -1, -1, -1};  // resident workgroups per CU, by waves per workgroup
   const int wpb = mfma_wpb<N>(L.wpb);
   const size_t lds = (size_t)wpb * mfma_lds_doubles<N>() * sizeof(double);
   if (per_cu[wpb] < 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rhs_mfma_kernel<N, GATHER, EPI>, 64 * wpb, lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rhs_mfma_kernel<N, GATHER, EPI, CG>, 64 * wpb, lds) !=
             hipSuccess ||
         nb < 1)
       nb = 8 / wpb > 0 ? 8 / wpb : 1;
@@ -324,25 +412,31 @@ static hipError_t launch_t(const Launch &L, const StageArgs &a) {
   // latency, so the grid stays at the occupancy limit.
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) return hipSuccess;
-  hipLaunchKernelGGL((rhs_mfma_kernel<N, GATHER, EPI>), dim3(grid), dim3(64 * wpb), lds, L.stream, a, L.ops,
-                     L.fmask, L.ph, (unsigned long long)L.facenodes, L.walk);
+  hipLaunchKernelGGL((rhs_mfma_kernel<N, GATHER, EPI, CG>), dim3(grid), dim3(64 * wpb), lds, L.stream, a,
+                     L.ops, L.fmask, L.ph, (unsigned long long)L.facenodes, L.walk);
   return hipGetLastError();
 }
 
-template <int N>
-static hipError_t launch_n(const Launch &L, const StageArgs &a, bool gather, int epi) {
-  if (!gather) return launch_t<N, false, EPI_RHS>(L, a);
+template <int N, bool CG>
+static hipError_t launch_g(const Launch &L, const StageArgs &a, int epi) {
   switch (epi) {
-    case EPI_RHS: return launch_t<N, true, EPI_RHS>(L, a);
-    case EPI_STAGE0: return launch_t<N, true, EPI_STAGE0>(L, a);
-    case EPI_STAGE1: return launch_t<N, true, EPI_STAGE1>(L, a);
-    case EPI_STAGE2: return launch_t<N, true, EPI_STAGE2>(L, a);
+    case EPI_RHS: return launch_t<N, true, EPI_RHS, CG>(L, a);
+    case EPI_STAGE0: return launch_t<N, true, EPI_STAGE0, CG>(L, a);
+    case EPI_STAGE1: return launch_t<N, true, EPI_STAGE1, CG>(L, a);
+    case EPI_STAGE2: return launch_t<N, true, EPI_STAGE2, CG>(L, a);
   }
   return hipErrorInvalidValue;
 }
 
+template <int N>
+static hipError_t launch_n(const Launch &L, const StageArgs &a, bool gather, int epi) {
+  if (!gather) return launch_t<N, false, EPI_RHS, false>(L, a);
+  return a.cmet ? launch_g<N, true>(L, a, epi) : launch_g<N, false>(L, a, epi);
+}
+
 hipError_t launch_rhs_mfma(const Launch &L, const StageArgs &a, bool gather, int epi) {
   if (!gather && epi != EPI_RHS) return hipErrorInvalidValue;
+  if (a.cmet && !gather) return hipErrorInvalidValue;
   switch (L.N) {
     case 1: return launch_n<1>(L, a, gather, epi);
     case 2: return launch_n<2>(L, a, gather, epi);

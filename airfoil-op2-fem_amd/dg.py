@@ -15,6 +15,7 @@ from ._lib import DGError, P, c_double, c_int, c_void_p, check, lib
 
 BC_INFLOW, BC_OUTFLOW, BC_WALL, BC_FARFIELD = 0, 1, 2, 3
 MODE_FUSED, MODE_REFERENCE = 0, 1
+GEOM_NODE, GEOM_CELL = 0, 1
 
 
 def _np_ptr(a):
@@ -415,6 +416,13 @@ class Solver:
     def set_launch(self, wpb=1, walk=0, max_blocks=0):
         """Launch shape of the matrix-core stage kernel (results unchanged)."""
         self._chk(lib.dg_solver_set_launch(self._h, int(wpb), int(walk), int(max_blocks)))
+
+    def set_geometry(self, mode):
+        """GEOM_NODE (default): the fused stage kernels read the per-node
+        factors, normals and fscale of init_grid, as the reference does;
+        GEOM_CELL: one metric record per cell (faster; leaves the reference
+        by init_grid's per-node rounding scatter, ~1e-12 relative)."""
+        self._chk(lib.dg_solver_set_geometry(self._h, int(mode)))
 
     def launch_config(self):
         w, k, b = c_int(), c_int(), c_int()

@@ -24,14 +24,16 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
-def cell_bytes(N, stage):
+def cell_bytes(N, stage, geometry="node"):
     """Algorithmic bytes per cell of one fused stage kernel (FP64, int32 map).
 
-    Reads: stage input state (Np*4), neighbour face traces (3*Nfp*4),
-    rx/ry/sx/sy (4*Np), nx/ny/fscale (3*3*Nfp), face map (3 ints); writes the
-    next state (Np*4).  Stages 1 and 2 also read the step's base state q."""
+    Reads: stage input state (Np*4), neighbour face traces (3*Nfp*4), the
+    geometry (per node: rx/ry/sx/sy (4*Np) and nx/ny/fscale (3*3*Nfp); per
+    cell: one (rx, ry, sx, sy) record), face map (3 ints); writes the next
+    state (Np*4).  Stages 1 and 2 also read the step's base state q."""
     Np, NF = (N + 1) * (N + 2) // 2, 3 * (N + 1)
-    b = 8 * (Np * 4 + NF * 4 + 4 * Np + 3 * NF + Np * 4) + 4 * 3
+    geo = 4 if geometry == "cell" else 4 * Np + 3 * NF
+    b = 8 * (Np * 4 + NF * 4 + geo + Np * 4) + 4 * 3
     if stage > 0:
         b += 8 * Np * 4
     return b
@@ -352,6 +354,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--geometry", choices=["node", "cell"], default="node",
+                    help="geometry the stage kernels read: per-node factors (the reference's "
+                         "op_dats) or one metric record per cell")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run plain device copies of GIB GiB (counter calibration)")
     args = ap.parse_args()
@@ -385,6 +390,7 @@ def main():
     ctx = dg.Context(N)
     ctx.set_freestream(gam, mach, 0.0, 1.0, 1.0)
     solver = dg.Solver(ctx, mesh)
+    solver.set_geometry(dg.GEOM_CELL if args.geometry == "cell" else dg.GEOM_NODE)
     solver.set_vortex(uinf, 5.0, 5.0, 0.0, 10.0)
     K, Np = mesh.ncells, ctx.Np
     dof = K * Np * 4
@@ -420,18 +426,18 @@ def main():
     kt = {k: solver.kernel_time(k) for k in ("rhs_stage0", "rhs_stage1", "rhs_stage2")}
     solver.profile(False)
     for k, (ms, cnt) in kt.items():
-        gbs = cell_bytes(N, int(k[-1])) * K / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        gbs = cell_bytes(N, int(k[-1]), args.geometry) * K / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         print(f"[bench] {k}: {ms * 1e3:.1f} us avg over {cnt} launches, {gbs:.0f} GB/s algorithmic",
               file=sys.stderr, flush=True)
     if not any(ms > 0 for ms, _ in kt.values()):
         kt = {"rhs_stage1": (elapsed * 1e3 / max(args.steps, 1) / 3.0, 0)}
     dom = max(kt, key=lambda k: kt[k][0])
     dom_ms = kt[dom][0]
-    achieved = cell_bytes(N, int(dom[-1])) * K / (dom_ms * 1e-3) / 1e9
-    traffic = load_traffic(dom)
+    achieved = cell_bytes(N, int(dom[-1]), args.geometry) * K / (dom_ms * 1e-3) / 1e9
+    traffic = load_traffic(dom) if args.geometry == "node" else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes_per_launch": cell_bytes(N, int(dom[-1])) * K,
+                "algorithmic_bytes_per_launch": cell_bytes(N, int(dom[-1]), args.geometry) * K,
                 "avg_launch_ms": round(dom_ms, 4)}
 
     cpu = None
@@ -462,7 +468,8 @@ def main():
            "dtype": "f64", "data": "synthetic (seeded jittered periodic mesh, analytic vortex state)",
            "config": {"workload": f"isentropic vortex, {n}x{n} jittered periodic grid "
                                   f"({K} triangles), N={N}, SSPRK3 CFL {args.cfl}, fused stage kernels",
-                      "cells": K, "order": N, "dof": dof, "parallelism": "single GPU"},
+                      "cells": K, "order": N, "dof": dof, "geometry": args.geometry,
+                      "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "launch": tune}
     print(json.dumps(out))
 

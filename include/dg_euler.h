@@ -282,6 +282,24 @@ int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int 
  * persistent grid (0: occupancy-limited).  The shape never changes results. */
 int dg_solver_set_launch(dg_solver *s, int wpb, int walk, int max_blocks);
 int dg_solver_launch_config(const dg_solver *s, int *wpb, int *walk, int *max_blocks);
+/* Geometry the fused stage kernels read (no reference counterpart).
+ * DG_GEOM_NODE (default): the per-node rx, ry, sx, sy and per face-node nx,
+ * ny, fscale of src/init_grid.h, as src/euler_rhs.h and the per-loop entries
+ * take them (608 bytes per cell at N=3).  DG_GEOM_CELL: one (rx, ry, sx, sy)
+ * record per cell (the factors at node 0), with each face's normal and
+ * fscale formed from it and the volume term taken as
+ * Drw (rx F + ry G) + Dsw (sx F + sy G): 32 bytes of geometry per cell.
+ * Straight-sided cells have the same factors at every node in exact
+ * arithmetic, but init_grid's per-node values (Dr and Ds applied to node
+ * coordinates) scatter by about 1e-15 x (coordinate size / cell size)
+ * relative, 1e-12 and more on fine or thin cells, and DG_GEOM_CELL does not
+ * carry that scatter: its results leave the reference's by that much (about
+ * 2e-12 relative in rho v after 100 steps of the 64x64 vortex).  The
+ * geometry is fixed when the solver is created; the vector-ALU stage kernel
+ * (context variant 1) always reads the per-node factors. */
+#define DG_GEOM_NODE 0
+#define DG_GEOM_CELL 1
+int dg_solver_set_geometry(dg_solver *s, int mode);
 /* Autotuner: times trial_steps fused steps (CFL cfl) of the solver's own
  * cases for each of 18 shapes (wpb 1/2/4 x walk 0/1 x 4/8/12 resident waves
  * per CU), restores the state, time and dt it started from and keeps the

@@ -205,6 +205,27 @@ class Oracle:
                     bedge_type=np.ascontiguousarray(bedge_type, dtype=np.int32).ravel())
         return case
 
+    @staticmethod
+    def use_cell_geometry(case):
+        """Replaces a case's per-node geometry, in place, by the per-cell form
+        of the fused solver's DG_GEOM_CELL mode: rx, ry, sx, sy at every node
+        = their node-0 values, and on face f the normal and fscale of the
+        vectors (-sx, -sy), (rx + sx, ry + sy), (-rx, -ry), which are
+        src/init_grid.h:80-104's face vectors divided by J."""
+        g = case["geo"]
+        met = [g[k][:, 0].copy() for k in ("rx", "ry", "sx", "sy")]
+        for k, v in zip(("rx", "ry", "sx", "sy"), met):
+            g[k][:] = v[:, None]
+        rx, ry, sx, sy = met
+        nfp = g["nx"].shape[1] // 3
+        for f, (a, b) in enumerate([(-sx, -sy), (rx + sx, ry + sy), (-rx, -ry)]):
+            L = np.sqrt(a * a + b * b)
+            sl = slice(f * nfp, (f + 1) * nfp)
+            g["fscale"][:, sl] = L[:, None]
+            g["nx"][:, sl] = (a / L)[:, None]
+            g["ny"][:, sl] = (b / L)[:, None]
+        return case
+
     def _mesh_struct(self, case):
         g = case["geo"]
         m = MeshS()

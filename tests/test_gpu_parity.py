@@ -169,19 +169,42 @@ def vortex64(dg, orc):
     return dict(mesh=mesh, q0=q0, q=st["q"].copy(), t=t, dt=dtn, case=case, o=o)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_vortex_100_steps_match_oracle(dg, vortex64, mode):
+@pytest.fixture(scope="module")
+def vortex64_cell(dg, orc, vortex64):
+    """The same 100 steps in the oracle with the per-cell geometry of
+    GEOM_CELL (Oracle.use_cell_geometry)."""
+    import os
+    o = vortex64["o"]
+    mesh = vortex64["mesh"]
+    case = o.use_cell_geometry(o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum,
+                                           mesh.bedge2cells, mesh.bedgeNum, mesh.bedge_type))
+    st = o.new_state(case, vortex64["q0"])
+    t, dtn = o.run(case, st, 100, cfl=0.5, nthreads=min(16, os.cpu_count() or 1))
+    return dict(q=st["q"].copy(), t=t, dt=dtn)
+
+
+@pytest.mark.parametrize("mode,geom", [(0, "node"), (0, "cell"), (1, "node")])
+def test_vortex_100_steps_match_oracle(dg, vortex64, vortex64_cell, mode, geom):
+    """Fused stages on the per-node (the reference's) or the per-cell
+    geometry, and the nine-loop reference sequence (mode 1), each against the
+    oracle on the same geometry."""
     mesh, ctx, s = _vortex_solver(dg, 64)
+    s.set_geometry(dg.GEOM_CELL if geom == "cell" else dg.GEOM_NODE)
+    ref = vortex64_cell if geom == "cell" else vortex64
     assert np.array_equal(s.get_q(), vortex64["q0"]) or rel(s.get_q(), vortex64["q0"]) < 1e-15
     s.set_q(vortex64["q0"])  # identical start
     s.run(100, cfl=0.5, mode=mode)
     assert s.check() == 0
     q = s.get_q()
-    errs = rel_per_var(q, vortex64["q"])
+    errs = rel_per_var(q, ref["q"])
     assert max(errs) <= 1e-12, errs
     t, dt = s.time()
-    assert abs(t - vortex64["t"]) <= 1e-12 * vortex64["t"]
-    assert abs(dt - vortex64["dt"]) <= 1e-11 * vortex64["dt"]
+    assert abs(t - ref["t"]) <= 1e-12 * ref["t"]
+    assert abs(dt - ref["dt"]) <= 1e-11 * ref["dt"]
+    if geom == "cell":
+        # the per-cell geometry leaves the reference's per-node result by
+        # init_grid's rounding scatter only
+        assert max(rel_per_var(q, vortex64["q"])) <= 1e-11
 
 
 def test_fused_run_is_deterministic_and_resumable(dg, vortex64):
@@ -221,21 +244,30 @@ def test_vortex_error_matches_oracle_and_converges(dg, orc):
     assert np.all(ratio > 8.0), ratio  # observed order > 3 at N=3
 
 
-def test_airfoil_ogrid_short_run_matches_oracle(dg, orc):
-    """O-grid with wall and characteristic far field, alpha = 2 deg, M = 0.4."""
+@pytest.mark.parametrize("geom", ["node", "cell"])
+def test_airfoil_ogrid_short_run_matches_oracle(dg, orc, geom):
+    """O-grid with wall and characteristic far field, alpha = 2 deg, M = 0.4,
+    per-node (the reference's) and per-cell geometry in the stage kernels,
+    each against the oracle on the same geometry; forces and C_p on the
+    per-node geometry in both."""
     import os
     mesh = dg.naca0012_ogrid(48, 16, stretch=1.15, rfar=20.0)
     ctx = dg.Context(3)
     ctx.set_freestream(GAM, 0.4, 2.0, 1.0, 1.0)
     s = dg.Solver(ctx, mesh)
+    s.set_geometry(dg.GEOM_CELL if geom == "cell" else dg.GEOM_NODE)
     q0 = s.get_q()
     s.run(50, cfl=1.0)
     assert s.check() == 0
     o = orc.Oracle(dg.generate_operators(3), orc.freestream(GAM, 0.4, 2.0))
-    case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
-                       mesh.bedgeNum, mesh.bedge_type)
-    st = o.new_state(case, q0)
-    o.run(case, st, 50, cfl=1.0, nthreads=min(16, os.cpu_count() or 1))
+
+    def mk():
+        return o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
+                           mesh.bedgeNum, mesh.bedge_type)
+    case = mk()
+    run_case = o.use_cell_geometry(mk()) if geom == "cell" else case
+    st = o.new_state(run_case, q0)
+    o.run(run_case, st, 50, cfl=1.0, nthreads=min(16, os.cpu_count() or 1))
     qg = s.get_q()
     assert max(rel_per_var(qg, st["q"])) <= 1e-12
     # surface force: GPU quadrature vs the oracle's on the same state, and

@@ -329,3 +329,41 @@ def test_vertex_fields_last_cell_wins():
         assert vx[v] == mx / r and vy[v] == my / r
         assert p[v] == (gam - 1) * (e - 0.5 * (mx * (mx / r) + my * (my / r)))
     assert rho[4] == 0.0 and p[4] == 0.0
+
+
+@pytest.mark.parametrize("kind", ["jittered", "ogrid"])
+def test_per_cell_geometry_reproduces_init_grid(kind):
+    """The fused stage kernels' per-cell geometry: straight-sided cells have
+    the same metric factors at every node, and each face's unit normal and
+    fscale follow from node 0's (rx, ry, sx, sy) as the normalised vectors
+    (-sx, -sy), (rx + sx, ry + sy), (-rx, -ry) and their lengths
+    (src/init_grid.h:80-104 with rx = ys/J, ry = -xs/J, sx = -yr/J,
+    sy = xr/J).  Checked against the oracle's per-node init_grid on a
+    jittered grid and on thin, stretched O-grid cells, cell by cell within
+    the rounding scatter of init_grid's own per-node values (Dr and Ds applied
+    to node coordinates that are large against the cell: up to a few 1e-12
+    relative on the thin cells); a wrong face, sign or component is O(1)."""
+    n = 3
+    o = oracle.Oracle(build_operators(n))
+    if kind == "jittered":
+        _, cell_xy = oracle.periodic_grid_np(12, 12, seed=3, jitter=0.2)
+    else:
+        import importlib
+        dg = importlib.import_module("airfoil-op2-fem_amd")
+        cell_xy = dg.naca0012_ogrid(64, 24, 1.3, 20.0).cell_xy
+    g = o.init_grid_all(cell_xy)
+    keys = ("rx", "ry", "sx", "sy")
+    met = np.stack([g[k][:, 0] for k in keys], axis=1)
+    scale = np.abs(met).max(axis=1)
+    scatter = np.max([np.abs(g[k] - met[:, i:i + 1]).max(axis=1) for i, k in enumerate(keys)], axis=0) / scale
+    assert scatter.max() < 1e-10
+    tol = 8 * scatter + 1e-14
+    rx, ry, sx, sy = met.T
+    vec = [(-sx, -sy), (rx + sx, ry + sy), (-rx, -ry)]
+    for f, (a, b) in enumerate(vec):
+        L = np.hypot(a, b)
+        for i in range(n + 1):
+            mi = f * (n + 1) + i
+            assert np.all(np.abs(g["fscale"][:, mi] / L - 1) <= tol)
+            assert np.all(np.abs(g["nx"][:, mi] - a / L) <= tol)
+            assert np.all(np.abs(g["ny"][:, mi] - b / L) <= tol)

@@ -46,6 +46,13 @@ __device__ __forceinline__ double sqrt_nr(double x) {
   return __builtin_fma(d, h, g);
 }
 
+// sqrt of a value that may also be 0 (a speed at rest) or not finite: the
+// refined form for positive values, the IEEE one otherwise (sqrt(0) = 0,
+// NaN stays NaN).
+__device__ __forceinline__ double sqrt_nn(double x) {
+  return x > 0.0 && x < __builtin_huge_val() ? sqrt_nr(x) : sqrt(x);
+}
+
 // Outward unit normal and fscale of face f of a straight-sided cell from its
 // metric factors (rx, ry, sx, sy).  src/init_grid.h:80-104 forms the face
 // vectors (yr, -xr), (ys - yr, xr - xs), (-ys, xs) and divides by their
@@ -194,8 +201,8 @@ __device__ __forceinline__ double cfl_node(double gam, double n1sq, const double
   double u = q[1] * ir;
   double v = q[2] * ir;
   double p = (gam - 1.0) * (q[3] - rho * (u * u + v * v) * 0.5);
-  double c = sqrt(fabs(gam * p * ir));
-  return n1sq * 0.5 * fs * (sqrt(u * u + v * v) + c);
+  double c = sqrt_nn(fabs(gam * p * ir));
+  return n1sq * 0.5 * fs * (sqrt_nn(u * u + v * v) + c);
 }
 
 __device__ __forceinline__ unsigned long long max_bits(unsigned long long a, unsigned long long b) {

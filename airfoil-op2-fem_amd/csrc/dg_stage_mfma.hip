@@ -193,22 +193,25 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         } else {
           nxv = a.nx[gm]; nyv = a.ny[gm]; fs = a.fscale[gm];
         }
-        const double2 *ms = reinterpret_cast<const double2 *>(a.w + (cell * Np + sFm[m]) * 4);
+        // own and exterior trace loads issued together, before any branch:
+        // a boundary slot reads its own record as a stand-in exterior, which
+        // boundary_state then replaces
+        const size_t own = cell * Np + sFm[m];
+        const bool interior = GATHER && code[j] >= 0;
+        const size_t nrec =
+            interior ? (size_t)(code[j] >> 2) * Np + sFm[(code[j] & 3) * Nfp + (Nfp - 1 - i)] : own;
+        const double2 *ms = reinterpret_cast<const double2 *>(a.w + own * 4);
+        const double2 *ps = GATHER ? reinterpret_cast<const double2 *>(a.w + nrec * 4)
+                                   : reinterpret_cast<const double2 *>(a.ext + gm * 4);
         const double2 m01 = ms[0], m23 = ms[1];
-        double mQ[4] = {m01.x, m01.y, m23.x, m23.y}, pQ[4];
+        const double2 p01 = ps[0], p23 = ps[1];
+        double mQ[4] = {m01.x, m01.y, m23.x, m23.y};
+        double pQ[4] = {p01.x, p01.y, p23.x, p23.y};
         if (!GATHER) {
-          double2 *src = reinterpret_cast<double2 *>(a.ext + gm * 4);
-          const double2 p01 = src[0], p23 = src[1];
-          pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
-          src[0] = make_double2(0.0, 0.0);  // OP_RW: euler_rhs zeroes exteriorQ
-          src[1] = make_double2(0.0, 0.0);
-        } else if (code[j] >= 0) {
-          const int nb = code[j] >> 2, nf = code[j] & 3;
-          const int node = sFm[nf * Nfp + (Nfp - 1 - i)];
-          const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)nb * Np + node) * 4);
-          const double2 p01 = src[0], p23 = src[1];
-          pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
-        } else {
+          double2 *z = reinterpret_cast<double2 *>(a.ext + gm * 4);
+          z[0] = make_double2(0.0, 0.0);  // OP_RW: euler_rhs zeroes exteriorQ
+          z[1] = make_double2(0.0, 0.0);
+        } else if (!interior) {
           boundary_state(ph, -code[j] - 1, mQ, nxv, nyv, pQ);
         }
         double fl[4];
@@ -290,8 +293,31 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
 #pragma unroll
     for (int g = 0; g < G; g++) {
       if (4 * g >= ne) continue;
-      dbl4 c = acc[g];
       const int e = 4 * g + ce;
+      const bool ok = e < ne && jj < Np;
+      const size_t gn = (size_t)(e0 + (ok ? e : 0)) * Np + (ok ? jj : 0);
+      // This group's first-touch global loads (metric factors, base state
+      // q) go out before its products, so their latency overlaps the matrix
+      // work; the stage input w, which the Euler-flux phase has just brought
+      // into the cache, follows the products when q is needed too (fewer
+      // registers live across them).
+      constexpr bool early_w = !(EPI == EPI_STAGE1 || EPI == EPI_STAGE2);
+      double rxv = 0.0, sxv = 0.0, ryv = 0.0, syv = 0.0;
+      double2 w01 = make_double2(0.0, 0.0), w23 = w01, q01 = w01, q23 = w01;
+      if (ok) {
+        if constexpr (!CG) {
+          rxv = a.rx[gn]; sxv = a.sx[gn]; ryv = a.ry[gn]; syv = a.sy[gn];
+        }
+        if (EPI != EPI_RHS && early_w) {
+          const double2 *ws = reinterpret_cast<const double2 *>(a.w + gn * 4);
+          w01 = ws[0]; w23 = ws[1];
+        }
+        if (EPI == EPI_STAGE1 || EPI == EPI_STAGE2) {
+          const double2 *qs = reinterpret_cast<const double2 *>(a.q + gn * 4);
+          q01 = qs[0]; q23 = qs[1];
+        }
+      }
+      dbl4 c = acc[g];
       if constexpr (CG) {
 #pragma unroll
         for (int ks = 0; ks < KP; ks++) {
@@ -299,40 +325,43 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
           c = mfma16x16x4(sG[(g * KP + ks) * 64 + lane], bS[ks], c);
         }
       } else {
-        dbl4 d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0}, d3 = {0, 0, 0, 0}, d4 = {0, 0, 0, 0};
+        // F's two products with rx, sx, then G's with ry, sy: two live
+        // accumulators instead of four
+        dbl4 d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < KP; ks++) {
           const double aF = sF[(g * KP + ks) * 64 + lane];
-          const double aG = sG[(g * KP + ks) * 64 + lane];
           d1 = mfma16x16x4(aF, bR[ks], d1);
           d2 = mfma16x16x4(aF, bS[ks], d2);
-          d3 = mfma16x16x4(aG, bR[ks], d3);
-          d4 = mfma16x16x4(aG, bS[ks], d4);
-        }
-        double rxv = 0.0, sxv = 0.0, ryv = 0.0, syv = 0.0;
-        if (e < ne && jj < Np) {
-          const size_t gn = (size_t)(e0 + e) * Np + jj;
-          rxv = a.rx[gn]; sxv = a.sx[gn]; ryv = a.ry[gn]; syv = a.sy[gn];
         }
 #pragma unroll
-        for (int r = 0; r < 4; r++) c[r] += (rxv * d1[r] + sxv * d2[r]) + (ryv * d3[r] + syv * d4[r]);
+        for (int r = 0; r < 4; r++) c[r] += rxv * d1[r] + sxv * d2[r];
+        d1 = dbl4{0, 0, 0, 0};
+        d2 = dbl4{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KP; ks++) {
+          const double aG = sG[(g * KP + ks) * 64 + lane];
+          d1 = mfma16x16x4(aG, bR[ks], d1);
+          d2 = mfma16x16x4(aG, bS[ks], d2);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) c[r] += ryv * d1[r] + syv * d2[r];
       }
-      if (e >= ne || jj >= Np) continue;
-      const size_t gn = (size_t)(e0 + e) * Np + jj;
+      if (!ok) continue;
+      if (EPI != EPI_RHS && !early_w) {
+        const double2 *ws = reinterpret_cast<const double2 *>(a.w + gn * 4);
+        w01 = ws[0]; w23 = ws[1];
+      }
       double o[4];
       if (EPI == EPI_RHS) {
 #pragma unroll
         for (int v = 0; v < 4; v++) o[v] = c[v];
       } else {
-        const double2 *ws = reinterpret_cast<const double2 *>(a.w + gn * 4);
-        const double2 w01 = ws[0], w23 = ws[1];
         const double wv[4] = {w01.x, w01.y, w23.x, w23.y};
         if (EPI == EPI_STAGE0) {
 #pragma unroll
           for (int v = 0; v < 4; v++) o[v] = wv[v] + dt * c[v];
         } else {
-          const double2 *qs = reinterpret_cast<const double2 *>(a.q + gn * 4);
-          const double2 q01 = qs[0], q23 = qs[1];
           const double qv[4] = {q01.x, q01.y, q23.x, q23.y};
           if (EPI == EPI_STAGE1) {
 #pragma unroll
@@ -350,6 +379,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         if ((facenodes >> jj) & 1ull) {
           // src/calc_dt.h:11 reads fscale at face-node index FMASK[i], i.e.
           // at this node's index jj read as a face-node index (face jj / Nfp)
+          // (the tile's Roe phase has just read these cache lines)
           double fsj;
           if constexpr (CG) {
             const double2 *mt = reinterpret_cast<const double2 *>(a.cmet + (size_t)(e0 + e) * 4);

@@ -10,8 +10,9 @@
 // columns: the Np source nodes padded to a multiple of 4) and B = Drw^T or
 // Dsw^T (padded to 16 output nodes).  B is the same for every cell, so each
 // lane keeps its B entries in registers for the whole kernel; A is staged in
-// LDS already in the operand layout, so every MFMA reads one contiguous
-// 512-byte row.  The FP64 accumulator layout is row-interleaved: lane l holds
+// LDS already in the operand layout (the padding columns are not stored: the
+// lanes that would read them supply zeros), so every MFMA reads one
+// contiguous 512-byte row.  The FP64 accumulator layout is row-interleaved: lane l holds
 // D[l/16 + 4*r][l%16], r = 0..3.  Rows are therefore ordered (cell + 4*var),
 // so lane l holds cell l/16 of the group, node l%16, variables r = 0..3: the
 // metric terms are then lane-local, the LIFT product (A = Roe fluxes,
@@ -21,17 +22,19 @@
 //
 // The Euler fluxes (one lane per (cell, node)) and the Roe fluxes (one lane
 // per (cell, face node), neighbour traces gathered from the neighbour's
-// state record through the face map) run on the vector ALUs; 16 cells per
-// tile make each of the two phases exactly three full passes of the wave at
-// N=3.  A tile runs: Roe fluxes -> LIFT products (the accumulators start
-// from the surface term) -> Euler fluxes -> volume products -> epilogue.
-// Within each flux phase a lane issues the loads of all its items before it
-// computes any of them (the Roe phase in two rounds: face-map code, geometry
-// and own trace, then the neighbour traces), so a tile waits on memory three
-// times instead of once per item and round.  Running the surface term first
-// keeps the accumulators out of the Roe phase, the phase with the most live
-// values, which is what lets the loads of all items be in flight within the
-// register budget of three waves per SIMD.
+// state record through the face map) run on the vector ALUs; at N=3 a
+// 16-cell tile is 192 Roe items (three full passes of the wave) and 160
+// Euler items.  A tile runs: Roe fluxes -> LIFT products (the accumulators
+// start from the surface term) -> Euler fluxes -> volume products ->
+// epilogue.  The kernel is bound by memory latency more than by bandwidth or
+// arithmetic, so the order of loads is chosen to shorten each tile's chain
+// of dependent memory round trips within the register budget of three waves
+// per SIMD: the next tile's face-map codes are loaded one tile ahead (during
+// the volume products), a Roe item's own trace, exterior trace and geometry
+// go out together (a boundary slot reads its own record as a stand-in
+// exterior), and each group's first-touch epilogue loads go out before its
+// products.  Running the surface term first keeps the accumulators out of
+// the Roe phase, the phase with the most live values.
 //
 // Compact geometry (CG): on straight-sided cells the metric factors are the
 // same at every node, so the kernel reads one (rx, ry, sx, sy) record per
@@ -53,9 +56,17 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// LDS per wave (doubles): the F and G tiles [4 groups][Np columns][16 rows]
+// (the face-flux tile [4][KF][64] reuses them); the stage-2 kernel also
+// keeps the fscale values its CFL quantity reads, [16 cells][Np].
 template <int N>
+constexpr int mfma_tile_doubles() {
+  constexpr int fg = 2 * 4 * 16 * Ord<N>::Np, fl = 4 * ((Ord<N>::NF + 3) / 4) * 64;
+  return fg > fl ? fg : fl;
+}
+template <int N, int EPI>
 constexpr int mfma_lds_doubles() {
-  return 2 * 4 * ((Ord<N>::Np + 3) / 4) * 64;  // per wave: F and G tiles [4][KP][64]
+  return mfma_tile_doubles<N>() + (EPI == EPI_STAGE2 ? kMfmaTile * Ord<N>::Np : 0);
 }
 
 // A workgroup is 1, 2 or 4 waves (16, 32 or 64 cells per workgroup visit);
@@ -71,18 +82,21 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   constexpr int KP = (Np + 3) / 4;  // K-steps over source nodes
   constexpr int KF = (NF + 3) / 4;  // K-steps over face nodes
   constexpr int TE = kMfmaTile, G = 4;
-  constexpr int NIT = (TE * 4 * KP + 63) / 64;  // Euler-flux items per lane
-  constexpr int NRT = (TE * NF + 63) / 64;      // Roe-flux items per lane
+  constexpr int NIT = (TE * Np + 63) / 64;  // Euler-flux items per lane
+  constexpr int NRT = (TE * NF + 63) / 64;  // Roe-flux items per lane
+  constexpr int GS = 16 * Np;               // doubles per group in the F and G tiles
   static_assert(Np <= 16, "one 16-row output tile per cell group");
-  static_assert(KF <= 2 * KP, "face tile must fit in the volume tiles it reuses");
-  // per wave: [0]: F tiles, [1]: G tiles, each [G][KP][64]; the face-flux
-  // tiles [G][KF][64] use the same memory before the Euler fluxes.
+  // per wave: F tiles [G][Np][16] then G tiles [G][Np][16] (column k of a
+  // group holds its rows cell + 4*var; the padding columns k >= Np of the
+  // last K-step are not stored, the lanes supply them as zeros); the
+  // face-flux tiles [G][KF][64] use the same memory before the Euler fluxes.
   extern __shared__ double lds_dyn[];
   __shared__ int sFm[NF];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
-  double *sA = lds_dyn + wave * mfma_lds_doubles<N>();
-  for (int i = lane; i < 2 * G * KP * 64; i += 64) sA[i] = 0.0;
+  double *sA = lds_dyn + wave * mfma_lds_doubles<N, EPI>();
+  double *sCfs = sA + mfma_tile_doubles<N>();  // stage 2: fscale at [cell][face-node slot < Np]
+  for (int i = lane; i < mfma_tile_doubles<N>(); i += 64) sA[i] = 0.0;
   if (threadIdx.x < NF) sFm[threadIdx.x] = fmask[threadIdx.x];
 
   // B operands: lane l holds B[k = 4*ks + l/16][j = l%16]
@@ -112,8 +126,13 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   int cfl_case = -1;
   bool bad = false;
   const double n1sq = (double)((N + 1) * (N + 1));
-  double *sF = sA, *sG = sA + G * KP * 64, *sL = sA;
+  double *sF = sA, *sG = sA + G * GS, *sL = sA;
   const int ce = lane >> 4;  // cell within a group owned by this lane's outputs
+  // A operand of K-step ks of group g from an F or G tile: lane l supplies
+  // column k = 4 ks + l/16, row l%16, and zero for the padding k >= Np
+  auto opA = [&](const double *t, int g, int ks) -> double {
+    return (4 * ks + 3 < Np || 4 * ks + kk < Np) ? t[g * GS + 64 * ks + lane] : 0.0;
+  };
 
   const int ntiles = multi ? a.tile_off[a.ncases] : (a.ncells + TE - 1) / TE;
   const int nblk = (ntiles + wpb - 1) / wpb;  // blocks of wpb tiles
@@ -215,6 +234,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
           boundary_state(ph, -code[j] - 1, mQ, nxv, nyv, pQ);
         }
         double fl[4];
+        if (EPI == EPI_STAGE2 && m < Np) sCfs[e * Np + m] = fs;
         roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
         const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
 #pragma unroll
@@ -244,36 +264,34 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     __syncthreads();  // the Euler fluxes overwrite the face-flux tiles
 
     // ---- Euler fluxes per (cell, node), written in the A-operand layout.
-    // The padding columns k >= Np are written with zeros on every tile: the
-    // face phase left its values there, and a zero B entry does not cancel a
-    // non-finite one (a failed case must not reach the next tile, which may
-    // belong to another case).
+    // The padding columns are never read (opA supplies their zeros), so a
+    // value the face phase left in this memory cannot reach the products (a
+    // failed case must not reach the next tile, which may belong to another
+    // case).
     {
-      const int nit = ne * 4 * KP;
+      const int nit = ne * Np;
 #pragma unroll
       for (int j = 0; j < NIT; j++) {
         const int it = lane + 64 * j;
         if (it >= nit) continue;
-        const int e = it / (4 * KP), k = it - e * (4 * KP);
-        double F[4] = {0.0, 0.0, 0.0, 0.0}, Gv[4] = {0.0, 0.0, 0.0, 0.0};
-        if (k < Np) {
-          const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)(e0 + e) * Np + k) * 4);
-          const double2 q01 = src[0], q23 = src[1];
-          double qv[4] = {q01.x, q01.y, q23.x, q23.y};
-          euler_flux(ph.gam, qv, F, Gv);
-          if constexpr (CG) {  // contravariant fluxes
-            const double2 *mt = reinterpret_cast<const double2 *>(a.cmet + (size_t)(e0 + e) * 4);
-            const double2 rxy = mt[0], sxy = mt[1];
+        const int e = it / Np, k = it - e * Np;
+        double F[4], Gv[4];
+        const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)(e0 + e) * Np + k) * 4);
+        const double2 q01 = src[0], q23 = src[1];
+        double qv[4] = {q01.x, q01.y, q23.x, q23.y};
+        euler_flux(ph.gam, qv, F, Gv);
+        if constexpr (CG) {  // contravariant fluxes
+          const double2 *mt = reinterpret_cast<const double2 *>(a.cmet + (size_t)(e0 + e) * 4);
+          const double2 rxy = mt[0], sxy = mt[1];
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-              const double f = F[v], g = Gv[v];
-              F[v] = rxy.x * f + rxy.y * g;
-              Gv[v] = sxy.x * f + sxy.y * g;
-            }
+          for (int v = 0; v < 4; v++) {
+            const double f = F[v], g = Gv[v];
+            F[v] = rxy.x * f + rxy.y * g;
+            Gv[v] = sxy.x * f + sxy.y * g;
           }
         }
         // A row = cell + 4*var, A column = source node k
-        const int off = ((e >> 2) * KP + (k >> 2)) * 64 + 16 * (k & 3) + (e & 3);
+        const int off = (e >> 2) * GS + 16 * k + (e & 3);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           sF[off + 4 * v] = F[v];
@@ -321,8 +339,8 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       if constexpr (CG) {
 #pragma unroll
         for (int ks = 0; ks < KP; ks++) {
-          c = mfma16x16x4(sF[(g * KP + ks) * 64 + lane], bR[ks], c);
-          c = mfma16x16x4(sG[(g * KP + ks) * 64 + lane], bS[ks], c);
+          c = mfma16x16x4(opA(sF, g, ks), bR[ks], c);
+          c = mfma16x16x4(opA(sG, g, ks), bS[ks], c);
         }
       } else {
         // F's two products with rx, sx, then G's with ry, sy: two live
@@ -330,7 +348,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         dbl4 d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < KP; ks++) {
-          const double aF = sF[(g * KP + ks) * 64 + lane];
+          const double aF = opA(sF, g, ks);
           d1 = mfma16x16x4(aF, bR[ks], d1);
           d2 = mfma16x16x4(aF, bS[ks], d2);
         }
@@ -340,7 +358,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         d2 = dbl4{0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < KP; ks++) {
-          const double aG = sG[(g * KP + ks) * 64 + lane];
+          const double aG = opA(sG, g, ks);
           d1 = mfma16x16x4(aG, bR[ks], d1);
           d2 = mfma16x16x4(aG, bS[ks], d2);
         }
@@ -378,18 +396,9 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       if (EPI == EPI_STAGE2) {
         if ((facenodes >> jj) & 1ull) {
           // src/calc_dt.h:11 reads fscale at face-node index FMASK[i], i.e.
-          // at this node's index jj read as a face-node index (face jj / Nfp)
-          // (the tile's Roe phase has just read these cache lines)
-          double fsj;
-          if constexpr (CG) {
-            const double2 *mt = reinterpret_cast<const double2 *>(a.cmet + (size_t)(e0 + e) * 4);
-            const double2 rxy = mt[0], sxy = mt[1];
-            double nxj, nyj;
-            cell_face_geometry(rxy.x, rxy.y, sxy.x, sxy.y, jj / Nfp, nxj, nyj, fsj);
-          } else {
-            fsj = a.fscale[(size_t)(e0 + e) * NF + jj];
-          }
-          const double cv = cfl_node(ph.gam, n1sq, o, fsj) + 0.0;
+          // at this node's index jj read as a face-node slot; the tile's Roe
+          // phase left slot jj's fscale in LDS
+          const double cv = cfl_node(ph.gam, n1sq, o, sCfs[e * Np + jj]) + 0.0;
           cfl_bits = max_bits(cfl_bits, (unsigned long long)__double_as_longlong(cv));
         }
         bad |= !(isfinite(o[0]) && isfinite(o[1]) && isfinite(o[2]) && isfinite(o[3])) ||
@@ -410,20 +419,20 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   }
 }
 
-// Waves per workgroup actually used for order N: 1, 2 or 4, within 48 KB of
+// Waves per workgroup actually used for order N: 1, 2 or 4, within 64 KB of
 // LDS per workgroup.
-template <int N>
+template <int N, int EPI>
 static int mfma_wpb(int want) {
   int w = want >= 4 ? 4 : (want >= 2 ? 2 : 1);
-  while (w > 1 && (size_t)w * mfma_lds_doubles<N>() * sizeof(double) > 48 * 1024) w /= 2;
+  while (w > 1 && (size_t)w * mfma_lds_doubles<N, EPI>() * sizeof(double) > 64 * 1024) w /= 2;
   return w;
 }
 
 template <int N, bool GATHER, int EPI, bool CG>
 static hipError_t launch_t(const Launch &L, const StageArgs &a) {
   static int per_cu[5] = {-1, -1, -1, -1, -1};  // resident workgroups per CU, by waves per workgroup
-  const int wpb = mfma_wpb<N>(L.wpb);
-  const size_t lds = (size_t)wpb * mfma_lds_doubles<N>() * sizeof(double);
+  const int wpb = mfma_wpb<N, EPI>(L.wpb);
+  const size_t lds = (size_t)wpb * mfma_lds_doubles<N, EPI>() * sizeof(double);
   if (per_cu[wpb] < 0) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rhs_mfma_kernel<N, GATHER, EPI, CG>, 64 * wpb, lds) !=

@@ -87,6 +87,9 @@ struct dg_sens {
   double stretch = 1.0, rfar = 20.0, width = 3.0;
   std::vector<double> xpk;     // peak positions of the bumps (upper ones first)
   std::vector<int> side;       // +1 upper, -1 lower
+  // hh[k * nb + i]: surface offset of circumferential index k per unit
+  // amplitude of bump i (0 on the other side of the profile)
+  std::vector<double> hh;
   double mach = 0.4, alpha = 0.0;
   std::vector<double> bumps;   // current shape
   dg::Mesh mesh;
@@ -311,9 +314,7 @@ int wall_data(dg_sens *s, const double *q, WallData &d) {
 
 // d(node y)/d(bump i) for every node (x does not move)
 double node_dy_db(const dg_sens *s, int node, int i) {
-  const int k = s->mesh.node_k[node];
-  if (dg::ogrid_surface_side(k, s->ncirc) != s->side[i]) return 0.0;
-  return s->mesh.node_w[node] * dg::hicks_henne(dg::ogrid_surface_x(k, s->ncirc), s->xpk[i], s->width);
+  return s->mesh.node_w[node] * s->hh[(size_t)s->mesh.node_k[node] * s->xpk.size() + i];
 }
 
 }  // namespace
@@ -342,6 +343,12 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   s->nbu = nbu; s->nbl = nbl; s->mach = mach; s->alpha = alpha_deg;
   for (int i = 0; i < nbu; i++) { s->xpk.push_back((i + 1.0) / (nbu + 1.0)); s->side.push_back(1); }
   for (int i = 0; i < nbl; i++) { s->xpk.push_back((i + 1.0) / (nbl + 1.0)); s->side.push_back(-1); }
+  const int nbt = nbu + nbl;
+  s->hh.assign((size_t)ncirc * nbt, 0.0);
+  for (int k = 0; k < ncirc; k++)
+    for (int i = 0; i < nbt; i++)
+      if (dg::ogrid_surface_side(k, ncirc) == s->side[i])
+        s->hh[(size_t)k * nbt + i] = dg::hicks_henne(dg::ogrid_surface_x(k, ncirc), s->xpk[i], s->width);
   std::string err;
   std::vector<double> zero(ncirc, 0.0);
   if (!dg::gen_ogrid_shaped(ncirc, nrad, stretch, rfar, &zero, s->mesh, err)) {
@@ -772,11 +779,12 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
     std::vector<double> gnode_y(s->mesh.nnodes, 0.0);
     for (size_t k = 0; k < K; k++)
       for (int t = 0; t < 3; t++) gnode_y[s->mesh.cells[3 * k + t]] += gv[6 * k + 2 * t + 1];
-    for (int i = 0; i < nb; i++) {
-      double g = 0.0;
-      for (int n = 0; n < s->mesh.nnodes; n++)
-        if (gnode_y[n] != 0.0) g += gnode_y[n] * node_dy_db(s, n, i);
-      dcl_db[i] = g;
+    for (int i = 0; i < nb; i++) dcl_db[i] = 0.0;
+    for (int n = 0; n < s->mesh.nnodes; n++) {
+      if (gnode_y[n] == 0.0) continue;
+      const double gw = gnode_y[n] * s->mesh.node_w[n];
+      const double *h = &s->hh[(size_t)s->mesh.node_k[n] * s->xpk.size()];
+      for (int i = 0; i < nb; i++) dcl_db[i] += gw * h[i];
     }
   }
   if (cl) *cl = clv;

@@ -773,6 +773,9 @@ int dg_solver_set_launch(dg_solver *s, int wpb, int walk, int max_blocks) {
 int dg_solver_set_geometry(dg_solver *s, int mode) {
   if (!s || (mode != DG_GEOM_CELL && mode != DG_GEOM_NODE))
     return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_set_geometry: mode is DG_GEOM_CELL or DG_GEOM_NODE");
+  if (mode == DG_GEOM_CELL && s->ctx->variant != 0)
+    return fail(s->ctx, DG_ERR_ARG,
+                "dg_solver_set_geometry: DG_GEOM_CELL needs the matrix-core stage kernel (variant 0)");
   drop_graph(s);
   s->cell_geometry = mode == DG_GEOM_CELL;
   return DG_OK;

@@ -107,10 +107,11 @@ def summarize_pmc(fetch_db, write_db, cfetch_db, cwrite_db, copy_bytes):
             continue
         rd, wr = f[k] * 1024.0 * rscale, w[k] * 1024.0 * wscale
         print(f"| `{k}` | {f[k]:.1f} | {w[k]:.1f} | {rd:.4e} | {wr:.4e} | {rd + wr:.4e} |")
-        for tag in ("rhs_mfma_kernel<3, true, 1>", "rhs_mfma_kernel<3, true, 2>", "rhs_mfma_kernel<3, true, 3>"):
-            if tag in k:
-                out["rhs_stage" + str(int(tag[-2]) - 1)] = {"bytes_per_launch": round(rd + wr),
-                                                           "read": round(rd), "write": round(wr)}
+        # stage kernels rhs_mfma_kernel<N, gather, stage + 1, per-cell geometry>
+        for st in range(3):
+            if f"rhs_mfma_kernel<3, true, {st + 1}, false>" in k:
+                out[f"rhs_stage{st}"] = {"bytes_per_launch": round(rd + wr), "read": round(rd),
+                                         "write": round(wr)}
     print(json.dumps(out))
 
 

@@ -295,8 +295,8 @@ int dg_solver_launch_config(const dg_solver *s, int *wpb, int *walk, int *max_bl
  * relative, 1e-12 and more on fine or thin cells, and DG_GEOM_CELL does not
  * carry that scatter: its results leave the reference's by that much (about
  * 2e-12 relative in rho v after 100 steps of the 64x64 vortex).  The
- * geometry is fixed when the solver is created; the vector-ALU stage kernel
- * (context variant 1) always reads the per-node factors. */
+ * geometry is fixed when the solver is created; DG_GEOM_CELL needs the
+ * matrix-core stage kernel (context variant 0, else DG_ERR_ARG). */
 #define DG_GEOM_NODE 0
 #define DG_GEOM_CELL 1
 int dg_solver_set_geometry(dg_solver *s, int mode);

@@ -37,9 +37,16 @@ struct AdjArgs {
   const double *y0;
   double f0, f1, f2;
   double *out2;
-  double *g_rx, *g_ry, *g_sx, *g_sy;    // += scale (dL/dgeo)^T mu  [K][Np]
-  double *g_nx, *g_ny, *g_fs;           // [K][NF]
+  double *g_rx, *g_ry, *g_sx, *g_sy;    // += scale (dL/dgeo)^T mu  [K][Np], or [K] (compact)
+  double *g_nx, *g_ny, *g_fs;           // [K][NF] (boundary pass: boundary slots only when compact)
+  double *g_nxf, *g_nyf, *g_fsf;        // compact: face-pass cotangents per face [K][3]
   double *g_bcu, *g_bcv;                // [K][NF] (boundary slots only)
+  // compact: the metric cotangents are summed over each cell's nodes and
+  // the face pass's normal / fscale cotangents over each face's nodes (the
+  // init_grid factors of a straight-sided cell are the same at all its
+  // nodes, and a face's normal and fscale at all the face's nodes, so the
+  // geometry chain needs only the sums); 0: per node, per slot
+  int compact;
   int nbedges;                          // boundary edges (cell, local face)
   const int *b2c, *bnum;
 };
@@ -59,9 +66,13 @@ hipError_t launch_record_dt(const Launch &L, const unsigned long long *dt1, doub
 // of cotangent * d(factor)/d(coordinate d) (forward mode, one dual
 // direction per thread).  g_J (cotangent of J at node 0) may be null.
 struct GeoCot {
-  const double *rx, *ry, *sx, *sy;  // [K][Np]
+  const double *rx, *ry, *sx, *sy;  // [K][Np], or [K] when compact
   const double *nx, *ny, *fs;       // [K][NF]
   const double *J;                  // [K]
+  // compact: rx..sy are per-cell sums, and nxf, nyf, fsf [K][3] per-face
+  // sums that add to the per-slot nx, ny, fs
+  int compact;
+  const double *nxf, *nyf, *fsf;
 };
 hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_xy, const GeoCot &g,
                           double *gv);

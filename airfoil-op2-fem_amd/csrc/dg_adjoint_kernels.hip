@@ -337,6 +337,7 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_face_kernel(AdjArgs a, Phy
   constexpr int CPB = kAdjThreads / NF;
   __shared__ double sL[Np * NF];
   __shared__ double sMu[4][CPB * Np];
+  __shared__ double sG[3][CPB * NF];  // compact: per-slot geometry cotangents of the block
   for (int k = threadIdx.x; k < Np * NF; k += kAdjThreads) sL[k] = ops[2 * Np * Np + k];
   const int t = threadIdx.x;
   const int cl = t / NF, m = t - cl * NF;
@@ -349,39 +350,58 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_face_kernel(AdjArgs a, Phy
       sMu[0][k] = mu.x; sMu[1][k] = mu.y; sMu[2][k] = mu.z; sMu[3][k] = mu.w;
     }
     __syncthreads();
-    if (cl >= nc) continue;
-    const size_t c = base + cl;
-    const size_t it = c * NF + m;
-    // cotangent of the scaled flux at this slot: L = vol - LIFT flux
-    double psi[4] = {0, 0, 0, 0};
-    for (int j = 0; j < Np; j++) {
-      const double l = sL[j * NF + m];
-      for (int v = 0; v < 4; v++) psi[v] -= l * sMu[v][cl * Np + j];
+    if (cl < nc) {
+      const size_t c = base + cl;
+      const size_t it = c * NF + m;
+      // cotangent of the scaled flux at this slot: L = vol - LIFT flux
+      double psi[4] = {0, 0, 0, 0};
+      for (int j = 0; j < Np; j++) {
+        const double l = sL[j * NF + m];
+        for (int v = 0; v < 4; v++) psi[v] -= l * sMu[v][cl * Np + j];
+      }
+      const int node = a.fmask[m];
+      double mQ[4], pQ[4];
+      for (int v = 0; v < 4; v++) mQ[v] = a.w[(c * Np + node) * 4 + v];
+      const double nx = a.nx[it], ny = a.ny[it], fs = a.fscale[it];
+      const int code = a.fmap[c * 3 + f];
+      if (code >= 0) {
+        const size_t nb = code >> 2;
+        const int nn = a.fmask[(code & 3) * Nfp + (Nfp - 1 - i)];
+        for (int v = 0; v < 4; v++) pQ[v] = a.w[(nb * Np + nn) * 4 + v];
+      } else {
+        boundary_state(ph, -code - 1, mQ, nx, ny, pQ);  // the forward's exterior state
+      }
+      double bm[4] = {0, 0, 0, 0}, bp[4] = {0, 0, 0, 0}, bnx = 0, bny = 0, bfs = 0;
+      roe_vjp(ph.gam, mQ, pQ, nx, ny, fs, psi, bm, bp, bnx, bny, bfs);
+      // On a boundary slot bp is the cotangent of the exterior (boundary)
+      // state; it stays in gpq (no neighbour reads a boundary slot's gpq) for
+      // adj_boundary_kernel, which reverses the boundary state into gmq.
+      for (int v = 0; v < 4; v++) {
+        a.gmq[it * 4 + v] = bm[v];
+        a.gpq[it * 4 + v] = bp[v];
+      }
+      if (a.compact) {
+        sG[0][t] = bnx; sG[1][t] = bny; sG[2][t] = bfs;
+      } else {
+        a.g_nx[it] += a.scale * bnx;
+        a.g_ny[it] += a.scale * bny;
+        a.g_fs[it] += a.scale * bfs;
+      }
     }
-    const int node = a.fmask[m];
-    double mQ[4], pQ[4];
-    for (int v = 0; v < 4; v++) mQ[v] = a.w[(c * Np + node) * 4 + v];
-    const double nx = a.nx[it], ny = a.ny[it], fs = a.fscale[it];
-    const int code = a.fmap[c * 3 + f];
-    if (code >= 0) {
-      const size_t nb = code >> 2;
-      const int nn = a.fmask[(code & 3) * Nfp + (Nfp - 1 - i)];
-      for (int v = 0; v < 4; v++) pQ[v] = a.w[(nb * Np + nn) * 4 + v];
-    } else {
-      boundary_state(ph, -code - 1, mQ, nx, ny, pQ);  // the forward's exterior state
+    if (a.compact) {
+      // one thread per face sums its Nfp slots in index order
+      __syncthreads();
+      if (cl < nc && i == 0) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int j = 0; j < Nfp; j++) {
+          s0 += sG[0][t + j]; s1 += sG[1][t + j]; s2 += sG[2][t + j];
+        }
+        const size_t fc = (base + cl) * 3 + f;
+        a.g_nxf[fc] += a.scale * s0;
+        a.g_nyf[fc] += a.scale * s1;
+        a.g_fsf[fc] += a.scale * s2;
+      }
     }
-    double bm[4] = {0, 0, 0, 0}, bp[4] = {0, 0, 0, 0}, bnx = 0, bny = 0, bfs = 0;
-    roe_vjp(ph.gam, mQ, pQ, nx, ny, fs, psi, bm, bp, bnx, bny, bfs);
-    // On a boundary slot bp is the cotangent of the exterior (boundary)
-    // state; it stays in gpq (no neighbour reads a boundary slot's gpq) for
-    // adj_boundary_kernel, which reverses the boundary state into gmq.
-    for (int v = 0; v < 4; v++) {
-      a.gmq[it * 4 + v] = bm[v];
-      a.gpq[it * 4 + v] = bp[v];
-    }
-    a.g_nx[it] += a.scale * bnx;
-    a.g_ny[it] += a.scale * bny;
-    a.g_fs[it] += a.scale * bfs;
   }
 }
 
@@ -451,61 +471,84 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phy
       for (int v = 0; v < 4; v++) { sFG[v][t] = F[v]; sFG[4 + v][t] = G[v]; }
     }
     __syncthreads();
-    if (!act) continue;
-    const int c0 = cl * Np;
-    // transposed weak derivatives with the metric terms
-    double phF[4] = {0, 0, 0, 0}, phG[4] = {0, 0, 0, 0};
-    for (int j = 0; j < Np; j++) {
-      const double cr = Drw[j * Np + i], cs = Dsw[j * Np + i];
-      const double aF = cr * sMet[0][c0 + j] + cs * sMet[1][c0 + j];
-      const double aG = cr * sMet[2][c0 + j] + cs * sMet[3][c0 + j];
-      for (int v = 0; v < 4; v++) {
-        const double mu = sMu[v][c0 + j];
-        phF[v] += aF * mu;
-        phG[v] += aG * mu;
-      }
-    }
-    double lam[4] = {0, 0, 0, 0};
-    euler_flux_vjp(ph.gam, wi, phF, phG, lam);
-    // face contributions that read this node
-    for (int s = 0; s < 2; s++) {
-      const int m = node_slots[2 * i + s];
-      if (m < 0) continue;
-      for (int v = 0; v < 4; v++) lam[v] += a.gmq[(c * NF + m) * 4 + v];
-      const int code = a.fmap[c * 3 + m / Nfp];
-      if (code >= 0) {
-        const size_t nb = code >> 2;
-        const int mp = (code & 3) * Nfp + (Nfp - 1 - m % Nfp);
-        for (int v = 0; v < 4; v++) lam[v] += a.gpq[(nb * NF + mp) * 4 + v];
-      }
-    }
-    double o[4];
-    for (int v = 0; v < 4; v++) o[v] = (a.x0 ? a.e0 * a.x0[it * 4 + v] : 0.0) + a.scale * lam[v];
-    *reinterpret_cast<double4 *>(a.out + it * 4) = make_double4(o[0], o[1], o[2], o[3]);
-    if (a.out2) {
-      double o2[4];
-      for (int v = 0; v < 4; v++)
-        o2[v] = a.f0 * a.y0[it * 4 + v] + (a.x0 ? a.f1 * a.x0[it * 4 + v] : 0.0) + a.f2 * o[v];
-      *reinterpret_cast<double4 *>(a.out2 + it * 4) = make_double4(o2[0], o2[1], o2[2], o2[3]);
-    }
-    // metric cotangents: d/drx[i] = mu[i] . (Drw F)[i], etc.
-    double dFdr[4] = {0, 0, 0, 0}, dFds[4] = {0, 0, 0, 0}, dGdr[4] = {0, 0, 0, 0}, dGds[4] = {0, 0, 0, 0};
-    for (int k = 0; k < Np; k++) {
-      const double ar = Drw[i * Np + k], as = Dsw[i * Np + k];
-      for (int v = 0; v < 4; v++) {
-        const double F = sFG[v][c0 + k], G = sFG[4 + v][c0 + k];
-        dFdr[v] += ar * F; dFds[v] += as * F; dGdr[v] += ar * G; dGds[v] += as * G;
-      }
-    }
     double grx = 0, gsx = 0, gry = 0, gsy = 0;
-    for (int v = 0; v < 4; v++) {
-      const double mu = sMu[v][t];
-      grx += mu * dFdr[v]; gsx += mu * dFds[v]; gry += mu * dGdr[v]; gsy += mu * dGds[v];
+    if (act) {
+      const int c0 = cl * Np;
+      // transposed weak derivatives with the metric terms
+      double phF[4] = {0, 0, 0, 0}, phG[4] = {0, 0, 0, 0};
+      for (int j = 0; j < Np; j++) {
+        const double cr = Drw[j * Np + i], cs = Dsw[j * Np + i];
+        const double aF = cr * sMet[0][c0 + j] + cs * sMet[1][c0 + j];
+        const double aG = cr * sMet[2][c0 + j] + cs * sMet[3][c0 + j];
+        for (int v = 0; v < 4; v++) {
+          const double mu = sMu[v][c0 + j];
+          phF[v] += aF * mu;
+          phG[v] += aG * mu;
+        }
+      }
+      double lam[4] = {0, 0, 0, 0};
+      euler_flux_vjp(ph.gam, wi, phF, phG, lam);
+      // face contributions that read this node
+      for (int s = 0; s < 2; s++) {
+        const int m = node_slots[2 * i + s];
+        if (m < 0) continue;
+        for (int v = 0; v < 4; v++) lam[v] += a.gmq[(c * NF + m) * 4 + v];
+        const int code = a.fmap[c * 3 + m / Nfp];
+        if (code >= 0) {
+          const size_t nb = code >> 2;
+          const int mp = (code & 3) * Nfp + (Nfp - 1 - m % Nfp);
+          for (int v = 0; v < 4; v++) lam[v] += a.gpq[(nb * NF + mp) * 4 + v];
+        }
+      }
+      double o[4];
+      for (int v = 0; v < 4; v++) o[v] = (a.x0 ? a.e0 * a.x0[it * 4 + v] : 0.0) + a.scale * lam[v];
+      *reinterpret_cast<double4 *>(a.out + it * 4) = make_double4(o[0], o[1], o[2], o[3]);
+      if (a.out2) {
+        double o2[4];
+        for (int v = 0; v < 4; v++)
+          o2[v] = a.f0 * a.y0[it * 4 + v] + (a.x0 ? a.f1 * a.x0[it * 4 + v] : 0.0) + a.f2 * o[v];
+        *reinterpret_cast<double4 *>(a.out2 + it * 4) = make_double4(o2[0], o2[1], o2[2], o2[3]);
+      }
+      // metric cotangents: d/drx[i] = mu[i] . (Drw F)[i], etc.
+      double dFdr[4] = {0, 0, 0, 0}, dFds[4] = {0, 0, 0, 0}, dGdr[4] = {0, 0, 0, 0}, dGds[4] = {0, 0, 0, 0};
+      for (int k = 0; k < Np; k++) {
+        const double ar = Drw[i * Np + k], as = Dsw[i * Np + k];
+        for (int v = 0; v < 4; v++) {
+          const double F = sFG[v][c0 + k], G = sFG[4 + v][c0 + k];
+          dFdr[v] += ar * F; dFds[v] += as * F; dGdr[v] += ar * G; dGds[v] += as * G;
+        }
+      }
+      for (int v = 0; v < 4; v++) {
+        const double mu = sMu[v][t];
+        grx += mu * dFdr[v]; gsx += mu * dFds[v]; gry += mu * dGdr[v]; gsy += mu * dGds[v];
+      }
+      if (!a.compact) {
+        a.g_rx[it] += a.scale * grx;
+        a.g_sx[it] += a.scale * gsx;
+        a.g_ry[it] += a.scale * gry;
+        a.g_sy[it] += a.scale * gsy;
+      }
     }
-    a.g_rx[it] += a.scale * grx;
-    a.g_sx[it] += a.scale * gsx;
-    a.g_ry[it] += a.scale * gry;
-    a.g_sy[it] += a.scale * gsy;
+    if (a.compact) {
+      // per-cell sums: each node's four metric cotangents go through LDS
+      // (the metric tiles are no longer read), one thread per cell adds
+      // them in node order
+      __syncthreads();
+      if (t < NT) {
+        sMet[0][t] = grx; sMet[1][t] = gsx; sMet[2][t] = gry; sMet[3][t] = gsy;
+      }
+      __syncthreads();
+      if (act && i == 0) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        for (int j = 0; j < Np; j++) {
+          s0 += sMet[0][t + j]; s1 += sMet[1][t + j]; s2 += sMet[2][t + j]; s3 += sMet[3][t + j];
+        }
+        a.g_rx[c] += a.scale * s0;
+        a.g_sx[c] += a.scale * s1;
+        a.g_ry[c] += a.scale * s2;
+        a.g_sy[c] += a.scale * s3;
+      }
+    }
   }
 }
 
@@ -540,9 +583,21 @@ __global__ __launch_bounds__(128) void geo_vjp_kernel(int ncells, const double *
     Dual rx[Np], ry[Np], sx[Np], sy[Np], nx[NF], ny[NF], fs[NF], Jc;
     init_grid_cell_t<N, Dual>(r, s, Dr, Ds, fmask, v, rx, ry, sx, sy, nx, ny, fs, Jc);
     double acc = g.J ? g.J[c] * Jc.d : 0.0;
-    for (int i = 0; i < Np; i++) {
-      const size_t k = c * Np + i;
-      acc += g.rx[k] * rx[i].d + g.ry[k] * ry[i].d + g.sx[k] * sx[i].d + g.sy[k] * sy[i].d;
+    if (g.compact) {
+      // the factors' derivatives are the same at every node of the cell
+      // (every node of a face) in exact arithmetic: node 0's (the face's
+      // first node's) stand for them
+      acc += g.rx[c] * rx[0].d + g.ry[c] * ry[0].d + g.sx[c] * sx[0].d + g.sy[c] * sy[0].d;
+      for (int f = 0; f < 3; f++) {
+        const size_t k = c * 3 + f;
+        const int m = f * (NF / 3);
+        acc += g.nxf[k] * nx[m].d + g.nyf[k] * ny[m].d + g.fsf[k] * fs[m].d;
+      }
+    } else {
+      for (int i = 0; i < Np; i++) {
+        const size_t k = c * Np + i;
+        acc += g.rx[k] * rx[i].d + g.ry[k] * ry[i].d + g.sx[k] * sx[i].d + g.sy[k] * sy[i].d;
+      }
     }
     for (int m = 0; m < NF; m++) {
       const size_t k = c * NF + m;

@@ -106,7 +106,11 @@ struct dg_sens {
   double *q, *wA, *wB, *qw;
   double *lam, *lamq, *M, *A, *B, *gmq, *gpq;
   double *Ld, *fluxd, *qd, *w1d, *w2d, *tmp;
+  // geometry cotangents of the reverse sweep: metric factors per cell (sums
+  // over the nodes), normals / fscale per face (face pass) and per slot
+  // (boundary pass, C_L), free-stream velocity per slot
   double *g_rx, *g_ry, *g_sx, *g_sy, *g_nx, *g_ny, *g_fs, *g_bcu, *g_bcv;
+  double *g_nxf, *g_nyf, *g_fsf;
   double *rxd, *ryd, *sxd, *syd, *nxd, *nyd, *fsd;
   // walls
   int nwall = 0;
@@ -389,9 +393,10 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   s->gmq = b.alloc<double>(FT); s->gpq = b.alloc<double>(FT);
   s->Ld = b.alloc<double>(ST); s->fluxd = b.alloc<double>(FT); s->qd = b.alloc<double>(ST);
   s->w1d = b.alloc<double>(ST); s->w2d = b.alloc<double>(ST); s->tmp = b.alloc<double>(ST);
-  s->g_rx = b.alloc<double>(K * Np); s->g_ry = b.alloc<double>(K * Np);
-  s->g_sx = b.alloc<double>(K * Np); s->g_sy = b.alloc<double>(K * Np);
+  s->g_rx = b.alloc<double>(K); s->g_ry = b.alloc<double>(K);
+  s->g_sx = b.alloc<double>(K); s->g_sy = b.alloc<double>(K);
   s->g_nx = b.alloc<double>(K * NF); s->g_ny = b.alloc<double>(K * NF); s->g_fs = b.alloc<double>(K * NF);
+  s->g_nxf = b.alloc<double>(K * 3); s->g_nyf = b.alloc<double>(K * 3); s->g_fsf = b.alloc<double>(K * 3);
   s->g_bcu = b.alloc<double>(K * NF); s->g_bcv = b.alloc<double>(K * NF);
   s->rxd = b.alloc<double>(K * Np); s->ryd = b.alloc<double>(K * Np);
   s->sxd = b.alloc<double>(K * Np); s->syd = b.alloc<double>(K * Np);
@@ -692,9 +697,10 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   HIPCHECK(c, hipMemcpyAsync(s->gmq, lidx.data(), sizeof(int) * lidx.size(), hipMemcpyHostToDevice, s->stream));
   HIPCHECK(c, hipMemcpyAsync(s->gpq, lval.data(), sizeof(double) * lval.size(), hipMemcpyHostToDevice, s->stream));
   HIPCHECK(c, dg::launch_add_indexed(L, (int)lidx.size(), reinterpret_cast<const int *>(s->gmq), s->gpq, s->lam));
-  double *acc[9] = {s->g_rx, s->g_ry, s->g_sx, s->g_sy, s->g_nx, s->g_ny, s->g_fs, s->g_bcu, s->g_bcv};
-  for (int a = 0; a < 9; a++)
-    HIPCHECK(c, hipMemsetAsync(acc[a], 0, sizeof(double) * (a < 4 ? K * Np : K * NF), s->stream));
+  double *acc[12] = {s->g_rx, s->g_ry, s->g_sx,  s->g_sy,  s->g_nx,  s->g_ny,
+                     s->g_fs, s->g_bcu, s->g_bcv, s->g_nxf, s->g_nyf, s->g_fsf};
+  for (int a = 0; a < 12; a++)
+    HIPCHECK(c, hipMemsetAsync(acc[a], 0, sizeof(double) * (a < 4 ? K : (a < 9 ? K * NF : K * 3)), s->stream));
   HIPCHECK(c, hipMemcpyAsync(s->wslot, wslot.data(), sizeof(int) * nws, hipMemcpyHostToDevice, s->stream));
   HIPCHECK(c, hipMemcpyAsync(s->wcot, wcot.data(), sizeof(double) * 3 * nws, hipMemcpyHostToDevice, s->stream));
   HIPCHECK(c, hipMemcpyAsync(s->gJ, hJ.data(), sizeof(double) * K, hipMemcpyHostToDevice, s->stream));
@@ -713,6 +719,8 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   a.fmap = s->fmap; a.fmask = c->d_fmask; a.gmq = s->gmq; a.gpq = s->gpq;
   a.g_rx = s->g_rx; a.g_ry = s->g_ry; a.g_sx = s->g_sx; a.g_sy = s->g_sy;
   a.g_nx = s->g_nx; a.g_ny = s->g_ny; a.g_fs = s->g_fs; a.g_bcu = s->g_bcu; a.g_bcv = s->g_bcv;
+  a.g_nxf = s->g_nxf; a.g_nyf = s->g_nyf; a.g_fsf = s->g_fsf;
+  a.compact = 1;
   a.nbedges = s->mesh.nbedges; a.b2c = s->d_b2c; a.bnum = s->d_bnum;
   for (int sg = nck - 1; sg >= 0; sg--) {
     const int n0 = sg * ckpt_every, n1 = std::min(s->nsteps, n0 + ckpt_every);
@@ -771,7 +779,8 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
     HIPCHECK(c, dg::launch_add_indexed(L, (int)nws, s->wslot, s->wcot, s->g_nx));
     HIPCHECK(c, dg::launch_add_indexed(L, (int)nws, s->wslot, s->wcot + nws, s->g_ny));
     HIPCHECK(c, dg::launch_add_indexed(L, (int)nws, s->wslot, s->wcot + 2 * nws, s->g_fs));
-    const dg::GeoCot gc{s->g_rx, s->g_ry, s->g_sx, s->g_sy, s->g_nx, s->g_ny, s->g_fs, s->gJ};
+    const dg::GeoCot gc{s->g_rx, s->g_ry, s->g_sx, s->g_sy, s->g_nx, s->g_ny, s->g_fs, s->gJ,
+                        1,       s->g_nxf, s->g_nyf, s->g_fsf};
     HIPCHECK(c, dg::launch_geo_vjp(L, s->K, s->cell_xy, gc, s->gv));
     std::vector<double> gv(6 * K);
     HIPCHECK(c, hipMemcpyAsync(gv.data(), s->gv, sizeof(double) * 6 * K, hipMemcpyDeviceToHost, s->stream));

@@ -292,6 +292,35 @@ def test_airfoil_ogrid_short_run_matches_oracle(dg, orc, geom):
     assert np.all((x >= -1e-12) & (x <= 1 + 1e-12))
 
 
+@pytest.mark.parametrize("order", [1, 2, 4])
+@pytest.mark.parametrize("geom", ["node", "cell"])
+def test_fused_solver_other_orders_match_oracle(dg, orc, order, geom):
+    """The fused stage kernels at N = 1, 2, 4 (other tile fillings, face-flux
+    padding columns, Np not a multiple of 4) on an O-grid with wall and far
+    field, 20 steps, each geometry mode against the oracle on the same
+    geometry."""
+    import os
+    mesh = dg.naca0012_ogrid(32, 8, stretch=1.2, rfar=10.0)
+    ctx = dg.Context(order)
+    ctx.set_freestream(GAM, 0.4, 3.0, 1.0, 1.0)
+    s = dg.Solver(ctx, mesh)
+    if geom == "cell" and ctx.variant != 0:
+        pytest.skip("per-cell geometry is a matrix-core kernel mode")
+    s.set_geometry(dg.GEOM_CELL if geom == "cell" else dg.GEOM_NODE)
+    q0 = s.get_q()
+    s.run(20, cfl=0.5)
+    assert s.check() == 0
+    o = orc.Oracle(dg.generate_operators(order), orc.freestream(GAM, 0.4, 3.0))
+    case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
+                       mesh.bedgeNum, mesh.bedge_type)
+    if geom == "cell":
+        o.use_cell_geometry(case)
+    st = o.new_state(case, q0)
+    o.run(case, st, 20, cfl=0.5, nthreads=min(16, os.cpu_count() or 1))
+    errs = rel_per_var(s.get_q(), st["q"])
+    assert max(errs) <= 1e-12, errs
+
+
 def test_reference_tables_order4(dg, orc, dev):
     """N=4 with the reference's own literal tables installed via set_const."""
     import json

@@ -338,48 +338,55 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_face_kernel(AdjArgs a, Phy
   __shared__ double sL[Np * NF];
   __shared__ double sMu[4][CPB * Np];
   __shared__ double sG[3][CPB * NF];  // compact: per-slot geometry cotangents of the block
+  __shared__ int sFm[NF];
   for (int k = threadIdx.x; k < Np * NF; k += kAdjThreads) sL[k] = ops[2 * Np * Np + k];
+  if (threadIdx.x < NF) sFm[threadIdx.x] = a.fmask[threadIdx.x];
   const int t = threadIdx.x;
   const int cl = t / NF, m = t - cl * NF;
   const int f = m / Nfp, i = m - f * Nfp;
+  const int node = m < NF ? a.fmask[m] : 0;
   for (size_t base = (size_t)blockIdx.x * CPB; base < (size_t)a.ncells; base += (size_t)gridDim.x * CPB) {
     __syncthreads();
     const int nc = (int)min((size_t)CPB, (size_t)a.ncells - base);
+    const bool act = cl < nc;
+    const size_t c = base + cl;
+    const size_t it = c * NF + m;
+    // the slot's own loads (independent of the staged mu) go out first, the
+    // exterior trace as soon as the face-map code is in
+    int code = 0;
+    double4 mq4 = make_double4(0.0, 0.0, 0.0, 0.0), pq4 = mq4;
+    double nx = 0.0, ny = 0.0, fs = 0.0;
+    if (act) {
+      code = a.fmap[c * 3 + f];
+      mq4 = *reinterpret_cast<const double4 *>(a.w + (c * Np + node) * 4);
+      nx = a.nx[it]; ny = a.ny[it]; fs = a.fscale[it];
+    }
     for (int k = t; k < nc * Np; k += kAdjThreads) {
       const double4 mu = *reinterpret_cast<const double4 *>(a.mu + (base * Np + k) * 4);
       sMu[0][k] = mu.x; sMu[1][k] = mu.y; sMu[2][k] = mu.z; sMu[3][k] = mu.w;
     }
+    if (act && code >= 0) {
+      const size_t nb = code >> 2;
+      const int nn = sFm[(code & 3) * Nfp + (Nfp - 1 - i)];
+      pq4 = *reinterpret_cast<const double4 *>(a.w + (nb * Np + nn) * 4);
+    }
     __syncthreads();
-    if (cl < nc) {
-      const size_t c = base + cl;
-      const size_t it = c * NF + m;
+    if (act) {
       // cotangent of the scaled flux at this slot: L = vol - LIFT flux
       double psi[4] = {0, 0, 0, 0};
       for (int j = 0; j < Np; j++) {
         const double l = sL[j * NF + m];
         for (int v = 0; v < 4; v++) psi[v] -= l * sMu[v][cl * Np + j];
       }
-      const int node = a.fmask[m];
-      double mQ[4], pQ[4];
-      for (int v = 0; v < 4; v++) mQ[v] = a.w[(c * Np + node) * 4 + v];
-      const double nx = a.nx[it], ny = a.ny[it], fs = a.fscale[it];
-      const int code = a.fmap[c * 3 + f];
-      if (code >= 0) {
-        const size_t nb = code >> 2;
-        const int nn = a.fmask[(code & 3) * Nfp + (Nfp - 1 - i)];
-        for (int v = 0; v < 4; v++) pQ[v] = a.w[(nb * Np + nn) * 4 + v];
-      } else {
-        boundary_state(ph, -code - 1, mQ, nx, ny, pQ);  // the forward's exterior state
-      }
+      double mQ[4] = {mq4.x, mq4.y, mq4.z, mq4.w}, pQ[4] = {pq4.x, pq4.y, pq4.z, pq4.w};
+      if (code < 0) boundary_state(ph, -code - 1, mQ, nx, ny, pQ);  // the forward's exterior state
       double bm[4] = {0, 0, 0, 0}, bp[4] = {0, 0, 0, 0}, bnx = 0, bny = 0, bfs = 0;
       roe_vjp(ph.gam, mQ, pQ, nx, ny, fs, psi, bm, bp, bnx, bny, bfs);
       // On a boundary slot bp is the cotangent of the exterior (boundary)
       // state; it stays in gpq (no neighbour reads a boundary slot's gpq) for
       // adj_boundary_kernel, which reverses the boundary state into gmq.
-      for (int v = 0; v < 4; v++) {
-        a.gmq[it * 4 + v] = bm[v];
-        a.gpq[it * 4 + v] = bp[v];
-      }
+      *reinterpret_cast<double4 *>(a.gmq + it * 4) = make_double4(bm[0], bm[1], bm[2], bm[3]);
+      *reinterpret_cast<double4 *>(a.gpq + it * 4) = make_double4(bp[0], bp[1], bp[2], bp[3]);
       if (a.compact) {
         sG[0][t] = bnx; sG[1][t] = bny; sG[2][t] = bfs;
       } else {
@@ -453,6 +460,8 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phy
   const double *Drw = sD, *Dsw = sD + Np * Np;
   const int t = threadIdx.x;
   const int cl = t / Np, i = t - cl * Np;
+  // the face slots that read this node (at most two)
+  const int ms0 = t < NT ? node_slots[2 * i] : -1, ms1 = t < NT ? node_slots[2 * i + 1] : -1;
   for (size_t base = (size_t)blockIdx.x * CPB; base < (size_t)a.ncells; base += (size_t)gridDim.x * CPB) {
     const int nc = (int)min((size_t)CPB, (size_t)a.ncells - base);
     const bool act = cl < nc;
@@ -488,25 +497,32 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phy
       }
       double lam[4] = {0, 0, 0, 0};
       euler_flux_vjp(ph.gam, wi, phF, phG, lam);
-      // face contributions that read this node
+      // face contributions that read this node: its own slots' interior
+      // side (gmq) and the facing slots' exterior side (gpq)
+      const int msl[2] = {ms0, ms1};
       for (int s = 0; s < 2; s++) {
-        const int m = node_slots[2 * i + s];
+        const int m = msl[s];
         if (m < 0) continue;
-        for (int v = 0; v < 4; v++) lam[v] += a.gmq[(c * NF + m) * 4 + v];
+        const double4 gm = *reinterpret_cast<const double4 *>(a.gmq + (c * NF + m) * 4);
+        lam[0] += gm.x; lam[1] += gm.y; lam[2] += gm.z; lam[3] += gm.w;
         const int code = a.fmap[c * 3 + m / Nfp];
         if (code >= 0) {
           const size_t nb = code >> 2;
           const int mp = (code & 3) * Nfp + (Nfp - 1 - m % Nfp);
-          for (int v = 0; v < 4; v++) lam[v] += a.gpq[(nb * NF + mp) * 4 + v];
+          const double4 gp = *reinterpret_cast<const double4 *>(a.gpq + (nb * NF + mp) * 4);
+          lam[0] += gp.x; lam[1] += gp.y; lam[2] += gp.z; lam[3] += gp.w;
         }
       }
+      const double4 x4 = a.x0 ? *reinterpret_cast<const double4 *>(a.x0 + it * 4) : make_double4(0, 0, 0, 0);
+      const double xv[4] = {x4.x, x4.y, x4.z, x4.w};
       double o[4];
-      for (int v = 0; v < 4; v++) o[v] = (a.x0 ? a.e0 * a.x0[it * 4 + v] : 0.0) + a.scale * lam[v];
+      for (int v = 0; v < 4; v++) o[v] = (a.x0 ? a.e0 * xv[v] : 0.0) + a.scale * lam[v];
       *reinterpret_cast<double4 *>(a.out + it * 4) = make_double4(o[0], o[1], o[2], o[3]);
       if (a.out2) {
+        const double4 y4 = *reinterpret_cast<const double4 *>(a.y0 + it * 4);
+        const double yv[4] = {y4.x, y4.y, y4.z, y4.w};
         double o2[4];
-        for (int v = 0; v < 4; v++)
-          o2[v] = a.f0 * a.y0[it * 4 + v] + (a.x0 ? a.f1 * a.x0[it * 4 + v] : 0.0) + a.f2 * o[v];
+        for (int v = 0; v < 4; v++) o2[v] = a.f0 * yv[v] + (a.x0 ? a.f1 * xv[v] : 0.0) + a.f2 * o[v];
         *reinterpret_cast<double4 *>(a.out2 + it * 4) = make_double4(o2[0], o2[1], o2[2], o2[3]);
       }
       // metric cotangents: d/drx[i] = mu[i] . (Drw F)[i], etc.

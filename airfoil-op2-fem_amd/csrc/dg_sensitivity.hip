@@ -840,8 +840,8 @@ int dg_rhs_adjoint(dg_ctx *c, const double *w, const double *mu, double scale, c
     HIPCHECK(c, hipMalloc(&c->d_node_slots, sizeof(int) * slots.size()));
     HIPCHECK(c, hipMemcpy(c->d_node_slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice));
   }
-  if (((uintptr_t)w | (uintptr_t)mu | (uintptr_t)acc) & 31u)
-    return fail(c, DG_ERR_ARG, "dg_rhs_adjoint: w, mu and acc must be 32-byte aligned");
+  if (((uintptr_t)w | (uintptr_t)mu | (uintptr_t)acc | (uintptr_t)scratch) & 31u)
+    return fail(c, DG_ERR_ARG, "dg_rhs_adjoint: w, mu, acc and scratch must be 32-byte aligned");
   dg::AdjArgs a{};
   a.ncells = c->ncells; a.scale = scale; a.w = w; a.mu = mu;
   a.rx = rx; a.ry = ry; a.sx = sx; a.sy = sy; a.nx = nx; a.ny = ny; a.fscale = fscale;

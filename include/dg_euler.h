@@ -323,7 +323,8 @@ int dg_rhs_tangent(dg_ctx *ctx, const double *d_w, const double *d_wd, const dou
                    double *d_out);
 /* d_acc += scale L'(w)^T mu; the geometry cotangents d_g* ([ncells][Np] for
  * rx..sy, [ncells][3*Nfp] for fscale, nx, ny, bcu, bcv) += scale (dL/dgeo)^T
- * mu.  Gather-only, deterministic.  d_scratch holds 2*ncells*3*Nfp*4 doubles. */
+ * mu.  Gather-only, deterministic.  d_scratch holds 2*ncells*3*Nfp*4 doubles.
+ * d_w, d_mu, d_acc and d_scratch must be 32-byte aligned (DG_ERR_ARG). */
 int dg_rhs_adjoint(dg_ctx *ctx, const double *d_w, const double *d_mu, double scale,
                    const double *d_rx, const double *d_ry, const double *d_sx,
                    const double *d_sy, const double *d_fscale, const double *d_nx,

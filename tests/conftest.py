@@ -30,6 +30,27 @@ def ops3(dg):
     return dg.generate_operators(3)
 
 
+@pytest.fixture(scope="session")
+def naca_vtk(tmp_path_factory):
+    """The reference's airfoil mesh (tests/golden/naca0012_mesh.npz, made by
+    make_naca0012_mesh.py from naca0012.vtk) written back out as a
+    triangle-only legacy-VTK file for the library's reader."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", "naca0012_mesh.npz"))
+    pts, tris = d["points"], d["triangles"]
+    path = tmp_path_factory.mktemp("mesh") / "naca0012.vtk"
+    with open(path, "w") as f:
+        f.write("# vtk DataFile Version 2.0\nnaca0012\nASCII\nDATASET UNSTRUCTURED_GRID\n")
+        f.write(f"POINTS {len(pts)} double\n")
+        for p in pts:  # shortest round-trip decimal form: the same doubles back
+            f.write(f"{float(p[0])!r} {float(p[1])!r} {float(p[2])!r}\n")
+        f.write(f"CELLS {len(tris)} {4 * len(tris)}\n")
+        for t in tris:
+            f.write(f"3 {t[0]} {t[1]} {t[2]}\n")
+        f.write(f"CELL_TYPES {len(tris)}\n")
+        f.write("5\n" * len(tris))
+    return str(path)
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -80,6 +80,21 @@ def test_reference_vtk_mesh_classification(dg):
         assert t[b] == expect
 
 
+def test_mesh_fixture_reads_like_the_reference_file(dg, naca_vtk):
+    """The committed airfoil-mesh fixture, read through the VTK reader, is the
+    mesh the reference file gives (same points, cells, edges and boundary
+    classes); checked against the file itself where the reference is
+    mounted."""
+    m = dg.read_vtk(naca_vtk)
+    assert (m.ncells, m.nnodes) == (2450, 1395)
+    assert set(np.unique(m.bedge_type)) <= {0, 1, 2}
+    assert np.all(_areas(m.cell_xy.reshape(-1, 3, 2)) > 0)
+    if os.path.exists(VTK):
+        r = dg.read_vtk(VTK)
+        for k in ("cells", "cell_xy", "edge2cells", "edgeNum", "bedge2cells", "bedgeNum", "bedge_type"):
+            assert np.array_equal(getattr(m, k), getattr(r, k)), k
+
+
 def test_mesh_errors_are_reported(dg):
     with pytest.raises(dg.DGError):
         dg.periodic_grid(2, 2)

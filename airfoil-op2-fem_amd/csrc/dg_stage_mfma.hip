@@ -56,12 +56,44 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// The lane index passed through an empty asm statement inside the tile loop.
+// A lane's (cell, node) items are the same on every tile, so the compiler
+// hoists their decomposition and addresses out of the loop; held for the
+// kernel's lifetime they overflow the register budget and are spilled to
+// scratch, and every reload then waits for all loads in flight.  Derived
+// from this value they are recomputed per tile (a few integer operations).
+// Measured at N=3: the face-map code prefetch of every stage and the Roe
+// and Euler-flux items of the stage-2 kernel (the one with the CFL
+// epilogue) gain from it; the other stages' items keep their registers.
+// At N=4 every stage spills without it (the items of all stages go through it).
+__device__ __forceinline__ int lane_in_loop(int l) {
+  asm volatile("" : "+v"(l));
+  return l;
+}
+
 // LDS per wave (doubles): the F and G tiles [4 groups][Np columns][16 rows]
-// (the face-flux tile [4][KF][64] reuses them); the stage-2 kernel also
-// keeps the fscale values its CFL quantity reads, [16 cells][Np].
+// (the face-flux tile [4][4 KF columns][16 rows] reuses them); the stage-2
+// kernel also keeps the fscale values its CFL quantity reads, [16 cells][Np].
+// A column holds its 16 rows in a slot of kColStride doubles and a group
+// starts kGroupSkew doubles past the end of the previous one.  The flux
+// writes put the nodes of one cell (same row, consecutive columns) on
+// neighbouring lanes; with 16-double columns those all fall in one bank pair
+// of the 32 four-byte banks (a 10- or 12-way conflict on every store).  An
+// 18-double slot moves consecutive columns two banks apart and the skew
+// separates cells of different groups, while the operand reads (16 rows of
+// one column per 16 lanes) stay within one slot.
+constexpr int kColStride = 18, kGroupSkew = 2;
+template <int N>
+constexpr int mfma_fg_group() {  // doubles per group in the F and G tiles
+  return kColStride * Ord<N>::Np + kGroupSkew;
+}
+template <int N>
+constexpr int mfma_fl_group() {  // doubles per group in the face-flux tile
+  return kColStride * 4 * ((Ord<N>::NF + 3) / 4) + kGroupSkew;
+}
 template <int N>
 constexpr int mfma_tile_doubles() {
-  constexpr int fg = 2 * 4 * 16 * Ord<N>::Np, fl = 4 * ((Ord<N>::NF + 3) / 4) * 64;
+  constexpr int fg = 2 * 4 * mfma_fg_group<N>(), fl = 4 * mfma_fl_group<N>();
   return fg > fl ? fg : fl;
 }
 template <int N, int EPI>
@@ -84,12 +116,15 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   constexpr int TE = kMfmaTile, G = 4;
   constexpr int NIT = (TE * Np + 63) / 64;  // Euler-flux items per lane
   constexpr int NRT = (TE * NF + 63) / 64;  // Roe-flux items per lane
-  constexpr int GS = 16 * Np;               // doubles per group in the F and G tiles
+  constexpr int GS = mfma_fg_group<N>();    // doubles per group in the F and G tiles
+  constexpr int GL = mfma_fl_group<N>();    // doubles per group in the face-flux tile
+  constexpr int CS = kColStride;
+  constexpr bool kItemsInLoop = EPI == EPI_STAGE2 || N >= 4;  // see lane_in_loop
   static_assert(Np <= 16, "one 16-row output tile per cell group");
-  // per wave: F tiles [G][Np][16] then G tiles [G][Np][16] (column k of a
+  // per wave: F tiles [G][Np][CS] then G tiles [G][Np][CS] (column k of a
   // group holds its rows cell + 4*var; the padding columns k >= Np of the
   // last K-step are not stored, the lanes supply them as zeros); the
-  // face-flux tiles [G][KF][64] use the same memory before the Euler fluxes.
+  // face-flux tiles [G][4 KF][CS] use the same memory before the Euler fluxes.
   extern __shared__ double lds_dyn[];
   __shared__ int sFm[NF];
 
@@ -131,7 +166,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   // A operand of K-step ks of group g from an F or G tile: lane l supplies
   // column k = 4 ks + l/16, row l%16, and zero for the padding k >= Np
   auto opA = [&](const double *t, int g, int ks) -> double {
-    return (4 * ks + 3 < Np || 4 * ks + kk < Np) ? t[g * GS + 64 * ks + lane] : 0.0;
+    return (4 * ks + 3 < Np || 4 * ks + kk < Np) ? t[g * GS + CS * (4 * ks + kk) + jj] : 0.0;
   };
 
   const int ntiles = multi ? a.tile_off[a.ncases] : (a.ncells + TE - 1) / TE;
@@ -162,9 +197,10 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   };
   // Face-map codes of a tile's Roe items (lane + 64 j -> (cell, face node)).
   auto load_codes = [&](int e0_, int ne_, int *c) {
+    const int lr = lane_in_loop(lane);
 #pragma unroll
     for (int j = 0; j < NRT; j++) {
-      const int it = lane + 64 * j;
+      const int it = lr + 64 * j;
       c[j] = (GATHER && it < ne_ * NF) ? a.fmap[(size_t)(e0_ + it / NF) * 3 + (it % NF) / Nfp] : 0;
     }
   };
@@ -196,9 +232,10 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     // ---- Roe fluxes per (cell, face node), written in the A-operand layout
     {
       const int nrt = ne * NF;
+      const int lr = kItemsInLoop ? lane_in_loop(lane) : lane;
 #pragma unroll
       for (int j = 0; j < NRT; j++) {
-        const int it = lane + 64 * j;
+        const int it = lr + 64 * j;
         if (it >= nrt) continue;
         const int e = it / NF, m = it - e * NF;
         const int i = m - (m / Nfp) * Nfp;
@@ -236,7 +273,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         double fl[4];
         if (EPI == EPI_STAGE2 && m < Np) sCfs[e * Np + m] = fs;
         roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
-        const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
+        const int off = (e >> 2) * GL + CS * m + (e & 3);
 #pragma unroll
         for (int v = 0; v < 4; v++) sL[off + 4 * v] = fl[v];
       }
@@ -244,7 +281,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         constexpr int PADF = 4 * KF - NF;
         for (int it = lane; it < ne * PADF; it += 64) {
           const int e = it / PADF, m = NF + (it - e * PADF);
-          const int off = ((e >> 2) * KF + (m >> 2)) * 64 + 16 * (m & 3) + (e & 3);
+          const int off = (e >> 2) * GL + CS * m + (e & 3);
 #pragma unroll
           for (int v = 0; v < 4; v++) sL[off + 4 * v] = 0.0;
         }
@@ -259,7 +296,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       acc[g] = dbl4{0.0, 0.0, 0.0, 0.0};
       if (4 * g >= ne) continue;
 #pragma unroll
-      for (int ks = 0; ks < KF; ks++) acc[g] = mfma16x16x4(sL[(g * KF + ks) * 64 + lane], bL[ks], acc[g]);
+      for (int ks = 0; ks < KF; ks++) acc[g] = mfma16x16x4(sL[g * GL + CS * (4 * ks + kk) + jj], bL[ks], acc[g]);
     }
     __syncthreads();  // the Euler fluxes overwrite the face-flux tiles
 
@@ -270,9 +307,10 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     // case).
     {
       const int nit = ne * Np;
+      const int lr = kItemsInLoop ? lane_in_loop(lane) : lane;
 #pragma unroll
       for (int j = 0; j < NIT; j++) {
-        const int it = lane + 64 * j;
+        const int it = lr + 64 * j;
         if (it >= nit) continue;
         const int e = it / Np, k = it - e * Np;
         double F[4], Gv[4];
@@ -291,7 +329,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
           }
         }
         // A row = cell + 4*var, A column = source node k
-        const int off = (e >> 2) * GS + 16 * k + (e & 3);
+        const int off = (e >> 2) * GS + CS * k + (e & 3);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           sF[off + 4 * v] = F[v];

@@ -131,9 +131,13 @@ struct dg_sens {
   double cfl = 1.0;
   double fwd_ms = 0.0, grad_ms = 0.0, tlm_ms = 0.0;
   double grad_fwd_ms = 0.0, grad_rev_ms = 0.0, grad_host_ms = 0.0;
-  // checkpoints (state every ckpt steps) and the segment store: q, w1, w2 of
-  // each step of one segment, [3 * j + {0, 1, 2}]
-  std::vector<double *> ckpt, seg;
+  // checkpoints (state every ckpt steps) and two segment stores: q, w1, w2 of
+  // each step of one segment, [3 * j + {0, 1, 2}].  The reverse sweep reads
+  // one store while the second stream recomputes the next segment into the
+  // other; ready[b]: store b holds its segment, free[b]: its reverse is done.
+  std::vector<double *> ckpt, seg, seg2;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
   int seg_len = 0;
 };
 
@@ -188,10 +192,12 @@ dg::StageArgs stage_args(dg_sens *s) {
 
 // One SSPRK3 step of q with the stage states w1 -> w1out, w2 -> w2out and the
 // new state -> qout (qout may be q; null skips the third stage); frozen dt
-// dts[n] unless recording (then dt = cfl / dt1 and dts[n] is recorded).
+// dts[n] unless recording (then dt = cfl / dt1 and dts[n] is recorded).  On
+// the solver's stream unless another is given.
 hipError_t step3(dg_sens *s, const double *q, double *w1out, double *w2out, double *qout, int n,
-                 bool record) {
+                 bool record, hipStream_t stream = nullptr) {
   dg::Launch L = launch_a(s);
+  if (stream) L.stream = stream;
   dg::StageArgs a = stage_args(s);
   const int p = n & 1;
   a.dt1_cur = s->dt1 + p;
@@ -223,6 +229,8 @@ hipError_t stages01(dg_sens *s, const double *q, int n) {
 double *seg_q(dg_sens *s, int j) { return s->seg[3 * j]; }
 double *seg_w1(dg_sens *s, int j) { return s->seg[3 * j + 1]; }
 double *seg_w2(dg_sens *s, int j) { return s->seg[3 * j + 2]; }
+// entry k = 3 j + {0, 1, 2} of segment store b
+double *seg_at(dg_sens *s, int b, int k) { return b ? s->seg2[k] : s->seg[k]; }
 
 int set_ic(dg_sens *s, double *q) {
   HIPCHECK(s->ctx, dg::launch_set_ic(launch_a(s), s->K, q, s->wA));
@@ -328,9 +336,16 @@ extern "C" {
 void dg_sens_destroy(dg_sens *s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->stream2) (void)hipStreamSynchronize(s->stream2);
   for (double *p : s->ckpt) (void)hipFree(p);
   for (double *p : s->seg) (void)hipFree(p);
+  for (double *p : s->seg2) (void)hipFree(p);
   if (s->dts) (void)hipFree(s->dts);
+  for (int b = 0; b < 2; b++) {
+    if (s->ev_ready[b]) (void)hipEventDestroy(s->ev_ready[b]);
+    if (s->ev_free[b]) (void)hipEventDestroy(s->ev_free[b]);
+  }
+  if (s->stream2) (void)hipStreamDestroy(s->stream2);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -370,7 +385,12 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   s->K = M.ncells; s->Np = c->Np; s->NF = c->NF; s->Nfp = c->Nfp;
   const size_t K = s->K, Np = s->Np, NF = s->NF, ST = K * Np * 4, FT = K * NF * 4;
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete s; return hipfail(c, e, "dg_sens_create: stream"); }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking);
+  for (int k = 0; k < 2 && e == hipSuccess; k++) {
+    e = hipEventCreateWithFlags(&s->ev_ready[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_free[k], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) { dg_sens_destroy(s); return hipfail(c, e, "dg_sens_create: streams"); }
   DevBuf &b = s->buf;
   s->cell_xy = b.alloc<double>(K * 6); s->nodeX = b.alloc<double>(K * 3); s->nodeY = b.alloc<double>(K * 3);
   s->x = b.alloc<double>(K * Np); s->y = b.alloc<double>(K * Np);
@@ -620,6 +640,11 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
     HIPCHECK(c, hipMalloc(&p, sizeof(double) * ST));
     s->seg.push_back(p);
   }
+  while (nck > 1 && (int)s->seg2.size() < 3 * ckpt_every) {
+    double *p = nullptr;
+    HIPCHECK(c, hipMalloc(&p, sizeof(double) * ST));
+    s->seg2.push_back(p);
+  }
   using clk = std::chrono::steady_clock;
   auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
   auto t0 = clk::now();
@@ -707,12 +732,36 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   HIPCHECK(c, hipStreamSynchronize(s->stream));  // host vectors leave scope
   s->grad_host_ms = ms_since(th);
   // reverse sweep over checkpoint segments; each segment but the last is
-  // recomputed from its checkpoint into the segment store (q, w1, w2 of
-  // every step), then reversed step by step:
+  // recomputed from its checkpoint into a segment store (q, w1, w2 of every
+  // step), then reversed step by step:
   //   A   = 2/3 lam' + 2/3 dt L'(w2)^T lam'
   //   B   = 1/4 A + 1/4 dt L'(w1)^T A,     T = 1/3 lam' + 3/4 A + B
   //   lam = T + dt L'(q)^T B
+  // Segment sg lives in store (nck - 1 - sg) & 1: the last one in store 0,
+  // where the forward run left it.  The recomputation of segment sg - 1
+  // runs on the second stream while the main stream reverses segment sg;
+  // both are latency-bound launches that leave compute units idle, so
+  // together they fill the device better than one after the other.  The
+  // kernels and their inputs are those of the serial order, so the results
+  // are the same bits.
   auto tr = clk::now();
+  const size_t sbytes = sizeof(double) * ST;
+  auto recompute = [&](int sg) -> hipError_t {
+    const int b = (nck - 1 - sg) & 1;
+    const int n0 = sg * ckpt_every, n1 = std::min(s->nsteps, n0 + ckpt_every);
+    hipError_t e = hipStreamWaitEvent(s->stream2, s->ev_free[b], 0);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(seg_at(s, b, 0), s->ckpt[sg], sbytes, hipMemcpyDeviceToDevice, s->stream2);
+    for (int n = n0; n < n1 && e == hipSuccess; n++) {
+      const int j = n - n0;
+      e = step3(s, seg_at(s, b, 3 * j), seg_at(s, b, 3 * j + 1), seg_at(s, b, 3 * j + 2),
+                n + 1 < n1 ? seg_at(s, b, 3 * j + 3) : nullptr, n, false, s->stream2);
+    }
+    return e == hipSuccess ? hipEventRecord(s->ev_ready[b], s->stream2) : e;
+  };
+  HIPCHECK(c, hipEventRecord(s->ev_free[0], s->stream));
+  HIPCHECK(c, hipEventRecord(s->ev_free[1], s->stream));
+  if (nck > 1) HIPCHECK(c, recompute(nck - 2));
   dg::AdjArgs a{};
   a.ncells = s->K;
   a.rx = s->rx; a.ry = s->ry; a.sx = s->sx; a.sy = s->sy; a.nx = s->nx; a.ny = s->ny; a.fscale = s->fs;
@@ -724,30 +773,27 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   a.nbedges = s->mesh.nbedges; a.b2c = s->d_b2c; a.bnum = s->d_bnum;
   for (int sg = nck - 1; sg >= 0; sg--) {
     const int n0 = sg * ckpt_every, n1 = std::min(s->nsteps, n0 + ckpt_every);
-    if (sg != nck - 1) {
-      HIPCHECK(c, hipMemcpyAsync(seg_q(s, 0), s->ckpt[sg], sizeof(double) * ST, hipMemcpyDeviceToDevice, s->stream));
-      for (int n = n0; n < n1; n++) {
-        const int j = n - n0;
-        HIPCHECK(c, step3(s, seg_q(s, j), seg_w1(s, j), seg_w2(s, j), n + 1 < n1 ? seg_q(s, j + 1) : nullptr,
-                          n, false));
-      }
-    }
+    const int b = (nck - 1 - sg) & 1;
+    if (sg != nck - 1) HIPCHECK(c, hipStreamWaitEvent(s->stream, s->ev_ready[b], 0));
     for (int n = n1 - 1; n >= n0; n--) {
       const double dt = s->h_dts[n];
       const int j = n - n0;
-      a.w = seg_w2(s, j); a.mu = s->lam;
+      a.w = seg_at(s, b, 3 * j + 2); a.mu = s->lam;
       a.x0 = s->lam; a.e0 = 2.0 / 3.0; a.scale = 2.0 / 3.0 * dt; a.out = s->A; a.out2 = nullptr;
       HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
-      a.w = seg_w1(s, j); a.mu = s->A;
+      a.w = seg_at(s, b, 3 * j + 1); a.mu = s->A;
       a.x0 = s->A; a.e0 = 0.25; a.scale = 0.25 * dt; a.out = s->B;
       a.out2 = s->lam; a.y0 = s->lam; a.f0 = 1.0 / 3.0; a.f1 = 0.75; a.f2 = 1.0;
       HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
-      a.w = seg_q(s, j); a.mu = s->B;
+      a.w = seg_at(s, b, 3 * j); a.mu = s->B;
       a.x0 = s->lam; a.e0 = 1.0; a.scale = dt; a.out = s->lam; a.out2 = nullptr;
       HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
     }
+    HIPCHECK(c, hipEventRecord(s->ev_free[b], s->stream));
+    if (sg >= 2) HIPCHECK(c, recompute(sg - 2));
   }
   HIPCHECK(c, hipStreamSynchronize(s->stream));
+  HIPCHECK(c, hipStreamSynchronize(s->stream2));
   s->grad_rev_ms = ms_since(tr);
   th = clk::now();
   // alpha through the initial state q0 = free stream(alpha) (sums of the

@@ -49,12 +49,22 @@ struct AdjArgs {
   int compact;
   int nbedges;                          // boundary edges (cell, local face)
   const int *b2c, *bnum;
+  // Face-pair form (compact only; null pairs: the per-slot form).  pairs
+  // [npairs][2] lists each interior face once as (c * 3 + f) of its two
+  // sides, the first the lower cell.  psi [K][NF][4] holds -LIFT^T mu, the
+  // cotangent of the scaled fluxes: read by the face-pair pass, rewritten
+  // by the cell pass from its out for the next launch (whose mu is out).
+  const int *pairs;
+  int npairs;
+  double *psi;
 };
 
 // Ld = L'(w) wd + dL/dgeo geod + dL/dbc (bcud, bcvd)
 hipError_t launch_rhs_tlm(const Launch &L, const TlmArgs &a, double bcud, double bcvd);
 // node_slots: [Np][2] face slots m with FMASK[m] == node (-1 when none)
 hipError_t launch_rhs_adj(const Launch &L, const AdjArgs &a, const int *node_slots);
+// a.psi = -LIFT^T a.mu (the face-pair form's psi of a first launch)
+hipError_t launch_adj_psi(const Launch &L, const AdjArgs &a);
 // z = a x + b y  (y may be null)
 hipError_t launch_lincomb(const Launch &L, size_t n, double a, const double *x, double b,
                           const double *y, double *z);

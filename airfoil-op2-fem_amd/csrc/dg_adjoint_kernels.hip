@@ -152,10 +152,12 @@ __device__ __forceinline__ void euler_flux_vjp(double gam, const double q[4], co
 }
 
 // Reverse of roe_t: given the cotangent o[4] of the scaled flux, returns the
-// cotangents of mQ, eQ (accumulated) and of nx, ny, fs.
-__device__ void roe_vjp(double gam, const double mQ[4], const double eQ[4], double nx, double ny,
-                        double fs, const double o[4], double bmQ[4], double beQ[4], double &bnx,
-                        double &bny, double &bfs) {
+// cotangents of mQ, eQ (accumulated) and of nx, ny, fs; with R non-null also
+// the unscaled flux R(mQ, eQ, n) itself (the scaled one is fs R).
+__device__ __forceinline__ void roe_vjp(double gam, const double mQ[4], const double eQ[4], double nx,
+                                        double ny, double fs, const double o[4], double bmQ[4],
+                                        double beQ[4], double &bnx, double &bny, double &bfs,
+                                        double *R = nullptr) {
   const double g1 = gam - 1.0;
   // forward sweep
   const double mR0 = mQ[0], mR1 = nx * mQ[1] + ny * mQ[2], mR2 = -ny * mQ[1] + nx * mQ[2], mR3 = mQ[3];
@@ -192,6 +194,9 @@ __device__ void roe_vjp(double gam, const double mQ[4], const double eQ[4], doub
                     (X1 * (H - u * c) + X2 * (u * u + v * v) + X3 * v + X4 * (H + u * c)) / 2.0;
   // reverse sweep
   bfs = o[0] * f0 + o[1] * (nx * f1 - ny * f2) + o[2] * (ny * f1 + nx * f2) + o[3] * f3;
+  if (R) {
+    R[0] = f0; R[1] = nx * f1 - ny * f2; R[2] = ny * f1 + nx * f2; R[3] = f3;
+  }
   const double b1o = o[1] * fs, b2o = o[2] * fs;
   bnx = b1o * f1 + b2o * f2;
   bny = -b1o * f2 + b2o * f1;
@@ -446,17 +451,178 @@ __global__ __launch_bounds__(256) void adj_boundary_kernel(AdjArgs a, Phys ph, c
   }
 }
 
+// psi = -LIFT^T mu per face slot, the cotangent of the scaled Roe fluxes
+// (one thread per slot, whole cells per workgroup, the cells' mu staged in
+// LDS): the face-pair form's psi for a first launch; the cell pass writes
+// it for the launches after.
+template <int N>
+__global__ __launch_bounds__(kAdjThreads) void adj_psi_kernel(AdjArgs a, const double *__restrict__ ops) {
+  constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
+  constexpr int CPB = kAdjThreads / NF;
+  __shared__ double sL[Np * NF];
+  __shared__ double sMu[4][CPB * Np];
+  for (int k = threadIdx.x; k < Np * NF; k += kAdjThreads) sL[k] = ops[2 * Np * Np + k];
+  const int t = threadIdx.x;
+  const int cl = t / NF, m = t - cl * NF;
+  for (size_t base = (size_t)blockIdx.x * CPB; base < (size_t)a.ncells; base += (size_t)gridDim.x * CPB) {
+    __syncthreads();
+    const int nc = (int)min((size_t)CPB, (size_t)a.ncells - base);
+    for (int k = t; k < nc * Np; k += kAdjThreads) {
+      const double4 mu = *reinterpret_cast<const double4 *>(a.mu + (base * Np + k) * 4);
+      sMu[0][k] = mu.x; sMu[1][k] = mu.y; sMu[2][k] = mu.z; sMu[3][k] = mu.w;
+    }
+    __syncthreads();
+    if (cl < nc) {
+      double p[4] = {0, 0, 0, 0};
+      for (int j = 0; j < Np; j++) {
+        const double l = sL[j * NF + m];
+        for (int v = 0; v < 4; v++) p[v] -= l * sMu[v][cl * Np + j];
+      }
+      *reinterpret_cast<double4 *>(a.psi + ((base + cl) * NF + m) * 4) = make_double4(p[0], p[1], p[2], p[3]);
+    }
+  }
+}
+
+// Face-pair form of the face and boundary passes.
+//
+// The Roe flux is antisymmetric, R(p, m, -n) = -R(m, p, n) (the flux of
+// src/euler_rhs.h including its dW2 (u^2 + v^2) term: swapping the sides and
+// reversing the normal negates the rotated velocities, which swaps the
+// 1st and 4th wave strengths with a sign change, negates the 2nd and keeps
+// the 3rd, so the mass and energy components change sign and the rotated
+// momentum components keep theirs, which the back-rotation with -n turns
+// into a sign change).  The two sides of an interior face carry opposite
+// normals, so the scaled fluxes of a face node pair, fs_o R(m, p, n) and
+// fs_p R(p, m, -n) = -fs_p R(m, p, n), are reversed together: one VJP with
+// the cotangent fs_o psi_o - fs_p psi_p gives both traces' cotangents
+// (each written to its own slot's gmq) and the normal's, which goes to the
+// first side's face (the second side's normal is its negative, so the
+// geometry chain sees the same total); fs_o and fs_p get psi_o . R and
+// -psi_p . R.  In exact arithmetic this is the per-slot reverse; init_grid's
+// per-node rounding makes the two normals opposite only to ~1e-15 relative
+// times the cell's aspect, and moves the gradient by that much.
+//
+// Blocks [0, nbblk): the boundary slots, 8 threads each, one per direction
+// of the boundary state's dual Jacobian (the 4 interior variables, nx, ny,
+// bc_u, bc_v); each reverses the slot's Roe flux itself (a few thousand
+// slots) and its direction of the boundary state; per-slot outputs, each
+// written by one thread.  Blocks [nbblk, grid): one thread per interior
+// face node pair, Nfp consecutive threads per face, the per-face geometry
+// sums in node order.
+template <int N>
+__global__ __launch_bounds__(kAdjThreads, 2) void adj_pair_kernel(AdjArgs a, Phys ph,
+                                                                const int *__restrict__ fmask, int nbblk) {
+  constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
+  constexpr int FPB = kAdjThreads / Nfp;  // faces per workgroup visit
+  __shared__ double sG[4][kAdjThreads];
+  __shared__ int sFm[NF];
+  const int t = threadIdx.x;
+  if (t < NF) sFm[t] = fmask[t];
+  __syncthreads();
+  if ((int)blockIdx.x < nbblk) {
+    const int n = a.nbedges * Nfp * 8;
+    for (int it = (int)blockIdx.x * kAdjThreads + t; it < n; it += nbblk * kAdjThreads) {
+      const int dir = it & 7, bi = it >> 3;
+      const int b = bi / Nfp, i = bi - b * Nfp;
+      const size_t c = a.b2c[b];
+      const int f = a.bnum[b], m = f * Nfp + i;
+      const int code = a.fmap[c * 3 + f];
+      if (code >= 0) continue;  // not a boundary face (never for a valid mesh)
+      const size_t slot = c * NF + m;
+      const double4 q4 = *reinterpret_cast<const double4 *>(a.w + (c * Np + sFm[m]) * 4);
+      const double4 p4 = *reinterpret_cast<const double4 *>(a.psi + slot * 4);
+      const double nx = a.nx[slot], ny = a.ny[slot], fs = a.fscale[slot];
+      double mQ[4] = {q4.x, q4.y, q4.z, q4.w}, pQ[4];
+      boundary_state(ph, -code - 1, mQ, nx, ny, pQ);  // the forward's exterior state
+      const double psi[4] = {p4.x, p4.y, p4.z, p4.w};
+      double bm[4] = {0, 0, 0, 0}, bp[4] = {0, 0, 0, 0}, bnx = 0, bny = 0, bfs = 0;
+      roe_vjp(ph.gam, mQ, pQ, nx, ny, fs, psi, bm, bp, bnx, bny, bfs);
+      Dual q[4], qb[4];
+      for (int v = 0; v < 4; v++) q[v] = Dual(mQ[v], dir == v ? 1.0 : 0.0);
+      const Dual dnx(nx, dir == 4 ? 1.0 : 0.0), dny(ny, dir == 5 ? 1.0 : 0.0);
+      const Dual du(ph.bc_u, dir == 6 ? 1.0 : 0.0), dv(ph.bc_v, dir == 7 ? 1.0 : 0.0);
+      boundary_t<Dual>(ph, -code - 1, du, dv, q, dnx, dny, qb);
+      double s = 0.0;
+      for (int v = 0; v < 4; v++) s += bp[v] * qb[v].d;
+      if (dir < 4) {
+        const double bmd = dir == 0 ? bm[0] : (dir == 1 ? bm[1] : (dir == 2 ? bm[2] : bm[3]));
+        a.gmq[slot * 4 + dir] = bmd + s;
+      } else if (dir == 4) {
+        a.g_nx[slot] += a.scale * (bnx + s);
+      } else if (dir == 5) {
+        a.g_ny[slot] += a.scale * (bny + s);
+      } else if (dir == 6) {
+        a.g_bcu[slot] += a.scale * s;
+        a.g_fs[slot] += a.scale * bfs;
+      } else {
+        a.g_bcv[slot] += a.scale * s;
+      }
+    }
+    return;
+  }
+  const int fl = t / Nfp, i = t - fl * Nfp;
+  const int nblk = (int)gridDim.x - nbblk;
+  for (size_t base = (size_t)((int)blockIdx.x - nbblk) * FPB; base < (size_t)a.npairs;
+       base += (size_t)nblk * FPB) {
+    const size_t e = base + fl;
+    const bool act = fl < FPB && e < (size_t)a.npairs;
+    int fo = 0, fp = 0;
+    double gnx = 0.0, gny = 0.0, gfo = 0.0, gfp = 0.0;
+    if (act) {
+      fo = a.pairs[2 * e];
+      fp = a.pairs[2 * e + 1];
+      const int co = fo / 3, cp = fp / 3;
+      const int mo = (fo - 3 * co) * Nfp + i, mp = (fp - 3 * cp) * Nfp + (Nfp - 1 - i);
+      const size_t so = (size_t)co * NF + mo, sp = (size_t)cp * NF + mp;
+      const double4 q4 = *reinterpret_cast<const double4 *>(a.w + ((size_t)co * Np + sFm[mo]) * 4);
+      const double4 e4 = *reinterpret_cast<const double4 *>(a.w + ((size_t)cp * Np + sFm[mp]) * 4);
+      const double4 po = *reinterpret_cast<const double4 *>(a.psi + so * 4);
+      const double4 pp = *reinterpret_cast<const double4 *>(a.psi + sp * 4);
+      const double nx = a.nx[so], ny = a.ny[so], fso = a.fscale[so], fsp = a.fscale[sp];
+      const double mQ[4] = {q4.x, q4.y, q4.z, q4.w}, pQ[4] = {e4.x, e4.y, e4.z, e4.w};
+      const double o[4] = {fso * po.x - fsp * pp.x, fso * po.y - fsp * pp.y, fso * po.z - fsp * pp.z,
+                           fso * po.w - fsp * pp.w};
+      double bm[4] = {0, 0, 0, 0}, bp[4] = {0, 0, 0, 0}, bnx = 0, bny = 0, bfs = 0, R[4];
+      roe_vjp(ph.gam, mQ, pQ, nx, ny, 1.0, o, bm, bp, bnx, bny, bfs, R);
+      *reinterpret_cast<double4 *>(a.gmq + so * 4) = make_double4(bm[0], bm[1], bm[2], bm[3]);
+      *reinterpret_cast<double4 *>(a.gmq + sp * 4) = make_double4(bp[0], bp[1], bp[2], bp[3]);
+      gnx = bnx;
+      gny = bny;
+      gfo = po.x * R[0] + po.y * R[1] + po.z * R[2] + po.w * R[3];
+      gfp = -(pp.x * R[0] + pp.y * R[1] + pp.z * R[2] + pp.w * R[3]);
+    }
+    sG[0][t] = gnx; sG[1][t] = gny; sG[2][t] = gfo; sG[3][t] = gfp;
+    __syncthreads();
+    if (act && i == 0) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      for (int j = 0; j < Nfp; j++) {
+        s0 += sG[0][t + j]; s1 += sG[1][t + j]; s2 += sG[2][t + j]; s3 += sG[3][t + j];
+      }
+      a.g_nxf[fo] += a.scale * s0;
+      a.g_nyf[fo] += a.scale * s1;
+      a.g_fsf[fo] += a.scale * s2;
+      a.g_fsf[fp] += a.scale * s3;
+    }
+    __syncthreads();
+  }
+}
+
 // One thread per node, whole cells per workgroup; mu, the metrics and the
 // nodal Euler fluxes of the cells are staged in LDS, so each flux is
-// evaluated once per node.
-template <int N>
+// evaluated once per node.  PAIR (face-pair form): the face contributions
+// are the own slots' gmq alone, and the cells' psi for the next launch is
+// formed from out.
+template <int N, bool PAIR>
 __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops,
                                 const int *__restrict__ node_slots) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   constexpr int CPB = kAdjThreads / Np, NT = CPB * Np;
   __shared__ double sD[2 * Np * Np];
   __shared__ double sMu[4][NT], sMet[4][NT], sFG[8][NT];
+  __shared__ double sL[PAIR ? Np * NF : 1];
   for (int k = threadIdx.x; k < 2 * Np * Np; k += kAdjThreads) sD[k] = ops[k];
+  if (PAIR)
+    for (int k = threadIdx.x; k < Np * NF; k += kAdjThreads) sL[k] = ops[2 * Np * Np + k];
   const double *Drw = sD, *Dsw = sD + Np * Np;
   const int t = threadIdx.x;
   const int cl = t / Np, i = t - cl * Np;
@@ -481,6 +647,7 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phy
     }
     __syncthreads();
     double grx = 0, gsx = 0, gry = 0, gsy = 0;
+    double o[4] = {0, 0, 0, 0};
     if (act) {
       const int c0 = cl * Np;
       // transposed weak derivatives with the metric terms
@@ -498,13 +665,15 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phy
       double lam[4] = {0, 0, 0, 0};
       euler_flux_vjp(ph.gam, wi, phF, phG, lam);
       // face contributions that read this node: its own slots' interior
-      // side (gmq) and the facing slots' exterior side (gpq)
+      // side (gmq) and the facing slots' exterior side (gpq; in the
+      // face-pair form gmq already holds both)
       const int msl[2] = {ms0, ms1};
       for (int s = 0; s < 2; s++) {
         const int m = msl[s];
         if (m < 0) continue;
         const double4 gm = *reinterpret_cast<const double4 *>(a.gmq + (c * NF + m) * 4);
         lam[0] += gm.x; lam[1] += gm.y; lam[2] += gm.z; lam[3] += gm.w;
+        if (PAIR) continue;
         const int code = a.fmap[c * 3 + m / Nfp];
         if (code >= 0) {
           const size_t nb = code >> 2;
@@ -515,7 +684,6 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phy
       }
       const double4 x4 = a.x0 ? *reinterpret_cast<const double4 *>(a.x0 + it * 4) : make_double4(0, 0, 0, 0);
       const double xv[4] = {x4.x, x4.y, x4.z, x4.w};
-      double o[4];
       for (int v = 0; v < 4; v++) o[v] = (a.x0 ? a.e0 * xv[v] : 0.0) + a.scale * lam[v];
       *reinterpret_cast<double4 *>(a.out + it * 4) = make_double4(o[0], o[1], o[2], o[3]);
       if (a.out2) {
@@ -563,6 +731,24 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phy
         a.g_sx[c] += a.scale * s1;
         a.g_ry[c] += a.scale * s2;
         a.g_sy[c] += a.scale * s3;
+      }
+    }
+    if (PAIR) {
+      // psi of the next launch, whose mu is this launch's out: -LIFT^T out
+      // per face slot of the block's cells (as adj_psi_kernel)
+      __syncthreads();  // the metric cotangents are done with sMu
+      if (act) {
+        sMu[0][t] = o[0]; sMu[1][t] = o[1]; sMu[2][t] = o[2]; sMu[3][t] = o[3];
+      }
+      __syncthreads();
+      for (int k = t; k < nc * NF; k += kAdjThreads) {
+        const int cc = k / NF, m = k - cc * NF;
+        double p[4] = {0, 0, 0, 0};
+        for (int j = 0; j < Np; j++) {
+          const double l = sL[j * NF + m];
+          for (int v = 0; v < 4; v++) p[v] -= l * sMu[v][cc * Np + j];
+        }
+        *reinterpret_cast<double4 *>(a.psi + ((base + cc) * NF + m) * 4) = make_double4(p[0], p[1], p[2], p[3]);
       }
     }
   }
@@ -692,17 +878,43 @@ static int cell_blocks(size_t ncells, int cells_per_block) {
 
 template <int N>
 static hipError_t adj_n(const Launch &L, const AdjArgs &a, const int *node_slots) {
+  hipError_t e = hipSuccess;
+  if (a.pairs) {
+    // boundary blocks first, so that their longer threads start early
+    const int nbblk = a.nbedges > 0 ? (a.nbedges * Ord<N>::Nfp * 8 + kAdjThreads - 1) / kAdjThreads : 0;
+    const int fpb = kAdjThreads / Ord<N>::Nfp;
+    const int npb = (a.npairs + fpb - 1) / fpb;
+    if (nbblk + npb > 0)
+      hipLaunchKernelGGL(adj_pair_kernel<N>, dim3(nbblk + npb), dim3(kAdjThreads), 0, L.stream, a, L.ph,
+                         L.fmask, nbblk);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL((adj_cell_kernel<N, true>), dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::Np)),
+                       dim3(kAdjThreads), 0, L.stream, a, L.ph, L.ops, node_slots);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(adj_face_kernel<N>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::NF)),
                      dim3(kAdjThreads), 0, L.stream, a, L.ph, L.ops);
-  hipError_t e = hipGetLastError();
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.nbedges > 0) {
     const int nb = (a.nbedges * Ord<N>::Nfp * 8 + 255) / 256;
     hipLaunchKernelGGL(adj_boundary_kernel<N>, dim3(nb), dim3(256), 0, L.stream, a, L.ph, L.fmask);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(adj_cell_kernel<N>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::Np)),
+  hipLaunchKernelGGL((adj_cell_kernel<N, false>), dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::Np)),
                      dim3(kAdjThreads), 0, L.stream, a, L.ph, L.ops, node_slots);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_psi(const Launch &L, const AdjArgs &a) {
+  if (a.ncells == 0) return hipSuccess;
+  switch (L.N) {
+    case 1: hipLaunchKernelGGL(adj_psi_kernel<1>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<1>::NF)), dim3(kAdjThreads), 0, L.stream, a, L.ops); break;
+    case 2: hipLaunchKernelGGL(adj_psi_kernel<2>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<2>::NF)), dim3(kAdjThreads), 0, L.stream, a, L.ops); break;
+    case 3: hipLaunchKernelGGL(adj_psi_kernel<3>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<3>::NF)), dim3(kAdjThreads), 0, L.stream, a, L.ops); break;
+    case 4: hipLaunchKernelGGL(adj_psi_kernel<4>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<4>::NF)), dim3(kAdjThreads), 0, L.stream, a, L.ops); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

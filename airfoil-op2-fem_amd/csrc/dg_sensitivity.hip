@@ -98,6 +98,11 @@ struct dg_sens {
   // device geometry and maps
   double *cell_xy, *nodeX, *nodeY, *x, *y, *xr, *yr, *xs, *ys, *rx, *ry, *sx, *sy, *nx, *ny, *fs, *J;
   int *fmap, *node_slots, *cell_off, *tile_off;
+  // interior faces once each, (c * 3 + f) of both sides, lower cell first,
+  // and the flux cotangents psi [K][NF][4] of the face-pair reverse passes
+  int *pairs;
+  int npairs = 0;
+  double *psi;
   dg::Phys *phys_d;
   unsigned long long *dt1;
   double *tdev;
@@ -401,6 +406,20 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   s->nx = b.alloc<double>(K * NF); s->ny = b.alloc<double>(K * NF); s->fs = b.alloc<double>(K * NF);
   s->J = b.alloc<double>(K);
   s->fmap = b.alloc<int>(fmap.size());
+  std::vector<int> pairs;
+  for (size_t k = 0; k < K; k++)
+    for (int f = 0; f < 3; f++) {
+      const int code = fmap[k * 3 + f];
+      if (code < 0) continue;
+      const size_t nb = (size_t)(code >> 2);
+      if (nb > k || (nb == k && (code & 3) > f)) {
+        pairs.push_back((int)(k * 3 + f));
+        pairs.push_back((int)(nb * 3 + (code & 3)));
+      }
+    }
+  s->npairs = (int)(pairs.size() / 2);
+  s->pairs = b.alloc<int>(pairs.size());
+  s->psi = b.alloc<double>(FT);
   s->node_slots = b.alloc<int>(2 * Np);
   s->cell_off = b.alloc<int>(2); s->tile_off = b.alloc<int>(2);
   s->phys_d = b.alloc<dg::Phys>(1);
@@ -452,6 +471,7 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
     if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
   };
   up(s->fmap, fmap.data(), sizeof(int) * fmap.size());
+  up(s->pairs, pairs.data(), sizeof(int) * pairs.size());
   up(s->node_slots, slots.data(), sizeof(int) * slots.size());
   up(s->cell_off, coff, sizeof(coff));
   up(s->tile_off, toff, sizeof(toff));
@@ -771,6 +791,13 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   a.g_nxf = s->g_nxf; a.g_nyf = s->g_nyf; a.g_fsf = s->g_fsf;
   a.compact = 1;
   a.nbedges = s->mesh.nbedges; a.b2c = s->d_b2c; a.bnum = s->d_bnum;
+  // face-pair form: psi of the first launch from lam here, of every later
+  // launch from the cell pass before it
+  a.pairs = s->pairs; a.npairs = s->npairs; a.psi = s->psi;
+  if (a.pairs) {
+    a.mu = s->lam;
+    HIPCHECK(c, dg::launch_adj_psi(L, a));
+  }
   for (int sg = nck - 1; sg >= 0; sg--) {
     const int n0 = sg * ckpt_every, n1 = std::min(s->nsteps, n0 + ckpt_every);
     const int b = (nck - 1 - sg) & 1;

@@ -47,6 +47,17 @@ def host_cores():
     return max(1, min(16, n))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def load_traffic(kernel):
     """Measured HBM bytes per launch from the committed rocprofv3 PMC summary."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
@@ -352,7 +363,8 @@ def main():
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=30,
+                    help="SSPRK3 steps of the CPU baseline sample (about 10 s on 16 host threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--geometry", choices=["node", "cell"], default="node",
@@ -474,14 +486,20 @@ def main():
         o.run(case, st, args.cpu_steps, cfl=args.cfl, nthreads=cores, ktime=kt_cpu)
         c1 = time.perf_counter()
         cpu_val = dof * 3 * args.cpu_steps / (c1 - c0)
+        # one thread, one step of the same state, for the scaling of the port
+        st1 = o.new_state(case, q_start)
+        s0 = time.perf_counter()
+        o.run(case, st1, 1, cfl=args.cfl, nthreads=1)
+        s1 = time.perf_counter()
         cpu = {"value": cpu_val, "unit": "DoF-updates/s per RK stage", "cores": cores, "kind": "port",
                "sample": f"oracle OpenMP restatement (reference op2 openmp loop structure), same "
                          f"{n}x{n} mesh and state, {args.cpu_steps} SSPRK3 steps incl. calc_dt, "
                          f"{c1 - c0:.1f}s wall",
                "kernel_s": dict(zip(["get_neighbour_q", "get_bedge_q", "euler_rhs", "set_workingQ",
-                                     "update_Q", "calc_dt"], [round(float(v), 3) for v in kt_cpu]))}
-        print(f"[bench] cpu baseline: {cpu_val:.4e} DoF-updates/s per RK stage on {cores} threads",
-              file=sys.stderr, flush=True)
+                                     "update_Q", "calc_dt"], [round(float(v), 3) for v in kt_cpu])),
+               "value_1thread": dof * 3 / (s1 - s0), "cpu_model": cpu_model()}
+        print(f"[bench] cpu baseline: {cpu_val:.4e} DoF-updates/s per RK stage on {cores} threads, "
+              f"{cpu['value_1thread']:.4e} on 1 ({cpu['cpu_model']})", file=sys.stderr, flush=True)
 
     out = {"metric": "DoF-updates/s per RK stage (FP64, N=3)", "value": value,
            "unit": "DoF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -56,6 +56,27 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+// Copies nch (>= 1) 16-byte chunks from global src to LDS dst with
+// direct-to-LDS loads (global_load_lds_dwordx4: 16 bytes per lane, one
+// contiguous KiB per instruction of the wave, no registers holding the
+// data).  dst is the same for all lanes of the wave (it goes to M0) and
+// holds 64 ceil(MAXCH / 64) chunks: every instruction is issued with all
+// lanes, those past nch re-reading the last chunk into their own slots, so
+// the number of loads in flight is the same on every path and the
+// compiler's counter waits for older loads stay exact.  The data is in LDS
+// once the wave's vector-memory counter has drained (s_waitcnt vmcnt(0)).
+template <int MAXCH>
+__device__ __forceinline__ void copy_to_lds(const double *src, double *dst, int nch, int lane) {
+#pragma unroll
+  for (int r = 0; r < (MAXCH + 63) / 64; r++) {
+    const int ch = min(64 * r + lane, nch - 1);
+    __builtin_amdgcn_global_load_lds((glb_void_t *)(src + 2 * ch), (lds_void_t *)(dst + 128 * r), 16, 0, 0);
+  }
+}
+
 // The lane index passed through an empty asm statement inside the tile loop.
 // A lane's (cell, node) items are the same on every tile, so the compiler
 // hoists their decomposition and addresses out of the loop; held for the
@@ -91,10 +112,22 @@ template <int N>
 constexpr int mfma_fl_group() {  // doubles per group in the face-flux tile
   return kColStride * 4 * ((Ord<N>::NF + 3) / 4) + kGroupSkew;
 }
+// The tile's state records (16 cells x Np x 4) are copied into LDS by the
+// direct-to-LDS loads issued at the start of the tile, past the face-flux
+// tile (which the face phase writes meanwhile); the Euler-flux phase reads
+// them there, then writes the F and G tiles over the same memory.
+template <int N>
+constexpr int mfma_wtile_off() {
+  return 4 * mfma_fl_group<N>();
+}
+template <int N>
+constexpr int mfma_wtile_doubles() {  // whole KiB per copy instruction (copy_to_lds)
+  return 128 * ((kMfmaTile * Ord<N>::Np * 2 + 63) / 64);
+}
 template <int N>
 constexpr int mfma_tile_doubles() {
-  constexpr int fg = 2 * 4 * mfma_fg_group<N>(), fl = 4 * mfma_fl_group<N>();
-  return fg > fl ? fg : fl;
+  constexpr int fg = 2 * 4 * mfma_fg_group<N>(), fw = mfma_wtile_off<N>() + mfma_wtile_doubles<N>();
+  return fg > fw ? fg : fw;
 }
 template <int N, int EPI>
 constexpr int mfma_lds_doubles() {
@@ -128,8 +161,10 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   extern __shared__ double lds_dyn[];
   __shared__ int sFm[NF];
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform in the wave
   double *sA = lds_dyn + wave * mfma_lds_doubles<N, EPI>();
+  double *sW = sA + mfma_wtile_off<N>();  // the tile's state records (direct-to-LDS copy)
   double *sCfs = sA + mfma_tile_doubles<N>();  // stage 2: fscale at [cell][face-node slot < Np]
   for (int i = lane; i < mfma_tile_doubles<N>(); i += 64) sA[i] = 0.0;
   if (threadIdx.x < NF) sFm[threadIdx.x] = fmask[threadIdx.x];
@@ -229,6 +264,10 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     }
     __syncthreads();  // the previous tile's volume products are done with the LDS tiles
 
+    // the tile's state records go to LDS while the face phase runs; the
+    // Euler-flux phase reads them there instead of waiting on its own loads
+    copy_to_lds<TE * Np * 2>(a.w + (size_t)e0 * Np * 4, sW, active ? ne * Np * 2 : 1, lane);
+
     // ---- Roe fluxes per (cell, face node), written in the A-operand layout
     {
       const int nrt = ne * NF;
@@ -308,15 +347,25 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     {
       const int nit = ne * Np;
       const int lr = kItemsInLoop ? lane_in_loop(lane) : lane;
+      // The copied state records: every item's record is read (into
+      // registers) before any F or G store, which reuse this memory; a
+      // wave's LDS instructions execute in order.
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      double2 wq[NIT][2];
+#pragma unroll
+      for (int j = 0; j < NIT; j++) {
+        const int it = lr + 64 * j;
+        const double2 *src = reinterpret_cast<const double2 *>(sW + (it < nit ? it : 0) * 4);
+        wq[j][0] = src[0];
+        wq[j][1] = src[1];
+      }
 #pragma unroll
       for (int j = 0; j < NIT; j++) {
         const int it = lr + 64 * j;
         if (it >= nit) continue;
         const int e = it / Np, k = it - e * Np;
         double F[4], Gv[4];
-        const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)(e0 + e) * Np + k) * 4);
-        const double2 q01 = src[0], q23 = src[1];
-        double qv[4] = {q01.x, q01.y, q23.x, q23.y};
+        double qv[4] = {wq[j][0].x, wq[j][0].y, wq[j][1].x, wq[j][1].y};
         euler_flux(ph.gam, qv, F, Gv);
         if constexpr (CG) {  // contravariant fluxes
           const double2 *mt = reinterpret_cast<const double2 *>(a.cmet + (size_t)(e0 + e) * 4);

@@ -37,9 +37,11 @@ Launch launch_of(const dg_ctx *c) {
   L.ops = c->d_ops;
   L.fmask = c->d_fmask;
   L.ph = {c->gam, c->bc_r, c->bc_u, c->bc_v, c->bc_e, c->bc_p};
-  uint64_t mask = 0;
+  uint64_t mask = 0, packed = 0;
   for (int f : c->FMASK) mask |= (1ull << f);
+  for (size_t m = 0; m < c->FMASK.size() && m < 16; m++) packed |= (uint64_t)(c->FMASK[m] & 15) << (4 * m);
   L.facenodes = mask;
+  L.fmask4 = packed;
   L.cus = c->cus;
   L.max_blocks = c->max_blocks;
   L.variant = c->variant;

@@ -23,11 +23,15 @@ struct Launch {
   const int *fmask;    // device FMASK [NF]
   Phys ph;
   uint64_t facenodes;  // bit j set when node j lies on a face
+  uint64_t fmask4;     // FMASK[m] in bits 4m..4m+3 (entries < 16, NF <= 15 for N <= 4)
   int cus;             // compute units of the device
   int max_blocks;      // optional cap on persistent grids (0: none)
   int variant;         // stage kernel: 0 MFMA (wave tiles), 1 LDS/VALU (workgroup tiles)
-  int wpb = 1;         // matrix-core kernel: waves (16-cell tiles) per workgroup, 1 / 2 / 4
-  int walk = 0;        // matrix-core kernel: 0 XCD-chunked block walk, 1 round-robin walk
+  // matrix-core kernel launch shape; the default is the autotuner's choice on
+  // the 724x724 vortex (N=3), where one-wave workgroups run the stage-2
+  // kernel ~45% slower
+  int wpb = 2;         // waves (16-cell tiles) per workgroup, 1 / 2 / 4
+  int walk = 1;        // 0 XCD-chunked block walk, 1 round-robin walk
 };
 
 // Arguments of the stage kernels.

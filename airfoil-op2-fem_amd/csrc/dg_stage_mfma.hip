@@ -95,22 +95,27 @@ __device__ __forceinline__ int lane_in_loop(int l) {
 // LDS per wave (doubles): the F and G tiles [4 groups][Np columns][16 rows]
 // (the face-flux tile [4][4 KF columns][16 rows] reuses them); the stage-2
 // kernel also keeps the fscale values its CFL quantity reads, [16 cells][Np].
-// A column holds its 16 rows in a slot of kColStride doubles and a group
-// starts kGroupSkew doubles past the end of the previous one.  The flux
-// writes put the nodes of one cell (same row, consecutive columns) on
+// A column holds its 16 rows in a slot of kColStride doubles and an F or G
+// group starts kGroupSkew doubles past the end of the previous one.  The
+// flux writes put the nodes of one cell (same row, consecutive columns) on
 // neighbouring lanes; with 16-double columns those all fall in one bank pair
 // of the 32 four-byte banks (a 10- or 12-way conflict on every store).  An
 // 18-double slot moves consecutive columns two banks apart and the skew
 // separates cells of different groups, while the operand reads (16 rows of
-// one column per 16 lanes) stay within one slot.
-constexpr int kColStride = 18, kGroupSkew = 2;
+// one column per 16 lanes) stay within one slot.  The face-flux tile uses
+// 17-double slots and no skew (consecutive columns one bank pair apart, at
+// most 2-way conflicts between neighbouring cells), so that it and the copy
+// of the tile's state records fit beside each other within the F and G
+// tiles' footprint (at N=3 a stage-2 wave needs 12,928 bytes, twelve waves
+// per compute unit 152 KiB of its 160).
+constexpr int kColStride = 18, kGroupSkew = 2, kFaceColStride = 17;
 template <int N>
 constexpr int mfma_fg_group() {  // doubles per group in the F and G tiles
   return kColStride * Ord<N>::Np + kGroupSkew;
 }
 template <int N>
 constexpr int mfma_fl_group() {  // doubles per group in the face-flux tile
-  return kColStride * 4 * ((Ord<N>::NF + 3) / 4) + kGroupSkew;
+  return kFaceColStride * 4 * ((Ord<N>::NF + 3) / 4);
 }
 // The tile's state records (16 cells x Np x 4) are copied into LDS by the
 // direct-to-LDS loads issued at the start of the tile, past the face-flux
@@ -141,7 +146,7 @@ constexpr int mfma_lds_doubles() {
 // the end only skip the work.
 template <int N, bool GATHER, int EPI, bool CG>
 __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const double *__restrict__ ops,
-                                                       const int *__restrict__ fmask, Phys ph0,
+                                                       unsigned long long fmask4, Phys ph0,
                                                        unsigned long long facenodes, int walk) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   constexpr int KP = (Np + 3) / 4;  // K-steps over source nodes
@@ -151,15 +156,18 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   constexpr int NRT = (TE * NF + 63) / 64;  // Roe-flux items per lane
   constexpr int GS = mfma_fg_group<N>();    // doubles per group in the F and G tiles
   constexpr int GL = mfma_fl_group<N>();    // doubles per group in the face-flux tile
-  constexpr int CS = kColStride;
+  constexpr int CS = kColStride, CSL = kFaceColStride;
   constexpr bool kItemsInLoop = EPI == EPI_STAGE2 || N >= 4;  // see lane_in_loop
   static_assert(Np <= 16, "one 16-row output tile per cell group");
   // per wave: F tiles [G][Np][CS] then G tiles [G][Np][CS] (column k of a
   // group holds its rows cell + 4*var; the padding columns k >= Np of the
   // last K-step are not stored, the lanes supply them as zeros); the
   // face-flux tiles [G][4 KF][CS] use the same memory before the Euler fluxes.
+  // No static LDS: a one-wave workgroup's dynamic share is sized so that 12
+  // of them fit a compute unit's 160 KiB in 512-byte allocation granules;
+  // FMASK comes packed in a kernel argument instead.
   extern __shared__ double lds_dyn[];
-  __shared__ int sFm[NF];
+  auto fmask_of = [&](int m) -> int { return (int)((fmask4 >> (4 * m)) & 15ull); };
 
   const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform in the wave
@@ -167,7 +175,6 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   double *sW = sA + mfma_wtile_off<N>();  // the tile's state records (direct-to-LDS copy)
   double *sCfs = sA + mfma_tile_doubles<N>();  // stage 2: fscale at [cell][face-node slot < Np]
   for (int i = lane; i < mfma_tile_doubles<N>(); i += 64) sA[i] = 0.0;
-  if (threadIdx.x < NF) sFm[threadIdx.x] = fmask[threadIdx.x];
 
   // B operands: lane l holds B[k = 4*ks + l/16][j = l%16]
   const int jj = lane & 15, kk = lane >> 4;
@@ -291,10 +298,10 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         // own and exterior trace loads issued together, before any branch:
         // a boundary slot reads its own record as a stand-in exterior, which
         // boundary_state then replaces
-        const size_t own = cell * Np + sFm[m];
+        const size_t own = cell * Np + fmask_of(m);
         const bool interior = GATHER && code[j] >= 0;
         const size_t nrec =
-            interior ? (size_t)(code[j] >> 2) * Np + sFm[(code[j] & 3) * Nfp + (Nfp - 1 - i)] : own;
+            interior ? (size_t)(code[j] >> 2) * Np + fmask_of((code[j] & 3) * Nfp + (Nfp - 1 - i)) : own;
         const double2 *ms = reinterpret_cast<const double2 *>(a.w + own * 4);
         const double2 *ps = GATHER ? reinterpret_cast<const double2 *>(a.w + nrec * 4)
                                    : reinterpret_cast<const double2 *>(a.ext + gm * 4);
@@ -312,7 +319,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         double fl[4];
         if (EPI == EPI_STAGE2 && m < Np) sCfs[e * Np + m] = fs;
         roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
-        const int off = (e >> 2) * GL + CS * m + (e & 3);
+        const int off = (e >> 2) * GL + CSL * m + (e & 3);
 #pragma unroll
         for (int v = 0; v < 4; v++) sL[off + 4 * v] = fl[v];
       }
@@ -320,7 +327,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         constexpr int PADF = 4 * KF - NF;
         for (int it = lane; it < ne * PADF; it += 64) {
           const int e = it / PADF, m = NF + (it - e * PADF);
-          const int off = (e >> 2) * GL + CS * m + (e & 3);
+          const int off = (e >> 2) * GL + CSL * m + (e & 3);
 #pragma unroll
           for (int v = 0; v < 4; v++) sL[off + 4 * v] = 0.0;
         }
@@ -335,7 +342,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       acc[g] = dbl4{0.0, 0.0, 0.0, 0.0};
       if (4 * g >= ne) continue;
 #pragma unroll
-      for (int ks = 0; ks < KF; ks++) acc[g] = mfma16x16x4(sL[g * GL + CS * (4 * ks + kk) + jj], bL[ks], acc[g]);
+      for (int ks = 0; ks < KF; ks++) acc[g] = mfma16x16x4(sL[g * GL + CSL * (4 * ks + kk) + jj], bL[ks], acc[g]);
     }
     __syncthreads();  // the Euler fluxes overwrite the face-flux tiles
 
@@ -539,7 +546,7 @@ static hipError_t launch_t(const Launch &L, const StageArgs &a) {
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) return hipSuccess;
   hipLaunchKernelGGL((rhs_mfma_kernel<N, GATHER, EPI, CG>), dim3(grid), dim3(64 * wpb), lds, L.stream, a,
-                     L.ops, L.fmask, L.ph, (unsigned long long)L.facenodes, L.walk);
+                     L.ops, (unsigned long long)L.fmask4, L.ph, (unsigned long long)L.facenodes, L.walk);
   return hipGetLastError();
 }
 

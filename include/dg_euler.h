@@ -279,7 +279,8 @@ int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int 
  * wpb = waves (16-cell tiles) per workgroup, 1, 2 or 4 (16 / 32 / 64
  * elements per workgroup); walk = 0 XCD-chunked (the workgroups of one XCD
  * walk one contiguous range of tiles), 1 round-robin; max_blocks caps the
- * persistent grid (0: occupancy-limited).  The shape never changes results. */
+ * persistent grid (0: occupancy-limited).  The shape never changes results.
+ * Default: wpb 2, walk 1, max_blocks 0. */
 int dg_solver_set_launch(dg_solver *s, int wpb, int walk, int max_blocks);
 int dg_solver_launch_config(const dg_solver *s, int *wpb, int *walk, int *max_blocks);
 /* Geometry the fused stage kernels read (no reference counterpart).

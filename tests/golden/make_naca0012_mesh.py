@@ -8,7 +8,7 @@ which is all the solver reads from it (edges and boundary classes are
 rebuilt from the triangles, as ``tools/vtk2CGNS.cpp:112-199`` and
 ``src/load_mesh.cpp:147-172`` do).  The tests write the fixture back out as a
 triangle-only legacy-VTK file and read it with the library's reader, so the
-reference scenario runs on the GPU box, where the reference tree is absent.
+reference scenario also runs where the reference tree is absent.
 
 Run in a container where the reference tree is mounted at /root/reference:
 

@@ -27,11 +27,11 @@ struct Launch {
   int cus;             // compute units of the device
   int max_blocks;      // optional cap on persistent grids (0: none)
   int variant;         // stage kernel: 0 MFMA (wave tiles), 1 LDS/VALU (workgroup tiles)
-  // matrix-core kernel launch shape; the default is the autotuner's choice on
-  // the 724x724 vortex (N=3), where one-wave workgroups run the stage-2
-  // kernel ~45% slower
-  int wpb = 2;         // waves (16-cell tiles) per workgroup, 1 / 2 / 4
-  int walk = 1;        // 0 XCD-chunked block walk, 1 round-robin walk
+  // matrix-core kernel launch shape (the autotuner picks per mesh; one-wave
+  // workgroups with the XCD-chunked walk are within 1% of the best shape on
+  // the 724x724 vortex and the best on 65k-cell meshes)
+  int wpb = 1;         // waves (16-cell tiles) per workgroup, 1 / 2 / 4
+  int walk = 0;        // 0 XCD-chunked block walk, 1 round-robin walk
 };
 
 // Arguments of the stage kernels.

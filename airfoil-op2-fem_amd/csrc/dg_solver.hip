@@ -64,7 +64,7 @@ struct dg_solver {
   hipGraphExec_t gexec = nullptr;
   double gcfl = -1.0;
   // launch shape of the matrix-core stage kernel (autotuned or set)
-  int wpb = 2, walk = 1, max_blocks = 0;  // as dg::Launch's defaults
+  int wpb = 1, walk = 0, max_blocks = 0;  // as dg::Launch's defaults
   // stage kernels read the per-cell metric record (DG_GEOM_CELL) or the
   // per-node factors, normals and fscale (DG_GEOM_NODE)
   bool cell_geometry = false;

@@ -134,9 +134,15 @@ constexpr int mfma_tile_doubles() {
   constexpr int fg = 2 * 4 * mfma_fg_group<N>(), fw = mfma_wtile_off<N>() + mfma_wtile_doubles<N>();
   return fg > fw ? fg : fw;
 }
-template <int N, int EPI>
+// The stage-2 kernel's CFL quantity reads fscale in its epilogue (from the
+// fscale array, or formed from the cell's metric record on the per-cell
+// geometry) rather than from a table its face phase would leave in LDS: with
+// that table (12.6 KB per wave) one-wave workgroups ran the stage-2 kernel
+// ~40% slower than at 11.4 KB (591 vs 429 us on the 724x724 vortex), as if
+// fewer of them were resident than the occupancy query reports.
+template <int N, int EPI, bool CG>
 constexpr int mfma_lds_doubles() {
-  return mfma_tile_doubles<N>() + (EPI == EPI_STAGE2 ? kMfmaTile * Ord<N>::Np : 0);
+  return mfma_tile_doubles<N>();
 }
 
 // A workgroup is 1, 2 or 4 waves (16, 32 or 64 cells per workgroup visit);
@@ -171,9 +177,8 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
 
   const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform in the wave
-  double *sA = lds_dyn + wave * mfma_lds_doubles<N, EPI>();
+  double *sA = lds_dyn + wave * mfma_lds_doubles<N, EPI, CG>();
   double *sW = sA + mfma_wtile_off<N>();  // the tile's state records (direct-to-LDS copy)
-  double *sCfs = sA + mfma_tile_doubles<N>();  // stage 2: fscale at [cell][face-node slot < Np]
   for (int i = lane; i < mfma_tile_doubles<N>(); i += 64) sA[i] = 0.0;
 
   // B operands: lane l holds B[k = 4*ks + l/16][j = l%16]
@@ -317,7 +322,6 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
           boundary_state(ph, -code[j] - 1, mQ, nxv, nyv, pQ);
         }
         double fl[4];
-        if (EPI == EPI_STAGE2 && m < Np) sCfs[e * Np + m] = fs;
         roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
         const int off = (e >> 2) * GL + CSL * m + (e & 3);
 #pragma unroll
@@ -490,9 +494,18 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       if (EPI == EPI_STAGE2) {
         if ((facenodes >> jj) & 1ull) {
           // src/calc_dt.h:11 reads fscale at face-node index FMASK[i], i.e.
-          // at this node's index jj read as a face-node slot; the tile's Roe
-          // phase left slot jj's fscale in LDS
-          const double cv = cfl_node(ph.gam, n1sq, o, sCfs[e * Np + jj]) + 0.0;
+          // at this node's index jj read as a face-node slot (jj < Np <= NF);
+          // on the per-cell geometry formed as the face phase formed it
+          double fsq;
+          if constexpr (CG) {
+            const double2 *mt = reinterpret_cast<const double2 *>(a.cmet + (size_t)(e0 + e) * 4);
+            const double2 rxy = mt[0], sxy = mt[1];
+            double nxq, nyq;
+            cell_face_geometry(rxy.x, rxy.y, sxy.x, sxy.y, jj / Nfp, nxq, nyq, fsq);
+          } else {
+            fsq = a.fscale[(size_t)(e0 + e) * NF + jj];
+          }
+          const double cv = cfl_node(ph.gam, n1sq, o, fsq) + 0.0;
           cfl_bits = max_bits(cfl_bits, (unsigned long long)__double_as_longlong(cv));
         }
         bad |= !(isfinite(o[0]) && isfinite(o[1]) && isfinite(o[2]) && isfinite(o[3])) ||
@@ -515,18 +528,18 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
 
 // Waves per workgroup actually used for order N: 1, 2 or 4, within 64 KB of
 // LDS per workgroup.
-template <int N, int EPI>
+template <int N, int EPI, bool CG>
 static int mfma_wpb(int want) {
   int w = want >= 4 ? 4 : (want >= 2 ? 2 : 1);
-  while (w > 1 && (size_t)w * mfma_lds_doubles<N, EPI>() * sizeof(double) > 64 * 1024) w /= 2;
+  while (w > 1 && (size_t)w * mfma_lds_doubles<N, EPI, CG>() * sizeof(double) > 64 * 1024) w /= 2;
   return w;
 }
 
 template <int N, bool GATHER, int EPI, bool CG>
 static hipError_t launch_t(const Launch &L, const StageArgs &a) {
   static int per_cu[5] = {-1, -1, -1, -1, -1};  // resident workgroups per CU, by waves per workgroup
-  const int wpb = mfma_wpb<N, EPI>(L.wpb);
-  const size_t lds = (size_t)wpb * mfma_lds_doubles<N, EPI>() * sizeof(double);
+  const int wpb = mfma_wpb<N, EPI, CG>(L.wpb);
+  const size_t lds = (size_t)wpb * mfma_lds_doubles<N, EPI, CG>() * sizeof(double);
   if (per_cu[wpb] < 0) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rhs_mfma_kernel<N, GATHER, EPI, CG>, 64 * wpb, lds) !=

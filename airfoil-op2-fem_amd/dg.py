@@ -413,7 +413,7 @@ class Solver:
         """Mesh and vertex fields as a legacy-VTK ASCII file."""
         self._chk(lib.dg_solver_write_vtk(self._h, int(case), str(path).encode()))
 
-    def set_launch(self, wpb=2, walk=1, max_blocks=0):
+    def set_launch(self, wpb=1, walk=0, max_blocks=0):
         """Launch shape of the matrix-core stage kernel (results unchanged)."""
         self._chk(lib.dg_solver_set_launch(self._h, int(wpb), int(walk), int(max_blocks)))
 

@@ -280,7 +280,7 @@ int dg_solver_kernel_time(dg_solver *s, const char *kernel, double *avg_ms, int 
  * elements per workgroup); walk = 0 XCD-chunked (the workgroups of one XCD
  * walk one contiguous range of tiles), 1 round-robin; max_blocks caps the
  * persistent grid (0: occupancy-limited).  The shape never changes results.
- * Default: wpb 2, walk 1, max_blocks 0. */
+ * Default: wpb 1, walk 0, max_blocks 0. */
 int dg_solver_set_launch(dg_solver *s, int wpb, int walk, int max_blocks);
 int dg_solver_launch_config(const dg_solver *s, int *wpb, int *walk, int *max_blocks);
 /* Geometry the fused stage kernels read (no reference counterpart).

@@ -84,8 +84,15 @@ struct GeoCot {
   int compact;
   const double *nxf, *nyf, *fsf;
 };
-hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_xy, const GeoCot &g,
+// cell_rel [K][6]: (x0, y0, x1 - x0, y1 - y0, x2 - x0, y2 - y0) per cell; the
+// factors are those of init_grid_cell_rel_t (as launch_init_grid_rel writes).
+hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_rel, const GeoCot &g,
                           double *gv);
+// rx, ry, sx, sy [K][Np], nx, ny, fs [K][NF] and J [K] (node 0) of the cells
+// cell_rel, in the translation-invariant form init_grid_cell_rel_t.
+hipError_t launch_init_grid_rel(const Launch &L, int ncells, const double *cell_rel, double *rx,
+                                double *ry, double *sx, double *sy, double *nx, double *ny,
+                                double *fs, double *J);
 // partial[0..kSumBlocks) = block sums of x[i * stride + offset], i < n, in a
 // fixed order; the caller adds the partials in index order.
 constexpr int kSumBlocks = 256;

@@ -765,10 +765,41 @@ __global__ void record_dt_kernel(const unsigned long long *dt1, double cfl, doub
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = cfl / __longlong_as_double((long long)*dt1);
 }
 
+// Geometry of the sensitivity path from the per-cell records cell_rel
+// [K][6] = (x0, y0, x1 - x0, y1 - y0, x2 - x0, y2 - y0): the factors of
+// init_grid_cell_rel_t (translation-invariant rounding), J at node 0.
+template <int N>
+__global__ __launch_bounds__(128) void init_grid_rel_kernel(int ncells, const double *__restrict__ ops,
+                                                            const int *__restrict__ fmask,
+                                                            const double *__restrict__ cell_rel,
+                                                            double *rx, double *ry, double *sx,
+                                                            double *sy, double *nx, double *ny,
+                                                            double *fs, double *J) {
+  constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
+  const double *Dr = ops + 2 * Np * Np + Np * NF;
+  const double *Ds = Dr + Np * Np;
+  const double *r = Ds + Np * Np;
+  const double *s = r + Np;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < (size_t)ncells;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const double e[4] = {cell_rel[6 * c + 2], cell_rel[6 * c + 3], cell_rel[6 * c + 4],
+                         cell_rel[6 * c + 5]};
+    double a[Np], b[Np], cc[Np], dd[Np], p[NF], q[NF], f[NF], Jc;
+    init_grid_cell_rel_t<N, double>(r, s, Dr, Ds, fmask, e, a, b, cc, dd, p, q, f, Jc);
+    for (int i = 0; i < Np; i++) {
+      rx[c * Np + i] = a[i]; ry[c * Np + i] = b[i]; sx[c * Np + i] = cc[i]; sy[c * Np + i] = dd[i];
+    }
+    for (int m = 0; m < NF; m++) { nx[c * NF + m] = p[m]; ny[c * NF + m] = q[m]; fs[c * NF + m] = f[m]; }
+    J[c] = Jc;
+  }
+}
+
+// One thread per cell and absolute vertex coordinate d = 2 t + (0: x, 1: y):
+// vertex t moves e (t = 1: edge 1, t = 2: edge 2, t = 0: both, negatively).
 template <int N>
 __global__ __launch_bounds__(128) void geo_vjp_kernel(int ncells, const double *__restrict__ ops,
                                                       const int *__restrict__ fmask,
-                                                      const double *__restrict__ cell_xy, GeoCot g,
+                                                      const double *__restrict__ cell_rel, GeoCot g,
                                                       double *gv) {
   constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
   const double *Dr = ops + 2 * Np * Np + Np * NF;
@@ -779,11 +810,15 @@ __global__ __launch_bounds__(128) void geo_vjp_kernel(int ncells, const double *
   for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
        it += (size_t)gridDim.x * blockDim.x) {
     const size_t c = it / 6;
-    const int d = (int)(it - c * 6);
-    Dual v[6];
-    for (int k = 0; k < 6; k++) v[k] = Dual(cell_xy[6 * c + k], k == d ? 1.0 : 0.0);
+    const int d = (int)(it - c * 6), t = d >> 1, comp = d & 1;
+    Dual e[4];
+    for (int k = 0; k < 4; k++) {
+      const int et = 1 + (k >> 1);  // the vertex whose edge vector e[k] belongs to
+      const double seed = (k & 1) != comp ? 0.0 : (t == 0 ? -1.0 : (t == et ? 1.0 : 0.0));
+      e[k] = Dual(cell_rel[6 * c + 2 + k], seed);
+    }
     Dual rx[Np], ry[Np], sx[Np], sy[Np], nx[NF], ny[NF], fs[NF], Jc;
-    init_grid_cell_t<N, Dual>(r, s, Dr, Ds, fmask, v, rx, ry, sx, sy, nx, ny, fs, Jc);
+    init_grid_cell_rel_t<N, Dual>(r, s, Dr, Ds, fmask, e, rx, ry, sx, sy, nx, ny, fs, Jc);
     double acc = g.J ? g.J[c] * Jc.d : 0.0;
     if (g.compact) {
       // the factors' derivatives are the same at every node of the cell
@@ -952,18 +987,37 @@ hipError_t launch_record_dt(const Launch &L, const unsigned long long *dt1, doub
   return hipGetLastError();
 }
 
-hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_xy, const GeoCot &g,
+hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_rel, const GeoCot &g,
                           double *gv) {
   if (ncells == 0) return hipSuccess;
   size_t nb = ((size_t)ncells * 6 + 127) / 128;
   const int grid = nb > 65536 ? 65536 : (int)nb;
   switch (L.N) {
-    case 1: hipLaunchKernelGGL(geo_vjp_kernel<1>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_xy, g, gv); break;
-    case 2: hipLaunchKernelGGL(geo_vjp_kernel<2>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_xy, g, gv); break;
-    case 3: hipLaunchKernelGGL(geo_vjp_kernel<3>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_xy, g, gv); break;
-    case 4: hipLaunchKernelGGL(geo_vjp_kernel<4>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_xy, g, gv); break;
+    case 1: hipLaunchKernelGGL(geo_vjp_kernel<1>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_rel, g, gv); break;
+    case 2: hipLaunchKernelGGL(geo_vjp_kernel<2>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_rel, g, gv); break;
+    case 3: hipLaunchKernelGGL(geo_vjp_kernel<3>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_rel, g, gv); break;
+    case 4: hipLaunchKernelGGL(geo_vjp_kernel<4>, dim3(grid), dim3(128), 0, L.stream, ncells, L.ops, L.fmask, cell_rel, g, gv); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_init_grid_rel(const Launch &L, int ncells, const double *cell_rel, double *rx,
+                                double *ry, double *sx, double *sy, double *nx, double *ny,
+                                double *fs, double *J) {
+  if (ncells == 0) return hipSuccess;
+  size_t nb = ((size_t)ncells + 127) / 128;
+  const int grid = nb > 65536 ? 65536 : (int)nb;
+#define DG_IGR(n) hipLaunchKernelGGL(init_grid_rel_kernel<n>, dim3(grid), dim3(128), 0, L.stream, ncells, \
+                                     L.ops, L.fmask, cell_rel, rx, ry, sx, sy, nx, ny, fs, J)
+  switch (L.N) {
+    case 1: DG_IGR(1); break;
+    case 2: DG_IGR(2); break;
+    case 3: DG_IGR(3); break;
+    case 4: DG_IGR(4); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef DG_IGR
   return hipGetLastError();
 }
 

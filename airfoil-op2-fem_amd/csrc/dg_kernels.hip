@@ -338,13 +338,22 @@ __global__ __launch_bounds__(256) void calc_dt_kernel(int ncells, int N, int Np,
   if (threadIdx.x == 0) atomic_max_if_larger(out, r);
 }
 
-// src/init_grid.h: one lane per cell.
+// src/init_grid.h: one lane per cell.  Every product and sum is rounded on
+// its own (no multiply-add contraction) in the order of the reference's BLAS
+// calls (dcopy/daxpy/dscal for the node coordinates, a dot product per row
+// for dgemv), and division and square root are the correctly rounded IEEE
+// operations, so the factors are the same bits as a sequential host build
+// with contraction off.  The metrics of a cell whose coordinates are many
+// cell sizes from the origin are differences of nearly equal numbers, and a
+// contracted multiply-add moves them by ~1e-15 x (coordinate / cell size);
+// over many steps that difference would show in the solution.
 __global__ void init_grid_kernel(int ncells, int Np, int Nfp, const double *__restrict__ ops,
                                  const int *__restrict__ fmask, const double *__restrict__ cell_xy,
                                  double *nodeX, double *nodeY, double *x, double *y, double *xr,
                                  double *yr, double *xs, double *ys, double *rx, double *ry,
                                  double *sx, double *sy, double *nx, double *ny, double *fscale,
                                  double *Jc) {
+#pragma clang fp contract(off)
   const int NF = 3 * Nfp;
   const double *Dr = ops + 2 * Np * Np + Np * NF;
   const double *Ds = Dr + Np * Np;

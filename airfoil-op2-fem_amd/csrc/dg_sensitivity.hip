@@ -18,10 +18,18 @@
 // states q, w1, w2 of each step are recomputed from checkpoints stored every
 // ckpt steps (one level of revolve-style recomputation).
 //
-// Geometry chain: bump amplitudes -> surface offsets -> node y (linear,
-// gen_ogrid_shaped) -> cell vertices -> init_grid factors (rx, ry, sx, sy,
-// nx, ny, fscale, J), the last step differentiated on the host in dual
-// arithmetic, one direction per vertex coordinate.
+// Geometry chain: bump amplitudes -> surface offsets -> node displacement
+// (linear, the radial blend of gen_ogrid_shaped) -> cell edge vectors ->
+// init_grid factors (rx, ry, sx, sy, nx, ny, fscale, J), the last step in
+// dual arithmetic, one direction per vertex coordinate.  A cell's edge
+// vectors are the base mesh's (fixed) plus the differences of the nodes'
+// displacements, and the factors are formed from node coordinates relative
+// to the cell's first vertex (init_grid_cell_rel_t): the rounding of the
+// geometry is then relative to the cell size, so C_L is a smooth function
+// of the amplitudes down to the solver's own rounding, as central finite
+// differences need.  (Formed from absolute coordinates, the factors of the
+// 1e-4-chord wall cells carry ~1e-11 relative rounding that changes from
+// one shape to the next.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,18 +53,18 @@ using dg::hipfail;
 
 namespace {
 
-// init_grid of one cell in generic arithmetic, as init_grid_kernel
-// (src/init_grid.h:1-106), for the context's order.
+// init_grid of one cell from its edge vectors e (init_grid_cell_rel_t, as
+// the device geometry of this path), for the context's order.
 template <class T>
-void init_grid_cell(const dg_ctx *c, const T v[6], T *rx, T *ry, T *sx, T *sy, T *nx, T *ny, T *fs,
-                    T &Jc) {
+void init_grid_cell_rel(const dg_ctx *c, const T e[4], T *rx, T *ry, T *sx, T *sy, T *nx, T *ny,
+                        T *fs, T &Jc) {
   const double *r = c->r.data(), *s = c->s.data(), *Dr = c->Dr.data(), *Ds = c->Ds.data();
   const int *fm = c->FMASK.data();
   switch (c->N) {
-    case 1: dg::init_grid_cell_t<1, T>(r, s, Dr, Ds, fm, v, rx, ry, sx, sy, nx, ny, fs, Jc); break;
-    case 2: dg::init_grid_cell_t<2, T>(r, s, Dr, Ds, fm, v, rx, ry, sx, sy, nx, ny, fs, Jc); break;
-    case 3: dg::init_grid_cell_t<3, T>(r, s, Dr, Ds, fm, v, rx, ry, sx, sy, nx, ny, fs, Jc); break;
-    default: dg::init_grid_cell_t<4, T>(r, s, Dr, Ds, fm, v, rx, ry, sx, sy, nx, ny, fs, Jc); break;
+    case 1: dg::init_grid_cell_rel_t<1, T>(r, s, Dr, Ds, fm, e, rx, ry, sx, sy, nx, ny, fs, Jc); break;
+    case 2: dg::init_grid_cell_rel_t<2, T>(r, s, Dr, Ds, fm, e, rx, ry, sx, sy, nx, ny, fs, Jc); break;
+    case 3: dg::init_grid_cell_rel_t<3, T>(r, s, Dr, Ds, fm, e, rx, ry, sx, sy, nx, ny, fs, Jc); break;
+    default: dg::init_grid_cell_rel_t<4, T>(r, s, Dr, Ds, fm, e, rx, ry, sx, sy, nx, ny, fs, Jc); break;
   }
 }
 
@@ -92,11 +100,14 @@ struct dg_sens {
   std::vector<double> hh;
   double mach = 0.4, alpha = 0.0;
   std::vector<double> bumps;   // current shape
-  dg::Mesh mesh;
+  dg::Mesh mesh;               // the unshaped O-grid (zero amplitudes)
+  // per cell (x0, y0, x1 - x0, y1 - y0, x2 - x0, y2 - y0) of the current
+  // shape: base mesh edge vectors plus differences of the displacements
+  std::vector<double> h_cell_rel;
   int K = 0, Np = 0, NF = 0, Nfp = 0;
   DevBuf buf;
   // device geometry and maps
-  double *cell_xy, *nodeX, *nodeY, *x, *y, *xr, *yr, *xs, *ys, *rx, *ry, *sx, *sy, *nx, *ny, *fs, *J;
+  double *cell_rel, *rx, *ry, *sx, *sy, *nx, *ny, *fs, *J;
   int *fmap, *node_slots, *cell_off, *tile_off;
   // interior faces once each, (c * 3 + f) of both sides, lower cell first,
   // and the flux cotangents psi [K][NF][4] of the face-pair reverse passes
@@ -172,20 +183,37 @@ std::vector<double> surface_offsets(const dg_sens *s, const std::vector<double> 
   return dy;
 }
 
-// Builds the shaped mesh for the amplitudes b and refreshes the device
-// geometry (init_grid on the device, as the solver does).
+// Shapes the mesh for the amplitudes b (node y moves by node_w * offset of
+// its radial line, the blend of gen_ogrid_shaped) and refreshes the device
+// geometry from the cells' edge vectors.  A cell turned clockwise or
+// degenerate returns DG_ERR_MESH.
 int set_shape(dg_sens *s, const std::vector<double> &b) {
   dg_ctx *c = s->ctx;
-  std::vector<double> dy = surface_offsets(s, b);
-  std::string err;
-  if (!dg::gen_ogrid_shaped(s->ncirc, s->nrad, s->stretch, s->rfar, &dy, s->mesh, err))
-    return fail(c, DG_ERR_MESH, "dg_sens: " + err);
+  const std::vector<double> dy = surface_offsets(s, b);
+  const dg::Mesh &M = s->mesh;
+  std::vector<double> disp(M.nnodes);
+  for (int n = 0; n < M.nnodes; n++) disp[n] = M.node_w[n] * dy[M.node_k[n]];
+  std::vector<double> &cr = s->h_cell_rel;
+  cr.resize(6 * (size_t)s->K);
+  for (size_t k = 0; k < (size_t)s->K; k++) {
+    const int n0 = M.cells[3 * k], n1 = M.cells[3 * k + 1], n2 = M.cells[3 * k + 2];
+    const double *B0 = &M.coords[2 * (size_t)n0], *B1 = &M.coords[2 * (size_t)n1],
+                 *B2 = &M.coords[2 * (size_t)n2];
+    double *o = &cr[6 * k];
+    o[0] = B0[0];
+    o[1] = B0[1] + disp[n0];
+    o[2] = B1[0] - B0[0];
+    o[3] = (B1[1] - B0[1]) + (disp[n1] - disp[n0]);
+    o[4] = B2[0] - B0[0];
+    o[5] = (B2[1] - B0[1]) + (disp[n2] - disp[n0]);
+    if (!(o[2] * o[5] - o[3] * o[4] > 0.0))
+      return fail(c, DG_ERR_MESH, "dg_sens: the bump amplitudes fold cell " + std::to_string(k));
+  }
   s->bumps = b;
-  HIPCHECK(c, hipMemcpyAsync(s->cell_xy, s->mesh.cell_xy.data(), sizeof(double) * 6 * s->K,
-                             hipMemcpyHostToDevice, s->stream));
-  HIPCHECK(c, dg::launch_init_grid(launch_a(s), s->K, s->cell_xy, s->nodeX, s->nodeY, s->x, s->y,
-                                   s->xr, s->yr, s->xs, s->ys, s->rx, s->ry, s->sx, s->sy, s->nx,
-                                   s->ny, s->fs, s->J));
+  HIPCHECK(c, hipMemcpyAsync(s->cell_rel, cr.data(), sizeof(double) * cr.size(), hipMemcpyHostToDevice,
+                             s->stream));
+  HIPCHECK(c, dg::launch_init_grid_rel(launch_a(s), s->K, s->cell_rel, s->rx, s->ry, s->sx, s->sy,
+                                       s->nx, s->ny, s->fs, s->J));
   HIPCHECK(c, hipStreamSynchronize(s->stream));
   return DG_OK;
 }
@@ -445,10 +473,7 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   }
   if (e != hipSuccess) { dg_sens_destroy(s); return hipfail(c, e, "dg_sens_create: streams"); }
   DevBuf &b = s->buf;
-  s->cell_xy = b.alloc<double>(K * 6); s->nodeX = b.alloc<double>(K * 3); s->nodeY = b.alloc<double>(K * 3);
-  s->x = b.alloc<double>(K * Np); s->y = b.alloc<double>(K * Np);
-  s->xr = b.alloc<double>(K * Np); s->yr = b.alloc<double>(K * Np);
-  s->xs = b.alloc<double>(K * Np); s->ys = b.alloc<double>(K * Np);
+  s->cell_rel = b.alloc<double>(K * 6);
   s->rx = b.alloc<double>(K * Np); s->ry = b.alloc<double>(K * Np);
   s->sx = b.alloc<double>(K * Np); s->sy = b.alloc<double>(K * Np);
   s->nx = b.alloc<double>(K * NF); s->ny = b.alloc<double>(K * NF); s->fs = b.alloc<double>(K * NF);
@@ -599,14 +624,12 @@ int dg_sens_tangent(dg_sens *s, double dalpha, const double *dbumps, double *cl,
       for (int i = 0; i < nb; i++) if (dbumps && dbumps[i] != 0.0) ndy[n] += dbumps[i] * node_dy_db(s, n, i);
     std::vector<dg::Dual> a(Np), bb(Np), cc(Np), dd(Np), e(NF), f(NF), g(NF);
     for (size_t k = 0; k < K; k++) {
-      dg::Dual v[6];
-      for (int t = 0; t < 3; t++) {
-        const int node = s->mesh.cells[3 * k + t];
-        v[2 * t] = dg::Dual(s->mesh.cell_xy[6 * k + 2 * t], 0.0);
-        v[2 * t + 1] = dg::Dual(s->mesh.cell_xy[6 * k + 2 * t + 1], ndy[node]);
-      }
+      const int *nd = &s->mesh.cells[3 * k];
+      const double *cr = &s->h_cell_rel[6 * k];
+      const dg::Dual ev[4] = {dg::Dual(cr[2], 0.0), dg::Dual(cr[3], ndy[nd[1]] - ndy[nd[0]]),
+                              dg::Dual(cr[4], 0.0), dg::Dual(cr[5], ndy[nd[2]] - ndy[nd[0]])};
       dg::Dual Jc;
-      init_grid_cell<dg::Dual>(c, v, a.data(), bb.data(), cc.data(), dd.data(), e.data(), f.data(), g.data(), Jc);
+      init_grid_cell_rel<dg::Dual>(c, ev, a.data(), bb.data(), cc.data(), dd.data(), e.data(), f.data(), g.data(), Jc);
       for (size_t i = 0; i < Np; i++) { rxd[k * Np + i] = a[i].d; ryd[k * Np + i] = bb[i].d; sxd[k * Np + i] = cc[i].d; syd[k * Np + i] = dd[i].d; }
       for (size_t m = 0; m < NF; m++) { nxd[k * NF + m] = e[m].d; nyd[k * NF + m] = f[m].d; fsd[k * NF + m] = g[m].d; }
       Jd[k] = Jc.d;
@@ -906,7 +929,7 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
     HIPCHECK(c, dg::launch_add_indexed(L, (int)nws, s->wslot, s->wcot + 2 * nws, s->g_fs));
     const dg::GeoCot gc{s->g_rx, s->g_ry, s->g_sx, s->g_sy, s->g_nx, s->g_ny, s->g_fs, s->gJ,
                         1,       s->g_nxf, s->g_nyf, s->g_fsf};
-    HIPCHECK(c, dg::launch_geo_vjp(L, s->K, s->cell_xy, gc, s->gv));
+    HIPCHECK(c, dg::launch_geo_vjp(L, s->K, s->cell_rel, gc, s->gv));
     std::vector<double> gv(6 * K);
     HIPCHECK(c, hipMemcpyAsync(gv.data(), s->gv, sizeof(double) * 6 * K, hipMemcpyDeviceToHost, s->stream));
     HIPCHECK(c, hipStreamSynchronize(s->stream));

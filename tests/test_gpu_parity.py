@@ -20,6 +20,16 @@ GAM = 1.4
 UINF = 0.5 * math.sqrt(GAM)
 
 
+def oracle_threads():
+    """Host threads for the oracle's OpenMP loops: the process's CPU share."""
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n) or n)))
+
+
 def rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -60,8 +70,18 @@ def setup_case(dg, orc, mesh, order=3, alpha=0.0, mach=0.4):
     return ctx, o, case
 
 
-def test_init_grid(dg, orc, dev):
-    mesh = dg.periodic_grid(6, 5, seed=2)
+@pytest.mark.parametrize("kind", ["periodic", "ogrid", "vortex724"])
+def test_init_grid(dg, orc, dev, kind):
+    """init_grid on the device gives the oracle's bits (every product and sum
+    rounded on its own in the BLAS loop order, IEEE division and sqrt), on a
+    small periodic grid, an O-grid (thin wall cells far from the origin in
+    cell sizes) and the 1,048,352-cell bench mesh."""
+    if kind == "periodic":
+        mesh = dg.periodic_grid(6, 5, seed=2)
+    elif kind == "ogrid":
+        mesh = dg.naca0012_ogrid(64, 32, 1.08, 20.0)
+    else:
+        mesh = dg.periodic_grid(724, 724, 0.0, -5.0, 10.0, 10.0, seed=0, jitter=0.1)
     ctx, o, case = setup_case(dg, orc, mesh)
     K, Np, NF = mesh.ncells, ctx.Np, ctx.NF
     shapes = dict(nodeX=3, nodeY=3, x=Np, y=Np, xr=Np, yr=Np, xs=Np, ys=Np, rx=Np, ry=Np, sx=Np, sy=Np,
@@ -72,8 +92,7 @@ def test_init_grid(dg, orc, dev):
     for k in shapes:
         ref = case["geo"][k].ravel()
         got = d[k].cpu().numpy()
-        scale = np.abs(ref).max()
-        assert np.abs(got - ref).max() <= 1e-13 * scale, k
+        assert np.array_equal(got, ref), (k, np.abs(got - ref).max())
 
 
 @pytest.mark.parametrize("kind", ["periodic", "ogrid"])
@@ -165,7 +184,7 @@ def vortex64(dg, orc):
                        mesh.bedgeNum, mesh.bedge_type)
     st = o.new_state(case, q0)
     import os
-    t, dtn = o.run(case, st, 100, cfl=0.5, nthreads=min(16, os.cpu_count() or 1))
+    t, dtn = o.run(case, st, 100, cfl=0.5, nthreads=oracle_threads())
     return dict(mesh=mesh, q0=q0, q=st["q"].copy(), t=t, dt=dtn, case=case, o=o)
 
 
@@ -179,7 +198,7 @@ def vortex64_cell(dg, orc, vortex64):
     case = o.use_cell_geometry(o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum,
                                            mesh.bedge2cells, mesh.bedgeNum, mesh.bedge_type))
     st = o.new_state(case, vortex64["q0"])
-    t, dtn = o.run(case, st, 100, cfl=0.5, nthreads=min(16, os.cpu_count() or 1))
+    t, dtn = o.run(case, st, 100, cfl=0.5, nthreads=oracle_threads())
     return dict(q=st["q"].copy(), t=t, dt=dtn)
 
 
@@ -235,7 +254,7 @@ def test_vortex_error_matches_oracle_and_converges(dg, orc):
         case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
                            mesh.bedgeNum, mesh.bedge_type)
         st = o.new_state(case, q0)
-        to, _ = o.run(case, st, steps, cfl=0.5, nthreads=min(16, os.cpu_count() or 1))
+        to, _ = o.run(case, st, steps, cfl=0.5, nthreads=oracle_threads())
         g = case["geo"]
         area = 0.5 * ((g["nodeX"][:, 1] - g["nodeX"][:, 0]) * (g["nodeY"][:, 2] - g["nodeY"][:, 0])
                       - (g["nodeX"][:, 2] - g["nodeX"][:, 0]) * (g["nodeY"][:, 1] - g["nodeY"][:, 0]))
@@ -250,35 +269,26 @@ def test_vortex_error_matches_oracle_and_converges(dg, orc):
 
 def test_full_size_vortex_matches_oracle_and_conserves(dg, orc):
     """The bench workload at its full size (724x724 quads, 1,048,352 cells,
-    N=3, the launch shape the autotuner picks for it): 5 fused SSPRK3 steps
-    against the oracle's 5 steps on the host threads (<= 1e-12 relative L2
-    per variable); the mass-weighted totals of the conserved variables kept
-    over the run (periodic domain, conservative fluxes: within 1e-12 of the
-    largest total; the oracle keeps them to 2e-13); a second run the same
-    bits."""
+    N=3, CFL 0.5): 100 fused SSPRK3 steps against the oracle's 100 steps on
+    the host threads (<= 1e-12 relative L2 per conserved variable); the
+    mass-weighted totals of the conserved variables kept over the run
+    (periodic domain, conservative fluxes: within 1e-12 of the largest
+    total); a second run the same bits."""
     import os
     mesh, ctx, s = _vortex_solver(dg, 724)
     assert mesh.ncells == 1048352
     q0 = s.get_q()
-    s.run(5, cfl=0.5)
+    s.run(100, cfl=0.5)
     assert s.check() == 0
     q = s.get_q()
     o = orc.Oracle(dg.generate_operators(3), orc.freestream(GAM, 0.5, 0.0))
     case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
                        mesh.bedgeNum, mesh.bedge_type)
     st = o.new_state(case, q0)
-    t, dt = o.run(case, st, 5, cfl=0.5, nthreads=min(16, os.cpu_count() or 1))
+    t, dt = o.run(case, st, 100, cfl=0.5, nthreads=oracle_threads())
     errs = rel_per_var(q, st["q"])
-    # rho v is the vortex's own flow (its norm is 18% of the momentum's):
-    # measured 1.4e-12 relative to itself, 2.5e-13 relative to the momentum
-    # magnitude, the other variables <= 3.5e-13.  On this mesh the node
-    # coordinates are ~350 cell sizes, so rounding differences (the GPU
-    # contracts multiply-adds, the oracle rounds every product) enter
-    # init_grid's metrics amplified by that much.
-    mom = np.linalg.norm(st["q"][..., 1:3])
-    err_v = np.linalg.norm(q[..., 2] - st["q"][..., 2]) / mom
-    assert max(errs[0], errs[1], errs[3], err_v) <= 1e-12, (errs, err_v)
-    assert errs[2] <= 1e-11, errs
+    print(f"724x724 vortex, 100 steps: rel L2 per variable vs oracle {errs}", flush=True)
+    assert max(errs) <= 1e-12, errs
     assert abs(s.time()[0] - t) <= 1e-12 * t
     g = case["geo"]
     X, Y = g["nodeX"], g["nodeY"]
@@ -290,7 +300,7 @@ def test_full_size_vortex_matches_oracle_and_conserves(dg, orc):
     t0, t1 = totals(q0), totals(q)
     assert np.max(np.abs(t1 - t0)) <= 1e-12 * np.max(np.abs(t0)), (t0, t1)
     _, _, s2 = _vortex_solver(dg, 724)
-    s2.run(5, cfl=0.5)
+    s2.run(100, cfl=0.5)
     assert np.array_equal(s2.get_q(), q)
 
 
@@ -319,7 +329,7 @@ def test_airfoil_ogrid_short_run_matches_oracle(dg, orc, geom):
     case = mk()
     run_case = o.use_cell_geometry(mk()) if geom == "cell" else case
     st = o.new_state(run_case, q0)
-    o.run(run_case, st, 50, cfl=1.0, nthreads=min(16, os.cpu_count() or 1))
+    o.run(run_case, st, 50, cfl=1.0, nthreads=oracle_threads())
     qg = s.get_q()
     assert max(rel_per_var(qg, st["q"])) <= 1e-12
     # surface force: GPU quadrature vs the oracle's on the same state, and
@@ -362,7 +372,7 @@ def test_fused_solver_other_orders_match_oracle(dg, orc, order, geom):
     if geom == "cell":
         o.use_cell_geometry(case)
     st = o.new_state(case, q0)
-    o.run(case, st, 20, cfl=0.5, nthreads=min(16, os.cpu_count() or 1))
+    o.run(case, st, 20, cfl=0.5, nthreads=oracle_threads())
     errs = rel_per_var(s.get_q(), st["q"])
     assert max(errs) <= 1e-12, errs
 
@@ -392,17 +402,11 @@ def test_reference_airfoil_scenario(dg, orc, naca_vtk, mode):
     case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
                        mesh.bedgeNum, mesh.bedge_type)
     st = o.new_state(case, q0)
-    t, _ = o.run(case, st, 20, cfl=1.0, nthreads=min(16, os.cpu_count() or 1))
+    t, _ = o.run(case, st, 20, cfl=1.0, nthreads=oracle_threads())
     qg = s.get_q()
     errs = rel_per_var(qg, st["q"])
-    # at alpha 0 rho v is only what the profile deflects (its norm is 4% of
-    # rho u's after 20 steps), so its relative error is that of a small
-    # difference (measured 3.5e-12 in both modes, the other variables
-    # <= 2.7e-13); it is held to the bar relative to the momentum magnitude
-    mom = np.linalg.norm(st["q"][..., 1:3])
-    err_v = np.linalg.norm(qg[..., 2] - st["q"][..., 2]) / mom
-    assert max(errs[0], errs[1], errs[3], err_v) <= 1e-12, (errs, err_v)
-    assert errs[2] <= 1e-11, errs
+    print(f"naca0012.vtk N=4 mode {mode}, 20 steps: rel L2 per variable vs oracle {errs}", flush=True)
+    assert max(errs) <= 1e-12, errs
     assert abs(s.time()[0] - t) <= 1e-12 * t
 
 

@@ -95,6 +95,23 @@ def naca0012_ogrid(ncirc, nrad, stretch=1.08, rfar=20.0):
     return Mesh(h.value)
 
 
+def sweep_cases(ncases=64, seed=3):
+    """The batched sweep's seeded uneven O-grid cases: (ncirc, nrad, mach,
+    alpha_deg) with 2 * ncirc * nrad in about [8k, 130k] cells, alpha ~
+    U[-4, 8] deg, Mach ~ U[0.3, 0.6] (numpy's default_rng(seed))."""
+    import math
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(ncases):
+        cells = rng.uniform(8400, 128000)
+        ratio = rng.uniform(1.5, 3.0)
+        nrad = max(4, int(round(math.sqrt(cells / (2.0 * ratio)))))
+        ncirc = max(16, 2 * int(round(cells / (4.0 * nrad))))
+        out.append((ncirc, nrad, float(rng.uniform(0.3, 0.6)), float(rng.uniform(-4.0, 8.0))))
+    return out
+
+
 def read_vtk(path):
     """Legacy-VTK triangle mesh, boundary edges classified as src/load_mesh.cpp:147-172."""
     h = c_void_p()

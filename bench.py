@@ -178,21 +178,6 @@ def adjoint_bench(args):
                       "gradient_error": norm, "checks": checks}))
 
 
-def sweep_cases(ncases, seed=3):
-    """Seeded uneven O-grid sweep: 2*ncirc*nrad in [8k, 130k] cells,
-    alpha ~ U[-4, 8] deg, Mach ~ U[0.3, 0.6]."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(ncases):
-        cells = rng.uniform(8400, 128000)
-        ratio = rng.uniform(1.5, 3.0)
-        nrad = max(4, int(round(math.sqrt(cells / (2.0 * ratio)))))
-        ncirc = max(16, 2 * int(round(cells / (4.0 * nrad))))
-        out.append((ncirc, nrad, float(rng.uniform(0.3, 0.6)), float(rng.uniform(-4.0, 8.0))))
-    return out
-
-
 def batch_bench(args):
     """The batched sweep against the same cases run one after another on the
     same GPU with the single-case solver, and the bit-identity of every case."""
@@ -200,7 +185,7 @@ def batch_bench(args):
     import torch
     dg = importlib.import_module("airfoil-op2-fem_amd")
     torch.cuda.set_device(0)
-    specs = sweep_cases(args.cases)
+    specs = dg.sweep_cases(args.cases)
     meshes = [dg.naca0012_ogrid(nc, nr, 1.08, 20.0) for nc, nr, _, _ in specs]
     cells = [m.ncells for m in meshes]
     ktot = sum(cells)

@@ -692,3 +692,17 @@ void or_wall_forces(const or_consts *c, int nwall, const int *wcell, const int *
   fxy[0] = fx;
   fxy[1] = fy;
 }
+
+void or_wall_cp(const or_consts *c, int nwall, const int *wcell, const int *wface,
+                const double *q, double qdyn, double *cp) {
+  const int Np = c->Np, Nfp = c->Nfp;
+  for (int b = 0; b < nwall; b++) {
+    long k = wcell[b];
+    for (int i = 0; i < Nfp; i++) {
+      const double *qq = q + (k * Np + c->FMASK[wface[b] * Nfp + i]) * 4;
+      double rho = qq[0], u = qq[1] / rho, v = qq[2] / rho;
+      double p = (c->gam - 1.0) * (qq[3] - 0.5 * rho * (u * u + v * v));
+      cp[b * Nfp + i] = (p - c->bc_p) / qdyn;
+    }
+  }
+}

@@ -146,6 +146,12 @@ void or_wall_forces(const or_consts *c, int nwall, const int *wcell, const int *
                     const double *fscale, const double *J, const double *facew,
                     double *fxy);
 
+/* Pressure coefficient (p - bc_p) / qdyn at the Nfp nodes of each face listed
+ * in (wcell, wface), p from the state at the face node (FMASK), faces in list
+ * order: cp[b * Nfp + i].  No reference counterpart (surface C_p). */
+void or_wall_cp(const or_consts *c, int nwall, const int *wcell, const int *wface,
+                const double *q, double qdyn, double *cp);
+
 #ifdef __cplusplus
 }
 #endif

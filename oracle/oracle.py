@@ -186,6 +186,9 @@ class Oracle:
                   ("fscale", NF)]
         g = {k: np.zeros((K, s)) for k, s in shapes}
         lib.or_init_grid_all(ctypes.byref(self.c), K, _p(cxy), *[_p(g[k]) for k, _ in shapes])
+        # the cell's J: init_grid's value at node 0 (src/init_grid.h, same
+        # operation order), so that fscale * J is init_grid's face sJ
+        g["J"] = -g["xs"][:, 0] * g["yr"][:, 0] + g["xr"][:, 0] * g["ys"][:, 0]
         return g
 
     def make_case(self, cells, cell_xy, edge2cells=None, edgeNum=None, bedge2cells=None, bedgeNum=None,
@@ -287,18 +290,43 @@ class Oracle:
         return rhs, ext
 
     def wall_forces(self, case, q, facew):
-        """Pressure force (Fx, Fy) on the wall (type 2) boundary faces."""
+        """Pressure force (Fx, Fy) on the wall (type 2) boundary faces:
+        face quadrature with sJ = fscale * J (init_grid's J of the cell)."""
         sel = case["bedge_type"] == 2
         wc = np.ascontiguousarray(case["bedge2cells"][sel], dtype=np.int32)
         wf = np.ascontiguousarray(case["bedgeNum"][sel], dtype=np.int32)
         g = case["geo"]
-        x, y = g["nodeX"], g["nodeY"]
-        J = 0.25 * ((x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0]))
-        J = np.ascontiguousarray(J)
+        J = np.ascontiguousarray(g["J"])
         out = np.zeros(2)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (q, g["nx"], g["ny"], g["fscale"], J, facew)]
         lib.or_wall_forces(ctypes.byref(self.c), int(wc.size), _p(wc), _p(wf), *[_p(a) for a in arrs], _p(out))
         return out
+
+    def wall_cp(self, case, q):
+        """Pressure coefficient (p - p_inf) / (rho_inf |V_inf|^2 / 2) at the
+        face nodes of the wall (type 2) boundary faces, in boundary-edge
+        order: [nwall * Nfp]."""
+        sel = case["bedge_type"] == 2
+        wc = np.ascontiguousarray(case["bedge2cells"][sel], dtype=np.int32)
+        wf = np.ascontiguousarray(case["bedgeNum"][sel], dtype=np.int32)
+        ph = self.phys
+        qdyn = 0.5 * ph["bc_r"] * (ph["bc_u"] ** 2 + ph["bc_v"] ** 2)
+        out = np.zeros(wc.size * self.Nfp)
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        lib.or_wall_cp(ctypes.byref(self.c), int(wc.size), _p(wc), _p(wf), _p(qq), D(qdyn), _p(out))
+        return out
+
+    def lift_drag(self, case, q, facew, chord=1.0):
+        """(C_L, C_D) of the wall pressure force: the force of wall_forces
+        projected on the free-stream normal / direction, over
+        rho_inf |V_inf|^2 / 2 times the chord."""
+        import math
+        fx, fy = self.wall_forces(case, q, facew)
+        ph = self.phys
+        qdyn = 0.5 * ph["bc_r"] * (ph["bc_u"] ** 2 + ph["bc_v"] ** 2)
+        al = math.radians(ph["bc_alpha"])
+        return ((fy * math.cos(al) - fx * math.sin(al)) / (qdyn * chord),
+                (fx * math.cos(al) + fy * math.sin(al)) / (qdyn * chord))
 
     def vortex_state(self, uinf, beta, x0, y0, t, x, y, period_x=0.0):
         x = np.asarray(x, dtype=np.float64)

@@ -35,7 +35,7 @@ _SIGS = {
     "dg_order": ([c_void_p, P(c_int), P(c_int), P(c_int)], c_int),
     "dg_set_stream": ([c_void_p, c_void_p], c_int),
     "dg_set_variant": ([c_void_p, c_int, c_int], c_int),
-    "dg_generate_operators": ([c_int] + [c_void_p] * 9, c_int),
+    "dg_generate_operators": ([c_int] + [c_void_p] * 10, c_int),
     "dg_set_const": ([c_void_p, c_char_p, c_int, c_void_p], c_int),
     "dg_get_const": ([c_void_p, c_char_p, c_int, c_void_p], c_int),
     "dg_set_freestream": ([c_void_p, c_double, c_double, c_double, c_double, c_double], c_int),
@@ -79,7 +79,7 @@ _SIGS = {
     "dg_solver_forces": ([c_void_p, c_double, P(c_double), P(c_double), c_void_p], c_int),
     "dg_solver_wall_cp": ([c_void_p, P(c_int), c_void_p, c_void_p], c_int),
     "dg_rhs_tangent": ([c_void_p] + [c_void_p] * 16 + [c_double, c_double, c_void_p, c_void_p], c_int),
-    "dg_rhs_adjoint": ([c_void_p, c_void_p, c_void_p, c_double] + [c_void_p] * 17, c_int),
+    "dg_rhs_adjoint": ([c_void_p, c_void_p, c_void_p, c_double] + [c_void_p] * 18, c_int),
     "dg_sens_create": ([c_void_p, c_int, c_int, c_double, c_double, c_int, c_int, c_double, c_double,
                         P(c_void_p)], c_int),
     "dg_sens_destroy": ([c_void_p], None),

@@ -111,6 +111,32 @@ def test_abi_exports_every_declared_symbol(dg):
     assert not missing, missing
 
 
+def _header_prototypes():
+    """{name: parameter count} of every function declared in dg_euler.h."""
+    hdr = open(os.path.join(ROOT, "include", "dg_euler.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", " ", hdr, flags=re.S)
+    protos = {}
+    for m in re.finditer(r"^\s*(?:int|void|const char \*)\s*(dg_\w+)\s*\(([^;]*?)\)\s*;", hdr, re.M | re.S):
+        args = m.group(2).strip()
+        protos[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return protos
+
+
+def test_ctypes_signatures_match_the_header(dg):
+    """Every ctypes binding declares as many arguments as the C prototype it
+    binds (a short list would let ctypes convert the tail by default rules),
+    and every header function has a binding."""
+    from importlib import import_module
+    sigs = import_module("airfoil-op2-fem_amd._lib")._SIGS
+    protos = _header_prototypes()
+    assert len(protos) > 60
+    wrong = {n: (len(sigs[n][0]), protos[n]) for n in sigs if n in protos and len(sigs[n][0]) != protos[n]}
+    assert not wrong, wrong
+    assert set(sigs) <= set(protos), set(sigs) - set(protos)
+    unbound = sorted(set(protos) - set(sigs))
+    assert not unbound, unbound
+
+
 def test_op2_interface_exports_every_declared_function(dg):
     """Every non-template function of include/op2_dg.h (C++ linkage) is
     defined in the library, and the OP2 driver built against it links."""

@@ -80,6 +80,7 @@ _SIGS = {
     "dg_solver_wall_cp": ([c_void_p, P(c_int), c_void_p, c_void_p], c_int),
     "dg_rhs_tangent": ([c_void_p] + [c_void_p] * 16 + [c_double, c_double, c_void_p, c_void_p], c_int),
     "dg_rhs_adjoint": ([c_void_p, c_void_p, c_void_p, c_double] + [c_void_p] * 18, c_int),
+    "dg_rhs_adjoint_noalloc": ([c_void_p, c_void_p, c_void_p, c_double] + [c_void_p] * 17, c_int),
     "dg_sens_create": ([c_void_p, c_int, c_int, c_double, c_double, c_int, c_int, c_double, c_double,
                         P(c_void_p)], c_int),
     "dg_sens_destroy": ([c_void_p], None),

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,31 @@ namespace {
 thread_local std::string g_err;
 }
 
+// Device scratch of the no-allocation entry point dg_rhs_adjoint_noalloc: a
+// file-scope array every GPU holds its own copy of, for the life of the
+// process.  Host code never passes the array's name to a kernel or copy (on
+// the host it is only a stand-in for the symbol): dg_ctx_create looks up each
+// device's address of it once, with that device current, and everything
+// uses that address.
+static __device__ __align__(256) double scratch[DG_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this device's copy of `scratch` (the device must be current).
+hipError_t scratch_lookup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  double *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_ptr[device] = ptr;
+  return e;
+}
+}  // namespace
+
 namespace dg {
 
 int fail(dg_ctx *c, int code, const std::string &msg) {
@@ -28,6 +54,12 @@ int fail(dg_ctx *c, int code, const std::string &msg) {
 
 int hipfail(dg_ctx *c, hipError_t e, const char *what) {
   return fail(c, DG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+double *noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 Launch launch_of(const dg_ctx *c) {
@@ -85,6 +117,14 @@ int upload_ops(dg_ctx *c) {
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipMemcpy(c->d_ops, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
   HIPCHECK(c, hipMemcpy(c->d_fmask, c->FMASK.data(), sizeof(int) * NF, hipMemcpyHostToDevice));
+  // [Np][2]: the face slots m with FMASK[m] == node (-1 when fewer), read by
+  // the adjoint's cell pass
+  std::vector<int> slots(2 * Np, -1);
+  for (int m = 0; m < NF; m++) {
+    const int nd = c->FMASK[m];
+    if (slots[2 * nd] < 0) slots[2 * nd] = m; else slots[2 * nd + 1] = m;
+  }
+  HIPCHECK(c, hipMemcpy(c->d_node_slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice));
   return DG_OK;
 }
 
@@ -138,6 +178,8 @@ int dg_ctx_create(int order, dg_ctx **out) {
   if (he == hipSuccess) he = hipMalloc(&c->d_ops, sizeof(double) * dg::ops_size(c->Np, c->NF));
   if (he == hipSuccess) he = hipMalloc(&c->d_fmask, sizeof(int) * c->NF);
   if (he == hipSuccess) he = hipMalloc(&c->d_red, sizeof(unsigned long long) * 4);
+  if (he == hipSuccess) he = hipMalloc(&c->d_node_slots, sizeof(int) * 2 * c->Np);
+  if (he == hipSuccess) he = scratch_lookup(c->device);
   if (he != hipSuccess) {
     std::string msg = std::string("dg_ctx_create: ") + hipGetErrorString(he);
     dg_ctx_destroy(c);

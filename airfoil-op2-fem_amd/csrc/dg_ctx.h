@@ -43,6 +43,10 @@ Launch launch_of(const dg_ctx *c);
 // Free-stream physics constants for (gam, mach, alpha, p, rho) as
 // src/airfoil.cpp:79-99.
 Phys freestream(double gam, double mach, double alpha_deg, double p, double rho);
+// This device's address of the static scratch of dg_rhs_adjoint_noalloc
+// (DG_NOALLOC_SCRATCH_BYTES), looked up by dg_ctx_create; null before any
+// context was created on the device.
+double *noalloc_scratch(int device);
 }  // namespace dg
 
 #define HIPCHECK(ctx, call)                                          \

@@ -969,27 +969,26 @@ int dg_rhs_tangent(dg_ctx *c, const double *w, const double *wd, const double *r
   return DG_OK;
 }
 
-int dg_rhs_adjoint(dg_ctx *c, const double *w, const double *mu, double scale, const double *rx,
-                   const double *ry, const double *sx, const double *sy, const double *fscale,
-                   const double *nx, const double *ny, double *scratch, double *acc, double *grx,
-                   double *gry, double *gsx, double *gsy, double *gfs, double *gnx, double *gny,
-                   double *gbcu, double *gbcv) {
-  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_rhs_adjoint: null context");
-  if (c->ncells < 0) return fail(c, DG_ERR_STATE, "dg_rhs_adjoint: no mesh declared");
-  if (c->ncells > 0 && (!w || !mu || !rx || !ry || !sx || !sy || !fscale || !nx || !ny || !scratch ||
+}  // extern "C"
+
+namespace {
+
+// dg_rhs_adjoint and dg_rhs_adjoint_noalloc: the per-slot reverse passes
+// (face, boundary, cell) with their slot cotangents in `slots` (2 * ncells *
+// NF * 4 doubles, every slot written by the face pass before the cell pass
+// reads it).  Launches kernels only: no allocation, no copy, no
+// synchronisation.
+int rhs_adjoint_impl(dg_ctx *c, const char *fn, const double *w, const double *mu, double scale,
+                     const double *rx, const double *ry, const double *sx, const double *sy,
+                     const double *fscale, const double *nx, const double *ny, double *slots,
+                     double *acc, double *grx, double *gry, double *gsx, double *gsy, double *gfs,
+                     double *gnx, double *gny, double *gbcu, double *gbcv) {
+  if (c->ncells > 0 && (!w || !mu || !rx || !ry || !sx || !sy || !fscale || !nx || !ny || !slots ||
                         !acc || !grx || !gry || !gsx || !gsy || !gfs || !gnx || !gny || !gbcu || !gbcv))
-    return fail(c, DG_ERR_ARG, "dg_rhs_adjoint: null array");
-  if (!c->d_node_slots) {
-    std::vector<int> slots(2 * c->Np, -1);
-    for (int m = 0; m < c->NF; m++) {
-      const int nd = c->FMASK[m];
-      if (slots[2 * nd] < 0) slots[2 * nd] = m; else slots[2 * nd + 1] = m;
-    }
-    HIPCHECK(c, hipMalloc(&c->d_node_slots, sizeof(int) * slots.size()));
-    HIPCHECK(c, hipMemcpy(c->d_node_slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice));
-  }
-  if (((uintptr_t)w | (uintptr_t)mu | (uintptr_t)acc | (uintptr_t)scratch) & 31u)
-    return fail(c, DG_ERR_ARG, "dg_rhs_adjoint: w, mu, acc and scratch must be 32-byte aligned");
+    return fail(c, DG_ERR_ARG, std::string(fn) + ": null array");
+  if (((uintptr_t)w | (uintptr_t)mu | (uintptr_t)acc | (uintptr_t)slots) & 31u)
+    return fail(c, DG_ERR_ARG, std::string(fn) + ": w, mu, acc and scratch must be 32-byte aligned");
+  double *scratch = slots;
   dg::AdjArgs a{};
   a.ncells = c->ncells; a.scale = scale; a.w = w; a.mu = mu;
   a.rx = rx; a.ry = ry; a.sx = sx; a.sy = sy; a.nx = nx; a.ny = ny; a.fscale = fscale;
@@ -1001,6 +1000,43 @@ int dg_rhs_adjoint(dg_ctx *c, const double *w, const double *mu, double scale, c
   a.g_nx = gnx; a.g_ny = gny; a.g_fs = gfs; a.g_bcu = gbcu; a.g_bcv = gbcv;
   HIPCHECK(c, dg::launch_rhs_adj(dg::launch_of(c), a, c->d_node_slots));
   return DG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dg_rhs_adjoint(dg_ctx *c, const double *w, const double *mu, double scale, const double *rx,
+                   const double *ry, const double *sx, const double *sy, const double *fscale,
+                   const double *nx, const double *ny, double *scratch, double *acc, double *grx,
+                   double *gry, double *gsx, double *gsy, double *gfs, double *gnx, double *gny,
+                   double *gbcu, double *gbcv) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_rhs_adjoint: null context");
+  if (c->ncells < 0) return fail(c, DG_ERR_STATE, "dg_rhs_adjoint: no mesh declared");
+  return rhs_adjoint_impl(c, "dg_rhs_adjoint", w, mu, scale, rx, ry, sx, sy, fscale, nx, ny, scratch,
+                          acc, grx, gry, gsx, gsy, gfs, gnx, gny, gbcu, gbcv);
+}
+
+int dg_rhs_adjoint_noalloc(dg_ctx *c, const double *w, const double *mu, double scale,
+                           const double *rx, const double *ry, const double *sx, const double *sy,
+                           const double *fscale, const double *nx, const double *ny, double *acc,
+                           double *grx, double *gry, double *gsx, double *gsy, double *gfs,
+                           double *gnx, double *gny, double *gbcu, double *gbcv) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_rhs_adjoint_noalloc: null context");
+  if (c->ncells < 0) return fail(c, DG_ERR_STATE, "dg_rhs_adjoint_noalloc: no mesh declared");
+  const size_t need = 2 * (size_t)c->ncells * c->NF * 4;
+  const size_t cap = DG_NOALLOC_SCRATCH_BYTES / sizeof(double);
+  if (need > cap)
+    return fail(c, DG_ERR_ARG, "dg_rhs_adjoint_noalloc: " + std::to_string(c->ncells) +
+                                   " cells need " + std::to_string(need * sizeof(double)) +
+                                   " bytes of scratch; the static scratch holds " +
+                                   std::to_string(DG_NOALLOC_SCRATCH_BYTES) + " (at most " +
+                                   std::to_string(cap / (2 * (size_t)c->NF * 4)) +
+                                   " cells at this order); use dg_rhs_adjoint with a scratch buffer");
+  double *ptr = dg::noalloc_scratch(c->device);
+  if (!ptr) return fail(c, DG_ERR_STATE, "dg_rhs_adjoint_noalloc: no scratch address for this device");
+  return rhs_adjoint_impl(c, "dg_rhs_adjoint_noalloc", w, mu, scale, rx, ry, sx, sy, fscale, nx, ny,
+                          ptr, acc, grx, gry, gsx, gsy, gfs, gnx, gny, gbcu, gbcv);
 }
 
 int dg_sens_timing(const dg_sens *s, double *forward_ms, double *gradient_ms, double *tangent_ms) {

@@ -308,6 +308,18 @@ class Context:
         self._chk(lib.dg_rhs_adjoint(self._h, _dev(w, "w", n), _dev(mu, "mu", n), float(scale), *g,
                                      _dev(scratch, "scratch", 2 * K * NF * 4), _dev(acc, "acc", n), *ga))
 
+    def rhs_adjoint_noalloc(self, w, mu, scale, geo, acc, gacc):
+        """rhs_adjoint through dg_rhs_adjoint_noalloc: no scratch argument,
+        the library's static per-GPU scratch instead (meshes up to
+        DG_NOALLOC_SCRATCH_BYTES of slot cotangents)."""
+        K, Np, NF, n = self.ncells, self.Np, self.NF, self.state_numel()
+        sizes = dict(rx=K * Np, ry=K * Np, sx=K * Np, sy=K * Np, fscale=K * NF, nx=K * NF, ny=K * NF,
+                     bcu=K * NF, bcv=K * NF)
+        g = [_dev(geo[k], k, sizes[k]) for k in ("rx", "ry", "sx", "sy", "fscale", "nx", "ny")]
+        ga = [_dev(gacc[k], "g" + k, sizes[k]) for k in ("rx", "ry", "sx", "sy", "fscale", "nx", "ny", "bcu", "bcv")]
+        self._chk(lib.dg_rhs_adjoint_noalloc(self._h, _dev(w, "w", n), _dev(mu, "mu", n), float(scale), *g,
+                                             _dev(acc, "acc", n), *ga))
+
     def set_workingQ(self, dt, stage, q, k1, k2, workingQ):
         n = self.state_numel()
         self._chk(lib.dg_set_workingQ(self._h, float(dt), int(stage), _dev(q, "q", n), _dev(k1, "k1", n),

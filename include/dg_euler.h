@@ -333,6 +333,26 @@ int dg_rhs_adjoint(dg_ctx *ctx, const double *d_w, const double *d_mu, double sc
                    double *d_gry, double *d_gsx, double *d_gsy, double *d_gfs, double *d_gnx,
                    double *d_gny, double *d_gbcu, double *d_gbcv);
 
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): dg_rhs_adjoint without d_scratch, with the same kernels and
+ * results.  The slot cotangents live in a static device array of
+ * DG_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends, so 2 * ncells * 3 * Nfp * 4 doubles must fit: at most
+ * 87,381 cells at N=3 (69,905 at N=4); a larger mesh returns DG_ERR_ARG.
+ * The call allocates nothing and only launches kernels on the context's
+ * stream.  Every call on one GPU shares that GPU's array, which keeps the
+ * last call's contents (nothing needs clearing: every slot is written
+ * before it is read), so calls that overlap on one GPU, from other streams
+ * or threads, race.  d_w, d_mu and d_acc must be 32-byte aligned. */
+#define DG_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dg_rhs_adjoint_noalloc(dg_ctx *ctx, const double *d_w, const double *d_mu, double scale,
+                           const double *d_rx, const double *d_ry, const double *d_sx,
+                           const double *d_sy, const double *d_fscale, const double *d_nx,
+                           const double *d_ny, double *d_acc, double *d_grx, double *d_gry,
+                           double *d_gsx, double *d_gsy, double *d_gfs, double *d_gnx,
+                           double *d_gny, double *d_gbcu, double *d_gbcv);
+
 /* ------------------------------------------------------------------ */
 /* Lift sensitivities of the airfoil case (no reference counterpart)  */
 /* ------------------------------------------------------------------ */

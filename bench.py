@@ -1,21 +1,39 @@
 """Throughput benchmark of the fused HIP SSPRK3 path on one MI355X.
 
-Workload: isentropic vortex on the 724x724 jittered periodic grid
+Headline workload: isentropic vortex on the 724x724 jittered periodic grid
 (1,048,352 triangles), order N=3 (Np=10, 41.9M DoF), FP64, SSPRK3 with
-dt = 0.5/dt1 recomputed every step.  A step is one SSPRK3 step (three RK
-stages); the metric is DoF-updates per second per RK stage.
+dt = 0.5/dt1 recomputed every step, steps replayed from a hipGraph.  A step
+is one SSPRK3 step (three RK stages); the metric is DoF-updates per second
+per RK stage.
 
 Prints one JSON line.  Per-kernel times (HIP events on the solver's stream)
 give the roofline entry of the dominant stage kernel; the oracle's OpenMP
 restatement timed on a bounded sample on the host cores gives cpu_baseline.
+The same line carries the other two headline figures: the 64-case batched
+sweep against the same cases run one after another ("batch"), and the lift
+gradient's cost against its forward run with its checks ("adjoint").
 """
-import argparse
-import importlib
-import json
-import math
 import os
-import sys
-import time
+
+# The CPU share of this process, read before any OpenMP runtime starts: with
+# thread binding on, the runtime pins the initial thread to one place at
+# load, after which the process's affinity mask shows a single CPU.
+try:
+    _AFFINITY_CPUS = len(os.sched_getaffinity(0))
+except AttributeError:
+    _AFFINITY_CPUS = None
+_OMP_NUM_THREADS = os.environ.get("OMP_NUM_THREADS")
+# thread binding of the CPU baseline's OpenMP loops; set before any library
+# that starts an OpenMP runtime is loaded
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import argparse  # noqa: E402
+import importlib  # noqa: E402
+import json  # noqa: E402
+import math  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -40,11 +58,20 @@ def cell_bytes(N, stage, geometry="node"):
 
 
 def host_cores():
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    """Threads for the CPU baseline: the CPUs this process may run on (its
+    affinity mask at start), or fewer when OMP_NUM_THREADS said so."""
+    n = _AFFINITY_CPUS or os.cpu_count() or 1
+    env = _OMP_NUM_THREADS or ""
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def host_info():
+    return {"threads": host_cores(), "affinity_cpus": _AFFINITY_CPUS, "online_cpus": os.cpu_count(),
+            "omp_num_threads": _OMP_NUM_THREADS,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
+            "cpu_model": cpu_model()}
 
 
 def cpu_model():
@@ -126,125 +153,140 @@ def summarize_pmc(fetch_db, write_db, cfetch_db, cwrite_db, copy_bytes):
     print(json.dumps(out))
 
 
-def adjoint_bench(args):
-    """Lift sensitivities of the NACA0012 O-grid (256 x 128, stretch 1.08,
-    far field at 20 chords, 65,536 cells) with 16 Hicks-Henne bumps (8 upper,
-    8 lower, seeded amplitudes in +-1e-3), Mach 0.4, alpha 2 deg, N=3:
-    400 SSPRK3 steps recorded, adjoint with checkpoints every 20 steps,
-    checked against the tangent-linear run and central differences."""
+ADJ_WORKLOAD = ("NACA0012 O-grid 256x128 (65536 cells), 16 Hicks-Henne bumps (8 upper, 8 lower, seeded "
+                "amplitudes in +-1e-3 chord), M 0.4, alpha 2 deg, N={order}, {steps} SSPRK3 steps, checkpoints "
+                "every {ckpt} steps, frozen dt")
+
+
+def adjoint_leg(order=3, steps=400, ckpt=20, checks=True):
+    """Lift sensitivities: forward run (recorded), adjoint gradient with
+    checkpoints every `ckpt` steps, their wall times (the second of two calls
+    each), and the gradient against the tangent-linear run and against finite
+    differences of C_L with the recorded dt sequence (fourth-order central
+    difference at h = 1e-5, and the plain central difference at h = 1e-6)."""
     import numpy as np
     import torch
     dg = importlib.import_module("airfoil-op2-fem_amd")
     torch.cuda.set_device(0)
-    ctx = dg.Context(args.order)
+    ctx = dg.Context(order)
     ctx.set_freestream(1.4, 0.4, 2.0, 1.0, 1.0)
     s = dg.Sensitivity(ctx, 256, 128, 1.08, 20.0, 8, 8, 0.4, 2.0)
     bumps = np.random.default_rng(2024).uniform(-1e-3, 1e-3, 16)
-    cl = s.record(bumps, nsteps=args.adj_steps, cfl=1.0)
-    s.record(bumps, nsteps=args.adj_steps, cfl=1.0)  # warm
-    cl_g, da, db = s.gradient(ckpt_every=20)
-    cl_g, da, db = s.gradient(ckpt_every=20)
+    cl = s.record(bumps, nsteps=steps, cfl=1.0)
+    s.record(bumps, nsteps=steps, cfl=1.0)
+    fwd_ms = s.timing()["forward_ms"]
+    s.gradient(ckpt_every=ckpt)
+    cl_g, da, db = s.gradient(ckpt_every=ckpt)
     tm = s.timing()
-    h = 1e-6
-    checks = {}
-    fd = (s.eval(2.0 + h, bumps) - s.eval(2.0 - h, bumps)) / (2 * h)
-    _, tl = s.tangent(1.0, None)
-    checks["alpha"] = {"adjoint": da, "tangent": tl, "fd": fd, "rel_fd": abs(fd - da) / abs(da),
-                       "rel_tangent": abs(tl - da) / abs(da)}
-    g_fd, g_tl = np.zeros(17), np.zeros(17)
-    g_fd[0], g_tl[0] = fd, tl
+    tm["forward_ms"] = fwd_ms
+    g = np.concatenate([[da], db])
+    out = {"value": tm["gradient_ms"] / fwd_ms, "unit": "gradient time / forward time",
+           "forward_ms": fwd_ms, "gradient_ms": tm["gradient_ms"],
+           "gradient_parts_ms": {"checkpointed_forward": tm["gradient_forward_ms"],
+                                 "reverse_sweep": tm["gradient_reverse_ms"], "host": tm["gradient_host_ms"]},
+           "cl": cl, "dcl_dalpha_per_deg": da, "dcl_dbumps": db.tolist(),
+           "workload": ADJ_WORKLOAD.format(order=order, steps=steps, ckpt=ckpt)}
+    if not checks:
+        return out
+
+    def cl_at(i, x):
+        if i == 0:
+            return s.eval(2.0 + x, bumps)
+        b = bumps.copy()
+        b[i - 1] += x
+        return s.eval(2.0, b)
+    tl = np.zeros(17)
+    _, tl[0] = s.tangent(1.0, None)
     for i in range(16):
-        bp, bm = bumps.copy(), bumps.copy()
-        bp[i] += h
-        bm[i] -= h
-        fd = (s.eval(2.0, bp) - s.eval(2.0, bm)) / (2 * h)
         e = np.zeros(16)
         e[i] = 1.0
-        _, tl = s.tangent(0.0, e)
-        g_fd[i + 1], g_tl[i + 1] = fd, tl
-        checks[f"bump{i}"] = {"adjoint": db[i], "tangent": tl, "fd": fd,
-                              "rel_fd": abs(fd - db[i]) / max(abs(db[i]), 1e-300),
-                              "rel_tangent": abs(tl - db[i]) / max(abs(db[i]), 1e-300)}
-    g = np.concatenate([[da], db])
-    norm = {"fd_rel_l2": float(np.linalg.norm(g_fd - g) / np.linalg.norm(g)),
-            "tangent_rel_l2": float(np.linalg.norm(g_tl - g) / np.linalg.norm(g)),
-            "fd_max_abs": float(np.abs(g_fd - g).max())}
-    print(json.dumps({"metric": "adjoint gradient cost / forward cost", "value": tm["gradient_ms"] / tm["forward_ms"],
-                      "unit": "ratio", "higher_is_better": False, "dtype": "f64",
-                      "config": {"workload": "NACA0012 O-grid 256x128 (65536 cells), 16 Hicks-Henne bumps, "
-                                             f"M 0.4, alpha 2 deg, N={args.order}, {args.adj_steps} SSPRK3 steps, "
-                                             "checkpoints every 20 steps, frozen dt"},
-                      "cl": cl, "dcl_dalpha_per_deg": da, "dcl_dbumps": db.tolist(), "timing_ms": tm,
-                      "gradient_error": norm, "checks": checks}))
+        _, tl[i + 1] = s.tangent(0.0, e)
+    h2, h4 = 1e-6, 1e-5
+    fd2 = np.array([(cl_at(i, h2) - cl_at(i, -h2)) / (2 * h2) for i in range(17)])
+    fd4 = np.array([(-cl_at(i, 2 * h4) + 8 * cl_at(i, h4) - 8 * cl_at(i, -h4) + cl_at(i, -2 * h4)) / (12 * h4)
+                    for i in range(17)])
+    out["checks"] = {
+        "tangent_rel_per_component_max": float(np.max(np.abs(tl - g) / np.abs(g))),
+        "fd4_h1e-5_rel_per_component_max": float(np.max(np.abs(fd4 - g) / np.abs(g))),
+        "fd4_h1e-5_rel_per_component": (np.abs(fd4 - g) / np.abs(g)).tolist(),
+        "fd2_h1e-6_rel_per_component_max": float(np.max(np.abs(fd2 - g) / np.abs(g))),
+        "fd2_h1e-6_abs_max": float(np.abs(fd2 - g).max()),
+        "fd2_h1e-6_rel_l2": float(np.linalg.norm(fd2 - g) / np.linalg.norm(g)),
+        "components_within_1e-6_fd4": int(np.sum(np.abs(fd4 - g) <= 1e-6 * np.abs(g))),
+        "components_within_1e-6_fd2": int(np.sum(np.abs(fd2 - g) <= 1e-6 * np.abs(g)))}
+    return out
 
 
-def batch_bench(args):
-    """The batched sweep against the same cases run one after another on the
-    same GPU with the single-case solver, and the bit-identity of every case."""
+def adjoint_bench(args):
+    out = adjoint_leg(args.order, args.adj_steps, 20, checks=True)
+    print(json.dumps(dict({"metric": "adjoint gradient cost / forward cost", "higher_is_better": False,
+                           "dtype": "f64", "config": {"workload": out["workload"]}}, **out)))
+
+
+def batch_leg(order=3, ncases=64, steps=500, cfl=0.25):
+    """The batched sweep (one persistent launch per stage over all cases,
+    hipGraph-replayed steps, autotuned launch shape) against the same cases
+    run one after another on the same GPU by the single-case solver (each
+    autotuned, graph-replayed), and the bit-identity of every case with its
+    single run."""
     import numpy as np
     import torch
     dg = importlib.import_module("airfoil-op2-fem_amd")
     torch.cuda.set_device(0)
-    specs = dg.sweep_cases(args.cases)
+    specs = dg.sweep_cases(ncases)
     meshes = [dg.naca0012_ogrid(nc, nr, 1.08, 20.0) for nc, nr, _, _ in specs]
     cells = [m.ncells for m in meshes]
     ktot = sum(cells)
-    Np = (args.order + 1) * (args.order + 2) // 2
+    Np = (order + 1) * (order + 2) // 2
     print(f"[bench] sweep of {len(specs)} O-grid cases, {min(cells)}..{max(cells)} cells, {ktot} in total",
           file=sys.stderr, flush=True)
-    ctx = dg.Context(args.order)
+    ctx = dg.Context(order)
     ctx.set_freestream(1.4, 0.4, 0.0, 1.0, 1.0)
-    batch = dg.BatchSolver(ctx, meshes, [s[2] for s in specs], [s[3] for s in specs])
-    batch.run(2, cfl=args.cfl, graph=True)  # warm (graph capture), then restart from the free stream
+    batch = dg.BatchSolver(ctx, meshes, [sp[2] for sp in specs], [sp[3] for sp in specs])
+    batch.run(2, cfl=cfl, graph=True)  # warm (graph capture), then restart from the free stream
     del batch
-    batch = dg.BatchSolver(ctx, meshes, [s[2] for s in specs], [s[3] for s in specs])
-    batch.autotune(trial_steps=3, cfl=args.cfl)
+    batch = dg.BatchSolver(ctx, meshes, [sp[2] for sp in specs], [sp[3] for sp in specs])
+    batch.autotune(trial_steps=3, cfl=cfl)
     launch_batch = batch.launch_config()
-    print(f"[bench] autotuned batch launch: {launch_batch}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    batch.run(args.batch_steps, cfl=args.cfl, graph=True)
+    batch.run(steps, cfl=cfl, graph=True)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     bad = [k for k in range(len(specs)) if not np.isfinite(batch.case_q(k)).all()]
     print(f"[bench] batched: {t_batch:.3f}s, non-finite cases: {bad}", file=sys.stderr, flush=True)
-    seq = {}
+    t_seq = 0.0
     identical = True
-    for graph in (False, True):
-        tot = 0.0
-        for k, (m, (_, _, mach, alpha)) in enumerate(zip(meshes, specs)):
-            c1 = dg.Context(args.order)
-            c1.set_freestream(1.4, mach, alpha, 1.0, 1.0)
-            s = dg.Solver(c1, m)
-            s.run(1, cfl=args.cfl, graph=graph)
-            s = dg.Solver(c1, m)
-            s.autotune(trial_steps=2, cfl=args.cfl)
-            torch.cuda.synchronize()
-            a = time.perf_counter()
-            s.run(args.batch_steps, cfl=args.cfl, graph=graph)
-            torch.cuda.synchronize()
-            tot += time.perf_counter() - a
-            if graph:
-                qb, qs = batch.case_q(k), s.get_q()
-                identical &= bool((np.array_equal(qb, qs) or np.array_equal(np.isnan(qb), np.isnan(qs)) and
-                                   np.array_equal(qb[~np.isnan(qb)], qs[~np.isnan(qs)])) and
-                                  batch.case_time(k) == s.time())
-            del s
-        seq["graph" if graph else "direct"] = tot
-        print(f"[bench] sequential ({'graph' if graph else 'direct'}): {tot:.3f}s", file=sys.stderr, flush=True)
-    t_seq = min(seq.values())
-    dof_updates = ktot * Np * 4 * 3 * args.batch_steps
-    print(json.dumps({"metric": "batched-sweep speedup over sequential per-case runs", "value": t_seq / t_batch,
-                      "unit": "x", "higher_is_better": True, "dtype": "f64",
-                      "config": {"workload": f"{len(specs)} NACA0012 O-grid cases ({min(cells)}..{max(cells)} cells, "
-                                             f"{ktot} total), alpha U[-4,8] deg, Mach U[0.3,0.6], N={args.order}, "
-                                             f"{args.batch_steps} SSPRK3 steps, CFL {args.cfl}, one persistent "
-                                             "launch per stage, hipGraph-captured steps"},
-                      "batched_s": t_batch, "sequential_s": seq, "nonfinite_cases": bad,
-                      "batch_launch": launch_batch,
-                      "batched_dof_updates_per_s_per_stage": dof_updates / t_batch,
-                      "sequential_dof_updates_per_s_per_stage": dof_updates / t_seq,
-                      "bit_identical_all_cases": identical}))
+    for k, (m, (_, _, mach, alpha)) in enumerate(zip(meshes, specs)):
+        c1 = dg.Context(order)
+        c1.set_freestream(1.4, mach, alpha, 1.0, 1.0)
+        sv = dg.Solver(c1, m)
+        sv.run(2, cfl=cfl, graph=True)
+        sv = dg.Solver(c1, m)
+        sv.autotune(trial_steps=2, cfl=cfl)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        sv.run(steps, cfl=cfl, graph=True)
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - a
+        identical &= bool(np.array_equal(batch.case_q(k), sv.get_q()) and batch.case_time(k) == sv.time())
+        del sv
+    print(f"[bench] sequential: {t_seq:.3f}s", file=sys.stderr, flush=True)
+    dof_updates = ktot * Np * 4 * 3 * steps
+    return {"value": t_seq / t_batch, "unit": "x (sequential time / batched time)",
+            "batched_s": t_batch, "sequential_s": t_seq, "nonfinite_cases": bad, "batch_launch": launch_batch,
+            "batched_dof_updates_per_s_per_stage": dof_updates / t_batch,
+            "sequential_dof_updates_per_s_per_stage": dof_updates / t_seq,
+            "bit_identical_all_cases": identical,
+            "workload": f"{len(specs)} NACA0012 O-grid cases ({min(cells)}..{max(cells)} cells, {ktot} total), "
+                        f"alpha U[-4,8] deg, Mach U[0.3,0.6], N={order}, {steps} SSPRK3 steps, CFL {cfl}, one "
+                        "persistent launch per stage, hipGraph-replayed steps"}
+
+
+def batch_bench(args):
+    out = batch_leg(args.order, args.cases, args.batch_steps, args.cfl)
+    print(json.dumps(dict({"metric": "batched-sweep speedup over sequential per-case runs", "higher_is_better": True,
+                           "dtype": "f64", "config": {"workload": out["workload"]}}, **out)))
 
 
 def airfoil_bench(args):
@@ -349,8 +391,10 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=30,
-                    help="SSPRK3 steps of the CPU baseline sample (about 10 s on 16 host threads)")
+                    help="SSPRK3 steps of the CPU baseline sample (about 8 s on 16 host threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-case batched-sweep figure")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the lift-gradient figure")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--geometry", choices=["node", "cell"], default="node",
                     help="geometry the stage kernels read: per-node factors (the reference's "
@@ -401,13 +445,12 @@ def main():
         tune = dict(solver.launch_config(), table_ms_per_step={
             f"wpb{r['wpb']}_walk{r['walk']}_w{r['waves_per_cu']}": round(r["ms_per_step"], 4) for r in table})
         print(f"[bench] autotuned launch: {solver.launch_config()}", file=sys.stderr, flush=True)
-    solver.run(args.warmup, cfl=args.cfl)
-    torch.cuda.synchronize()
-    q_start = solver.get_q() if not args.no_cpu_baseline else None
-
+    # warm-up and the timed steps back to back (graph replay, no host work
+    # or transfer in between that would idle the GPU)
+    solver.run(args.warmup, cfl=args.cfl, graph=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    solver.run(args.steps, cfl=args.cfl)
+    solver.run(args.steps, cfl=args.cfl, graph=True)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     elapsed = t2 - t1
@@ -415,7 +458,7 @@ def main():
         raise SystemExit("non-finite state during the timed run")
     ms_step = 1e3 * elapsed / max(args.steps, 1)
     value = dof * 3 * args.steps / elapsed
-    print(f"[bench] {args.steps} steps in {elapsed:.3f}s: {ms_step:.3f} ms/step, "
+    print(f"[bench] {args.steps} steps in {elapsed:.4f}s: {ms_step:.4f} ms/step, "
           f"{value:.4e} DoF-updates/s per RK stage", file=sys.stderr, flush=True)
 
     # per-kernel durations with HIP events on the solver stream
@@ -436,7 +479,11 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": cell_bytes(N, int(dom[-1]), args.geometry) * K,
-                "avg_launch_ms": round(dom_ms, 4)}
+                "avg_launch_ms": round(dom_ms, 4),
+                "stages": {k: {"avg_launch_ms": round(ms, 4),
+                               "frac": round(cell_bytes(N, int(k[-1]), args.geometry) * K / (ms * 1e-3) / 1e9
+                                             / HBM_PEAK_GBS, 4) if ms > 0 else None}
+                           for k, (ms, _) in kt.items()}}
 
     # the same steps on the per-cell geometry (an opt-in mode whose results
     # leave the reference's by init_grid's per-node rounding scatter), for
@@ -444,10 +491,10 @@ def main():
     alt = None
     if args.geometry == "node" and ctx.variant == 0 and args.steps > 0:
         solver.set_geometry(dg.GEOM_CELL)
-        solver.run(max(args.warmup, 1), cfl=args.cfl)
+        solver.run(max(args.warmup, 1), cfl=args.cfl, graph=True)
         torch.cuda.synchronize()
         a1 = time.perf_counter()
-        solver.run(args.steps, cfl=args.cfl)
+        solver.run(args.steps, cfl=args.cfl, graph=True)
         torch.cuda.synchronize()
         a_el = time.perf_counter() - a1
         solver.set_geometry(dg.GEOM_NODE)
@@ -461,7 +508,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle
-        cores = host_cores()
+        q_start = solver.get_q()
+        hi = host_info()
+        cores = hi["threads"]
         o = oracle.Oracle(dg.generate_operators(N), oracle.freestream(gam, mach, 0.0))
         case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
                            mesh.bedgeNum, mesh.bedge_type)
@@ -479,22 +528,38 @@ def main():
         cpu = {"value": cpu_val, "unit": "DoF-updates/s per RK stage", "cores": cores, "kind": "port",
                "sample": f"oracle OpenMP restatement (reference op2 openmp loop structure), same "
                          f"{n}x{n} mesh and state, {args.cpu_steps} SSPRK3 steps incl. calc_dt, "
-                         f"{c1 - c0:.1f}s wall",
+                         f"{c1 - c0:.1f}s wall on {cores} threads (every CPU of the process's affinity "
+                         f"mask, OMP_PROC_BIND={hi['omp_proc_bind']}, OMP_PLACES={hi['omp_places']})",
                "kernel_s": dict(zip(["get_neighbour_q", "get_bedge_q", "euler_rhs", "set_workingQ",
                                      "update_Q", "calc_dt"], [round(float(v), 3) for v in kt_cpu])),
-               "value_1thread": dof * 3 / (s1 - s0), "cpu_model": cpu_model()}
+               "value_1thread": dof * 3 / (s1 - s0), "host": hi}
         print(f"[bench] cpu baseline: {cpu_val:.4e} DoF-updates/s per RK stage on {cores} threads, "
-              f"{cpu['value_1thread']:.4e} on 1 ({cpu['cpu_model']})", file=sys.stderr, flush=True)
+              f"{cpu['value_1thread']:.4e} on 1 ({hi['cpu_model']}, {hi['online_cpus']} CPUs online)",
+              file=sys.stderr, flush=True)
+        del st, st1, case
+
+    del solver
+    batch = adjoint = None
+    if not args.no_batch:
+        batch = batch_leg(N, 64, 500, 0.25)
+        print(f"[bench] batch: {batch['value']:.3f}x, bit-identical {batch['bit_identical_all_cases']}",
+              file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(N, 400, 20, checks=True)
+        print(f"[bench] adjoint: {adjoint['value']:.3f}x forward, checks {adjoint.get('checks', {})}",
+              file=sys.stderr, flush=True)
 
     out = {"metric": "DoF-updates/s per RK stage (FP64, N=3)", "value": value,
            "unit": "DoF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic (seeded jittered periodic mesh, analytic vortex state)",
            "config": {"workload": f"isentropic vortex, {n}x{n} jittered periodic grid "
-                                  f"({K} triangles), N={N}, SSPRK3 CFL {args.cfl}, fused stage kernels",
+                                  f"({K} triangles), N={N}, SSPRK3 CFL {args.cfl}, fused stage kernels, "
+                                  "hipGraph-replayed steps",
                       "cells": K, "order": N, "dof": dof, "geometry": args.geometry,
                       "parallelism": "single GPU"},
-           "roofline": roofline, "cpu_baseline": cpu, "launch": tune, "alt": alt}
+           "roofline": roofline, "cpu_baseline": cpu, "batch": batch, "adjoint": adjoint,
+           "launch": tune, "alt": alt}
     print(json.dumps(out))
 
 

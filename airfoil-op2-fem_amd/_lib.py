@@ -34,13 +34,15 @@ _SIGS = {
     "dg_last_error": ([c_void_p], c_char_p),
     "dg_order": ([c_void_p, P(c_int), P(c_int), P(c_int)], c_int),
     "dg_set_stream": ([c_void_p, c_void_p], c_int),
-    "dg_set_variant": ([c_void_p, c_int, c_int], c_int),
+    "dg_set_max_blocks": ([c_void_p, c_int], c_int),
     "dg_generate_operators": ([c_int] + [c_void_p] * 10, c_int),
     "dg_set_const": ([c_void_p, c_char_p, c_int, c_void_p], c_int),
     "dg_get_const": ([c_void_p, c_char_p, c_int, c_void_p], c_int),
     "dg_set_freestream": ([c_void_p, c_double, c_double, c_double, c_double, c_double], c_int),
     "dg_decl_mesh": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                       c_void_p, c_void_p], c_int),
+    "dg_mesh_check": ([c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                       c_void_p], c_int),
     "dg_init_grid": ([c_void_p] + [c_void_p] * 16, c_int),
     "dg_set_ic": ([c_void_p, c_void_p, c_void_p], c_int),
     "dg_neighbour_zero": ([c_void_p, c_void_p], c_int),

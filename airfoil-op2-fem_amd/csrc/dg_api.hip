@@ -76,7 +76,6 @@ Launch launch_of(const dg_ctx *c) {
   L.fmask4 = packed;
   L.cus = c->cus;
   L.max_blocks = c->max_blocks;
-  L.variant = c->variant;
   return L;
 }
 
@@ -174,7 +173,6 @@ int dg_ctx_create(int order, dg_ctx **out) {
     he = hipGetDeviceProperties(&prop, c->device);
     if (he == hipSuccess) c->cus = prop.multiProcessorCount;
   }
-  if (const char *v = std::getenv("DG_STAGE_VARIANT")) c->variant = std::atoi(v) == 1 ? 1 : 0;
   if (he == hipSuccess) he = hipMalloc(&c->d_ops, sizeof(double) * dg::ops_size(c->Np, c->NF));
   if (he == hipSuccess) he = hipMalloc(&c->d_fmask, sizeof(int) * c->NF);
   if (he == hipSuccess) he = hipMalloc(&c->d_red, sizeof(unsigned long long) * 4);
@@ -215,11 +213,9 @@ int dg_order(const dg_ctx *c, int *N, int *Np, int *Nfp) {
   return DG_OK;
 }
 
-int dg_set_variant(dg_ctx *c, int variant, int max_blocks) {
-  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_set_variant: null context");
-  if (variant != 0 && variant != 1) return fail(c, DG_ERR_ARG, "dg_set_variant: variant must be 0 or 1");
-  if (max_blocks < 0) return fail(c, DG_ERR_ARG, "dg_set_variant: max_blocks must be >= 0");
-  c->variant = variant;
+int dg_set_max_blocks(dg_ctx *c, int max_blocks) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_set_max_blocks: null context");
+  if (max_blocks < 0) return fail(c, DG_ERR_ARG, "dg_set_max_blocks: max_blocks must be >= 0");
   c->max_blocks = max_blocks;
   return DG_OK;
 }
@@ -486,6 +482,20 @@ int dg_mesh_ogrid(int ncirc, int nrad, double stretch, double rfar, dg_mesh **ou
     return fail(nullptr, DG_ERR_MESH, "dg_mesh_ogrid: " + err);
   }
   *out = m;
+  return DG_OK;
+}
+
+int dg_mesh_check(int ncells, const int *cell2nodes, int nedges, const int *edge2cells,
+                  const int *edgeNum, int nbedges, const int *bedge2cells, const int *bedgeNum,
+                  const int *bedge_type) {
+  if (ncells < 0 || nedges < 0 || nbedges < 0 || (nedges > 0 && (!edge2cells || !edgeNum)) ||
+      (nbedges > 0 && (!bedge2cells || !bedgeNum || !bedge_type)))
+    return fail(nullptr, DG_ERR_ARG, "dg_mesh_check: bad sizes or null arrays");
+  std::vector<int> fmap;
+  std::string err;
+  if (!dg::build_face_map(ncells, nedges, edge2cells, edgeNum, nbedges, bedge2cells, bedgeNum,
+                          bedge_type, cell2nodes, fmap, err))
+    return fail(nullptr, DG_ERR_MESH, "dg_mesh_check: " + err);
   return DG_OK;
 }
 

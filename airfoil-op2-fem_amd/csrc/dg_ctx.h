@@ -23,7 +23,6 @@ struct dg_ctx {
   int *d_fmask = nullptr;
   int cus = 0;
   int max_blocks = 0;
-  int variant = 0;
   // declared mesh
   int ncells = -1, nedges = 0, nbedges = 0;
   int *d_fmap = nullptr, *d_b2c = nullptr, *d_bnum = nullptr, *d_btype = nullptr;

@@ -26,7 +26,6 @@ struct Launch {
   uint64_t fmask4;     // FMASK[m] in bits 4m..4m+3 (entries < 16, NF <= 15 for N <= 4)
   int cus;             // compute units of the device
   int max_blocks;      // optional cap on persistent grids (0: none)
-  int variant;         // stage kernel: 0 MFMA (wave tiles), 1 LDS/VALU (workgroup tiles)
   // matrix-core kernel launch shape (the autotuner picks per mesh; one-wave
   // workgroups with the XCD-chunked walk are within 1% of the best shape on
   // the 724x724 vortex and the best on 65k-cell meshes)

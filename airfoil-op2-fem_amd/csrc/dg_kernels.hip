@@ -1,24 +1,8 @@
-// HIP kernels for gfx950 (MI355X): nodal DG Euler right-hand side, SSPRK3
-// stage updates, CFL reduction and set-up loops.  FP64 throughout.
-//
-// Work decomposition of the stage kernel (rhs_kernel):
-//   * a workgroup owns a tile of TE consecutive cells; the tile's state
-//     records ([TE][Np][4] doubles, contiguous in the op_dat layout) are
-//     staged in LDS with coalesced 16-byte loads;
-//   * face phase: one lane per (cell, face node) gathers the neighbour trace
-//     straight from the neighbour's state record through the face map
-//     (gather-only, owner computes: no atomics, no colouring), or builds the
-//     boundary state, and evaluates the Roe flux into LDS;
-//   * volume phase: one lane per (cell, node) evaluates the Euler fluxes F, G
-//     into LDS;
-//   * combine phase: one lane per (cell, node) applies the weak derivative
-//     rows of Drw, Dsw, the metric terms and the LIFT row, then runs the
-//     epilogue (write qRHS, or an SSPRK3 stage update, or the final update
-//     fused with the CFL max-reduction).
-// The grid is persistent (a few workgroups per CU) and walks tiles in
-// XCD-contiguous chunks so that a cell and its neighbours are usually
-// processed by workgroups of the same XCD, whose L2 then holds the shared
-// face traces.
+// HIP kernels for gfx950 (MI355X): the drop-in per-set loops of the DG
+// Euler path (face traces, boundary states, SSPRK3 stage updates, CFL
+// reduction), the set-up loops (init_grid, initial states) and the wall
+// forces.  FP64 throughout.  The fused stage kernel (Roe fluxes, volume and
+// LIFT products on the matrix cores, stage epilogues) is in dg_stage_mfma.hip.
 #include <hip/hip_runtime.h>
 
 #include "dg_device.h"
@@ -27,215 +11,12 @@
 namespace dg {
 
 // ------------------------------------------------------------------------
-// Stage kernel
+// Stage kernel: the matrix-core kernel of dg_stage_mfma.hip
 // ------------------------------------------------------------------------
-
-template <int N, int TE, int NT, bool GATHER, int EPI>
-__global__ __launch_bounds__(NT) void rhs_kernel(StageArgs a, const double *__restrict__ ops,
-                                                 const int *__restrict__ fmask, Phys ph,
-                                                 unsigned long long facenodes) {
-  constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
-  constexpr int SW = Np * 4;          // LDS stride of a state record
-  constexpr int SFG = Np * 8 + 2;     // padded so cells fall in different banks
-  constexpr int SFL = NF * 4 + 2;
-  constexpr int NOPS = 2 * Np * Np + Np * NF;
-  __shared__ double sW[TE * SW];
-  __shared__ double sFG[TE * SFG];
-  __shared__ double sFl[TE * SFL];
-  __shared__ double sOp[NOPS];
-  __shared__ int sFm[NF];
-  __shared__ unsigned long long sRed[NT / 64];
-
-  const int tid = threadIdx.x;
-  for (int i = tid; i < NOPS; i += NT) sOp[i] = ops[i];
-  if (tid < NF) sFm[tid] = fmask[tid];
-  const double *sDrw = sOp, *sDsw = sOp + Np * Np, *sLIFT = sOp + 2 * Np * Np;
-
-  double dt = 0.0;
-  if (EPI != EPI_RHS) dt = a.dt_fixed ? *a.dt_fixed : a.cfl / __longlong_as_double((long long)*a.dt1_cur);
-  if (EPI == EPI_STAGE0 && blockIdx.x == 0 && tid == 0) *a.dt1_next = 0ull;
-  unsigned long long cfl_bits = 0ull;
-  bool bad = false;
-  const double n1sq = (double)((N + 1) * (N + 1));
-
-  const int ntiles = (a.ncells + TE - 1) / TE;
-  for (int k = 0;; k++) {
-    const int tile = xcd_tile(k, ntiles);
-    if (tile >= ntiles) break;
-    const int e0 = tile * TE;
-    const int ne = min(TE, a.ncells - e0);
-    __syncthreads();  // LDS reuse across tiles (and sOp/sFm on the first pass)
-    {
-      const double2 *src = reinterpret_cast<const double2 *>(a.w + (size_t)e0 * SW);
-      double2 *dst = reinterpret_cast<double2 *>(sW);
-      for (int i = tid; i < ne * SW / 2; i += NT) dst[i] = src[i];
-    }
-    __syncthreads();
-
-    // face phase: Roe flux per (cell, face node)
-    for (int it = tid; it < ne * NF; it += NT) {
-      const int e = it / NF, m = it - e * NF;
-      const int f = m / Nfp, i = m - f * Nfp;
-      const size_t gm = (size_t)(e0 + e) * NF + m;
-      const double nxv = a.nx[gm], nyv = a.ny[gm], fs = a.fscale[gm];
-      double mQ[4], pQ[4];
-      const double *ms = &sW[e * SW + sFm[m] * 4];
-      mQ[0] = ms[0]; mQ[1] = ms[1]; mQ[2] = ms[2]; mQ[3] = ms[3];
-      if (GATHER) {
-        const int code = a.fmap[(size_t)(e0 + e) * 3 + f];
-        if (code >= 0) {
-          const int nb = code >> 2, nf = code & 3;
-          const int node = sFm[nf * Nfp + (Nfp - 1 - i)];
-          const double2 *src = reinterpret_cast<const double2 *>(a.w + ((size_t)nb * Np + node) * 4);
-          double2 p01 = src[0], p23 = src[1];
-          pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
-        } else {
-          boundary_state(ph, -code - 1, mQ, nxv, nyv, pQ);
-        }
-      } else {
-        const double2 *src = reinterpret_cast<const double2 *>(a.ext + gm * 4);
-        double2 p01 = src[0], p23 = src[1];
-        pQ[0] = p01.x; pQ[1] = p01.y; pQ[2] = p23.x; pQ[3] = p23.y;
-      }
-      double fl[4];
-      roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
-      double *d = &sFl[e * SFL + m * 4];
-      d[0] = fl[0]; d[1] = fl[1]; d[2] = fl[2]; d[3] = fl[3];
-    }
-    // volume phase: F, G per (cell, node)
-    for (int it = tid; it < ne * Np; it += NT) {
-      const int e = it / Np, i = it - e * Np;
-      double qv[4], F[4], G[4];
-      const double *s = &sW[e * SW + i * 4];
-      qv[0] = s[0]; qv[1] = s[1]; qv[2] = s[2]; qv[3] = s[3];
-      euler_flux(ph.gam, qv, F, G);
-      double *d = &sFG[e * SFG + i * 8];
-      d[0] = F[0]; d[1] = F[1]; d[2] = F[2]; d[3] = F[3];
-      d[4] = G[0]; d[5] = G[1]; d[6] = G[2]; d[7] = G[3];
-    }
-    __syncthreads();
-    if (!GATHER) {  // drop-in euler_rhs zeroes exteriorQ on exit (OP_RW)
-      double2 *z = reinterpret_cast<double2 *>(a.ext + (size_t)e0 * NF * 4);
-      for (int i = tid; i < ne * NF * 2; i += NT) z[i] = make_double2(0.0, 0.0);
-    }
-
-    // combine phase
-    for (int it = tid; it < ne * Np; it += NT) {
-      const int e = it / Np, j = it - e * Np;
-      const size_t gn = (size_t)(e0 + e) * Np + j;
-      double dFdr[4] = {0, 0, 0, 0}, dFds[4] = {0, 0, 0, 0};
-      double dGdr[4] = {0, 0, 0, 0}, dGds[4] = {0, 0, 0, 0};
-      const double *fg = &sFG[e * SFG];
-#pragma unroll
-      for (int i = 0; i < Np; i++) {
-        const double ar = sDrw[j * Np + i], as = sDsw[j * Np + i];
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const double Fv = fg[i * 8 + v], Gv = fg[i * 8 + 4 + v];
-          dFdr[v] += ar * Fv;
-          dFds[v] += as * Fv;
-          dGdr[v] += ar * Gv;
-          dGds[v] += as * Gv;
-        }
-      }
-      const double rxv = a.rx[gn], sxv = a.sx[gn], ryv = a.ry[gn], syv = a.sy[gn];
-      double res[4], lift[4] = {0, 0, 0, 0};
-      const double *fl = &sFl[e * SFL];
-#pragma unroll
-      for (int m = 0; m < NF; m++) {
-        const double l = sLIFT[j * NF + m];
-#pragma unroll
-        for (int v = 0; v < 4; v++) lift[v] += l * fl[m * 4 + v];
-      }
-#pragma unroll
-      for (int v = 0; v < 4; v++)
-        res[v] = ((rxv * dFdr[v] + sxv * dFds[v]) + (ryv * dGdr[v] + syv * dGds[v])) - lift[v];
-
-      double o[4];
-      const double *wv = &sW[e * SW + j * 4];
-      if (EPI == EPI_RHS) {
-#pragma unroll
-        for (int v = 0; v < 4; v++) o[v] = res[v];
-      } else if (EPI == EPI_STAGE0) {
-#pragma unroll
-        for (int v = 0; v < 4; v++) o[v] = wv[v] + dt * res[v];
-      } else {
-        const double2 *qs = reinterpret_cast<const double2 *>(a.q + gn * 4);
-        double2 q01 = qs[0], q23 = qs[1];
-        double qv[4] = {q01.x, q01.y, q23.x, q23.y};
-        if (EPI == EPI_STAGE1) {
-#pragma unroll
-          for (int v = 0; v < 4; v++) o[v] = 0.75 * qv[v] + 0.25 * (wv[v] + dt * res[v]);
-        } else {
-#pragma unroll
-          for (int v = 0; v < 4; v++) o[v] = (qv[v] + 2.0 * (wv[v] + dt * res[v])) * (1.0 / 3.0);
-        }
-      }
-      double2 *dst = reinterpret_cast<double2 *>(a.out + gn * 4);
-      dst[0] = make_double2(o[0], o[1]);
-      dst[1] = make_double2(o[2], o[3]);
-      if (EPI == EPI_STAGE2) {
-        if ((facenodes >> j) & 1ull) {
-          double c = cfl_node(ph.gam, n1sq, o, a.fscale[(size_t)(e0 + e) * NF + j]) + 0.0;
-          unsigned long long b = (unsigned long long)__double_as_longlong(c);
-          cfl_bits = max_bits(cfl_bits, b);
-        }
-        bad |= !(isfinite(o[0]) && isfinite(o[1]) && isfinite(o[2]) && isfinite(o[3])) ||
-               !(o[0] > 0.0);
-      }
-    }
-  }
-  if (EPI == EPI_STAGE2) {
-    unsigned long long r = block_max_bits<NT>(cfl_bits, sRed);
-    if (tid == 0) atomic_max_if_larger(a.dt1_next, r);
-    if (bad) atomicOr(a.flag, 1u);
-    if (blockIdx.x == 0 && tid == 0) *a.t_dev += dt;
-  }
-}
-
-template <int N>
-struct TileCfg {
-  static constexpr int TE = 32;
-  static constexpr int NT0 = Ord<N>::NF * TE;
-  static constexpr int NT = NT0 <= 256 ? 256 : (NT0 <= 384 ? 384 : (NT0 <= 512 ? 512 : 1024));
-};
-
-template <int N, bool GATHER, int EPI>
-static hipError_t launch_rhs_t(const Launch &L, const StageArgs &a) {
-  constexpr int TE = TileCfg<N>::TE, NT = TileCfg<N>::NT;
-  const int ntiles = (a.ncells + TE - 1) / TE;
-  int grid = L.cus > 0 ? 2 * L.cus : 512;
-  if (L.max_blocks > 0 && grid > L.max_blocks) grid = L.max_blocks;
-  if (grid > ntiles) grid = ntiles;
-  if (grid >= 8) grid -= grid % 8;
-  if (grid < 1) return hipSuccess;
-  hipLaunchKernelGGL((rhs_kernel<N, TE, NT, GATHER, EPI>), dim3(grid), dim3(NT), 0, L.stream, a,
-                     L.ops, L.fmask, L.ph, (unsigned long long)L.facenodes);
-  return hipGetLastError();
-}
-
-template <int N>
-static hipError_t launch_rhs_n(const Launch &L, const StageArgs &a, bool gather, int epi) {
-  if (!gather) return launch_rhs_t<N, false, EPI_RHS>(L, a);
-  switch (epi) {
-    case EPI_RHS: return launch_rhs_t<N, true, EPI_RHS>(L, a);
-    case EPI_STAGE0: return launch_rhs_t<N, true, EPI_STAGE0>(L, a);
-    case EPI_STAGE1: return launch_rhs_t<N, true, EPI_STAGE1>(L, a);
-    case EPI_STAGE2: return launch_rhs_t<N, true, EPI_STAGE2>(L, a);
-  }
-  return hipErrorInvalidValue;
-}
 
 hipError_t launch_rhs(const Launch &L, const StageArgs &a, bool gather, int epi) {
   if (!gather && epi != EPI_RHS) return hipErrorInvalidValue;
-  if (L.variant == 0) return launch_rhs_mfma(L, a, gather, epi);
-  switch (L.N) {
-    case 1: return launch_rhs_n<1>(L, a, gather, epi);
-    case 2: return launch_rhs_n<2>(L, a, gather, epi);
-    case 3: return launch_rhs_n<3>(L, a, gather, epi);
-    case 4: return launch_rhs_n<4>(L, a, gather, epi);
-  }
-  return hipErrorInvalidValue;
+  return launch_rhs_mfma(L, a, gather, epi);
 }
 
 // ------------------------------------------------------------------------

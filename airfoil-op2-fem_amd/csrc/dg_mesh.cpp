@@ -278,24 +278,29 @@ bool read_vtk(const std::string &path, Mesh &m, std::string &err) {
   std::vector<int> types;
   while (in >> tok) {
     if (tok == "POINTS") {
-      long n; std::string ty;
+      long n = -1; std::string ty;
       in >> n >> ty;
+      if (!in || n < 0 || n > (1l << 30)) { err = "bad POINTS header in " + path; return false; }
       pts.resize(3 * n);
       for (long i = 0; i < 3 * n; i++) in >> pts[i];
     } else if (tok == "CELLS") {
-      long n, sz;
+      long n = -1, sz = -1;
       in >> n >> sz;
+      if (!in || n < 0 || n > (1l << 29)) { err = "bad CELLS header in " + path; return false; }
       cellv.resize(n);
-      for (long i = 0; i < n; i++) {
-        int cnt; in >> cnt;
+      for (long i = 0; i < n && in; i++) {
+        int cnt = -1; in >> cnt;
+        if (!in || cnt < 1 || cnt > 64) { err = "bad cell record in " + path; return false; }
         cellv[i].resize(cnt);
         for (int j = 0; j < cnt; j++) in >> cellv[i][j];
       }
     } else if (tok == "CELL_TYPES") {
-      long n; in >> n;
+      long n = -1; in >> n;
+      if (!in || n < 0 || n > (1l << 29)) { err = "bad CELL_TYPES header in " + path; return false; }
       types.resize(n);
       for (long i = 0; i < n; i++) in >> types[i];
     }
+    if (in.fail()) { err = "truncated or malformed legacy VTK file " + path; return false; }
   }
   if (pts.empty() || cellv.empty() || types.size() != cellv.size()) {
     err = "malformed legacy VTK file " + path;
@@ -310,10 +315,21 @@ bool read_vtk(const std::string &path, Mesh &m, std::string &err) {
   for (size_t c = 0; c < cellv.size(); c++) {
     if (types[c] != 5 || cellv[c].size() != 3) continue;
     int p1 = cellv[c][0], p2 = cellv[c][1], p3 = cellv[c][2];
+    for (int p : {p1, p2, p3})
+      if (p < 0 || p >= m.nnodes) {
+        err = "cell " + std::to_string(c) + " names point " + std::to_string(p) + " of " +
+              std::to_string(m.nnodes);
+        return false;
+      }
     const double *x = m.coords.data();
+    // tools/vtk2CGNS.cpp:114-125: a clockwise triangle is turned round
     double val = (x[2 * p2] - x[2 * p1]) * (x[2 * p3 + 1] - x[2 * p2 + 1]) -
                  (x[2 * p2 + 1] - x[2 * p1 + 1]) * (x[2 * p3] - x[2 * p2]);
     if (val < 0) std::swap(p1, p3);
+    if (!(val != 0.0)) {
+      err = "cell " + std::to_string(c) + " has zero area";
+      return false;
+    }
     m.cells.push_back(p1); m.cells.push_back(p2); m.cells.push_back(p3);
   }
   m.ncells = (int)(m.cells.size() / 3);

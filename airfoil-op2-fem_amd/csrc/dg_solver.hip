@@ -214,8 +214,6 @@ int dg_solver_create_batch(dg_ctx *c, int ncases, const dg_mesh *const *meshes, 
   if (!c || !meshes || !out || ncases < 1)
     return fail(c, DG_ERR_ARG, "dg_solver_create_batch: null argument or ncases < 1");
   *out = nullptr;
-  if (c->variant != 0 && ncases > 1)
-    return fail(c, DG_ERR_ARG, "dg_solver_create_batch: batches need the matrix-core stage kernel (variant 0)");
   dg_solver *s = new dg_solver();
   s->ctx = c;
   s->ncases = ncases;
@@ -461,8 +459,6 @@ static int run_impl(dg_solver *s, int nsteps, double cfl, int mode, bool graph) 
     return fail(c, DG_ERR_ARG, "dg_solver_run: unknown mode");
   if (mode == DG_MODE_REFERENCE && s->ncases != 1)
     return fail(c, DG_ERR_ARG, "dg_solver_run: the reference loop sequence runs one case");
-  if (s->ncases > 1 && c->variant != 0)
-    return fail(c, DG_ERR_ARG, "dg_solver_run: batches need the matrix-core stage kernel (variant 0)");
   const dg::Launch L = launch_s(s);
   const int K = s->K;
   const size_t nstate = (size_t)K * c->Np * 4;
@@ -773,9 +769,6 @@ int dg_solver_set_launch(dg_solver *s, int wpb, int walk, int max_blocks) {
 int dg_solver_set_geometry(dg_solver *s, int mode) {
   if (!s || (mode != DG_GEOM_CELL && mode != DG_GEOM_NODE))
     return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_set_geometry: mode is DG_GEOM_CELL or DG_GEOM_NODE");
-  if (mode == DG_GEOM_CELL && s->ctx->variant != 0)
-    return fail(s->ctx, DG_ERR_ARG,
-                "dg_solver_set_geometry: DG_GEOM_CELL needs the matrix-core stage kernel (variant 0)");
   drop_graph(s);
   s->cell_geometry = mode == DG_GEOM_CELL;
   return DG_OK;
@@ -797,7 +790,6 @@ int dg_solver_launch_config(const dg_solver *s, int *wpb, int *walk, int *max_bl
 int dg_solver_autotune(dg_solver *s, int trial_steps, double cfl, double *table, int *ntable) {
   if (!s || trial_steps < 1 || !(cfl > 0.0)) return fail(s ? s->ctx : nullptr, DG_ERR_ARG, "dg_solver_autotune: bad argument");
   dg_ctx *c = s->ctx;
-  if (c->variant != 0) return fail(c, DG_ERR_ARG, "dg_solver_autotune: tunes the matrix-core stage kernel (variant 0)");
   const size_t nstate = (size_t)s->K * c->Np * 4;
   if (!s->dt_valid) {
     HIPCHECK(c, compute_dt1(s, (int)(s->step & 1)));

@@ -95,6 +95,18 @@ def naca0012_ogrid(ncirc, nrad, stretch=1.08, rfar=20.0):
     return Mesh(h.value)
 
 
+def mesh_check(cells, edge2cells, edgeNum, bedge2cells, bedgeNum, bedge_type):
+    """dg_mesh_check on host arrays: raises DGError (DG_ERR_MESH) with the
+    reason when the connectivity is not a valid counter-clockwise mesh
+    (cells may be None to skip the orientation check)."""
+    def arr(a):
+        return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+    c, e2c, en, b2c, bn, bt = (arr(a) for a in (cells, edge2cells, edgeNum, bedge2cells, bedgeNum, bedge_type))
+    ncells = len(c) if c is not None else int(max(e2c.max() if e2c.size else -1, b2c.max() if b2c.size else -1)) + 1
+    check(lib.dg_mesh_check(int(ncells), _np_ptr(c) if c is not None else None, int(e2c.size // 2),
+                            _np_ptr(e2c), _np_ptr(en), int(b2c.size), _np_ptr(b2c), _np_ptr(bn), _np_ptr(bt)))
+
+
 def sweep_cases(ncases=64, seed=3):
     """The batched sweep's seeded uneven O-grid cases: (ncirc, nrad, mach,
     alpha_deg) with 2 * ncirc * nrad in about [8k, 130k] cells, alpha ~
@@ -170,19 +182,9 @@ class Context:
         ptr = getattr(stream, "cuda_stream", stream)
         self._chk(lib.dg_set_stream(self._h, c_void_p(int(ptr) if ptr else 0)))
 
-    def set_variant(self, variant, max_blocks=0):
-        """Stage kernel: 0 matrix-core (default), 1 vector-ALU."""
-        self._chk(lib.dg_set_variant(self._h, int(variant), int(max_blocks)))
-        self._variant = int(variant)
-
-    @property
-    def variant(self):
-        """Stage kernel in use (the DG_STAGE_VARIANT environment variable
-        selects it at context creation)."""
-        v = getattr(self, "_variant", None)
-        if v is None:
-            v = 1 if os.environ.get("DG_STAGE_VARIANT") == "1" else 0
-        return v
+    def set_max_blocks(self, max_blocks=0):
+        """Cap on the stage kernels' persistent grid (0: occupancy-limited)."""
+        self._chk(lib.dg_set_max_blocks(self._h, int(max_blocks)))
 
     def _dim(self, name):
         if name in self._DBL1:

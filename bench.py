@@ -440,7 +440,7 @@ def main():
           file=sys.stderr, flush=True)
 
     tune = None
-    if not args.no_autotune and ctx.variant == 0:
+    if not args.no_autotune:
         table = solver.autotune(trial_steps=3, cfl=args.cfl)
         tune = dict(solver.launch_config(), table_ms_per_step={
             f"wpb{r['wpb']}_walk{r['walk']}_w{r['waves_per_cu']}": round(r["ms_per_step"], 4) for r in table})
@@ -489,7 +489,7 @@ def main():
     # leave the reference's by init_grid's per-node rounding scatter), for
     # comparison only
     alt = None
-    if args.geometry == "node" and ctx.variant == 0 and args.steps > 0:
+    if args.geometry == "node" and args.steps > 0:
         solver.set_geometry(dg.GEOM_CELL)
         solver.run(max(args.warmup, 1), cfl=args.cfl, graph=True)
         torch.cuda.synchronize()

@@ -71,13 +71,10 @@ void dg_ctx_destroy(dg_ctx *ctx);
  * failed call that had no context (creation, mesh generation). */
 const char *dg_last_error(const dg_ctx *ctx);
 int dg_order(const dg_ctx *ctx, int *N, int *Np, int *Nfp);
-/* Stage-kernel variant (no reference counterpart): 0 = matrix-core kernel
- * (one wave per 16-cell tile, v_mfma_f64_16x16x4f64 for the volume and LIFT
- * products; default), 1 = vector-ALU kernel (workgroup tiles of 32 cells,
- * contractions from LDS).  max_blocks > 0 caps the persistent grid (0 = one
- * full wave of resident workgroups).  Both give the same results to
- * rounding.  The environment variable DG_STAGE_VARIANT sets the default. */
-int dg_set_variant(dg_ctx *ctx, int variant, int max_blocks);
+/* Cap on the persistent grid of the stage kernels (no reference
+ * counterpart): max_blocks > 0 limits the workgroups of a launch, 0 (the
+ * default) launches as many as the compute units hold resident. */
+int dg_set_max_blocks(dg_ctx *ctx, int max_blocks);
 /* Stream for all later work of ctx (a hipStream_t; NULL = null stream). */
 int dg_set_stream(dg_ctx *ctx, void *hip_stream);
 
@@ -109,6 +106,15 @@ int dg_set_freestream(dg_ctx *ctx, double gam, double mach, double alpha_deg,
 int dg_decl_mesh(dg_ctx *ctx, int ncells, const int *cell2nodes, int nedges,
                  const int *edge2cells, const int *edgeNum, int nbedges,
                  const int *bedge2cells, const int *bedgeNum, const int *bedge_type);
+/* The connectivity checks of dg_decl_mesh on the host alone (no device, no
+ * context): every (cell, local edge) covered exactly once by the edge and
+ * boundary-edge lists, indices and boundary types in range, and, with
+ * cell2nodes, neighbours traversing their shared edge in opposite
+ * directions (a clockwise cell among counter-clockwise ones fails).
+ * DG_ERR_MESH with the reason in dg_last_error(NULL). */
+int dg_mesh_check(int ncells, const int *cell2nodes, int nedges, const int *edge2cells,
+                  const int *edgeNum, int nbedges, const int *bedge2cells,
+                  const int *bedgeNum, const int *bedge_type);
 
 /* ------------------------------------------------------------------ */
 /* The reference's per-set loops (drop-ins for op_par_loop_<kernel>)   */
@@ -296,8 +302,7 @@ int dg_solver_launch_config(const dg_solver *s, int *wpb, int *walk, int *max_bl
  * relative, 1e-12 and more on fine or thin cells, and DG_GEOM_CELL does not
  * carry that scatter: its results leave the reference's by that much (about
  * 2e-12 relative in rho v after 100 steps of the 64x64 vortex).  The
- * geometry is fixed when the solver is created; DG_GEOM_CELL needs the
- * matrix-core stage kernel (context variant 0, else DG_ERR_ARG). */
+ * geometry factors are formed when the solver is created. */
 #define DG_GEOM_NODE 0
 #define DG_GEOM_CELL 1
 int dg_solver_set_geometry(dg_solver *s, int mode);

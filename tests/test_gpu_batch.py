@@ -20,9 +20,6 @@ def single(dg, mesh, mach, alpha):
 
 
 def test_batch_cases_bit_identical_to_single_runs(dg):
-    import os
-    if os.environ.get("DG_STAGE_VARIANT") == "1":
-        pytest.skip("batches run the matrix-core stage kernel only")
     specs = [(24, 6, 0.35, -2.0), (40, 10, 0.5, 4.0), (32, 8, 0.45, 7.5)]
     meshes = [dg.naca0012_ogrid(nc, nr, 1.15, 15.0) for nc, nr, _, _ in specs]
     ctx = dg.Context(3)
@@ -42,9 +39,6 @@ def test_failed_case_does_not_reach_other_cases(dg):
     """A case whose state turns non-finite stays confined to itself: the
     other cases of the batch still match their single runs bit for bit
     (the stage kernel re-zeroes its padded operand columns on every tile)."""
-    import os
-    if os.environ.get("DG_STAGE_VARIANT") == "1":
-        pytest.skip("batches run the matrix-core stage kernel only")
     specs = [(24, 6, 0.4, 1.0), (16, 5, 0.45, 2.0), (40, 10, 0.5, -3.0), (20, 4, 0.3, 0.5)]
     meshes = [dg.naca0012_ogrid(nc, nr, 1.15, 15.0) for nc, nr, _, _ in specs]
     ctx = dg.Context(3)
@@ -75,9 +69,6 @@ def test_launch_shapes_and_autotune_keep_results(dg):
     """Every launch shape of the matrix-core stage kernel (waves per
     workgroup, walk order, grid cap) gives the same bits, and the autotuner
     restores the state it tried its shapes from."""
-    import os
-    if os.environ.get("DG_STAGE_VARIANT") == "1":
-        pytest.skip("launch shapes belong to the matrix-core stage kernel")
     ref = _vortex(dg)
     ref.run(5, cfl=0.5)
     q_ref, t_ref = ref.get_q(), ref.time()
@@ -100,9 +91,6 @@ def test_launch_shapes_and_autotune_keep_results(dg):
 
 
 def test_batch_launch_shapes_bit_identical(dg):
-    import os
-    if os.environ.get("DG_STAGE_VARIANT") == "1":
-        pytest.skip("batches run the matrix-core stage kernel only")
     specs = [(24, 6, 0.35, -2.0), (40, 10, 0.5, 4.0), (32, 8, 0.45, 7.5)]
     meshes = [dg.naca0012_ogrid(nc, nr, 1.15, 15.0) for nc, nr, _, _ in specs]
     out = []

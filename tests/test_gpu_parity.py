@@ -208,10 +208,6 @@ def test_vortex_100_steps_match_oracle(dg, vortex64, vortex64_cell, mode, geom):
     geometry, and the nine-loop reference sequence (mode 1), each against the
     oracle on the same geometry."""
     mesh, ctx, s = _vortex_solver(dg, 64)
-    if geom == "cell" and ctx.variant != 0:
-        with pytest.raises(dg.DGError):
-            s.set_geometry(dg.GEOM_CELL)
-        pytest.skip("per-cell geometry is a matrix-core kernel mode")
     s.set_geometry(dg.GEOM_CELL if geom == "cell" else dg.GEOM_NODE)
     ref = vortex64_cell if geom == "cell" else vortex64
     assert np.array_equal(s.get_q(), vortex64["q0"]) or rel(s.get_q(), vortex64["q0"]) < 1e-15
@@ -315,8 +311,6 @@ def test_airfoil_ogrid_short_run_matches_oracle(dg, orc, geom):
     ctx = dg.Context(3)
     ctx.set_freestream(GAM, 0.4, 2.0, 1.0, 1.0)
     s = dg.Solver(ctx, mesh)
-    if geom == "cell" and ctx.variant != 0:
-        pytest.skip("per-cell geometry is a matrix-core kernel mode")
     s.set_geometry(dg.GEOM_CELL if geom == "cell" else dg.GEOM_NODE)
     q0 = s.get_q()
     s.run(50, cfl=1.0)
@@ -360,8 +354,6 @@ def test_fused_solver_other_orders_match_oracle(dg, orc, order, geom):
     ctx = dg.Context(order)
     ctx.set_freestream(GAM, 0.4, 3.0, 1.0, 1.0)
     s = dg.Solver(ctx, mesh)
-    if geom == "cell" and ctx.variant != 0:
-        pytest.skip("per-cell geometry is a matrix-core kernel mode")
     s.set_geometry(dg.GEOM_CELL if geom == "cell" else dg.GEOM_NODE)
     q0 = s.get_q()
     s.run(20, cfl=0.5)

@@ -163,3 +163,89 @@ def test_bad_order_rejected_without_device(dg):
         dg.Context(order=0)
     with pytest.raises(dg.DGError):
         dg.Context(order=7)
+
+
+def _mesh_arrays(m):
+    return [np.array(getattr(m, k)) for k in ("cells", "edge2cells", "edgeNum", "bedge2cells", "bedgeNum",
+                                             "bedge_type")]
+
+
+def test_mesh_check_accepts_generated_meshes(dg):
+    for m in (dg.periodic_grid(6, 5, seed=3), dg.naca0012_ogrid(24, 6)):
+        dg.mesh_check(*_mesh_arrays(m))
+
+
+def _expect_mesh_error(dg, arrays, words):
+    with pytest.raises(dg.DGError) as ei:
+        dg.mesh_check(*arrays)
+    assert ei.value.code == -5, ei.value  # DG_ERR_MESH
+    assert all(w in str(ei.value) for w in words), str(ei.value)
+
+
+def test_mesh_check_rejects_faulty_connectivity(dg):
+    """A clockwise cell among counter-clockwise ones, an edge listed twice, a
+    face covered by no edge, and out-of-range cells, local edges and
+    boundary types: each is DG_ERR_MESH with its reason, found on the host
+    before any device work (dg_decl_mesh runs the same check)."""
+    m = dg.naca0012_ogrid(24, 6)
+    cells, e2c, en, b2c, bn, bt = _mesh_arrays(m)
+    cw = cells.copy()
+    cw[7, [1, 2]] = cw[7, [2, 1]]  # cell 7 turned clockwise
+    _expect_mesh_error(dg, [cw, e2c, en, b2c, bn, bt], ["opposite directions"])
+    _expect_mesh_error(dg, [cells, np.vstack([e2c, e2c[3:4]]), np.vstack([en, en[3:4]]), b2c, bn, bt],
+                       ["assigned twice"])
+    _expect_mesh_error(dg, [cells, e2c, en, np.concatenate([b2c, b2c[:1]]), np.concatenate([bn, bn[:1]]),
+                            np.concatenate([bt, bt[:1]])], ["assigned twice", "boundary edge"])
+    _expect_mesh_error(dg, [cells, e2c, en, b2c[1:], bn[1:], bt[1:]], ["neither a neighbour nor a boundary edge"])
+    bad = bt.copy()
+    bad[0] = 9
+    _expect_mesh_error(dg, [cells, e2c, en, b2c, bn, bad], ["out-of-range"])
+    badn = en.copy()
+    badn[0, 1] = 3
+    _expect_mesh_error(dg, [cells, e2c, badn, b2c, bn, bt], ["out-of-range"])
+
+
+def _write_vtk(path, pts, tris):
+    with open(path, "w") as f:
+        f.write("# vtk DataFile Version 2.0\ntest\nASCII\nDATASET UNSTRUCTURED_GRID\n")
+        f.write(f"POINTS {len(pts)} double\n")
+        for p in pts:
+            f.write(f"{float(p[0])!r} {float(p[1])!r} 0.0\n")
+        f.write(f"CELLS {len(tris)} {4 * len(tris)}\n")
+        for t in tris:
+            f.write(f"3 {t[0]} {t[1]} {t[2]}\n")
+        f.write(f"CELL_TYPES {len(tris)}\n" + "5\n" * len(tris))
+
+
+def test_vtk_reader_turns_clockwise_cells_and_rejects_faults(dg, tmp_path):
+    """tools/vtk2CGNS.cpp:114-125 turns a clockwise triangle round; the
+    reader does too.  A boundary edge that src/load_mesh.cpp:147-172's rules
+    do not classify, an edge shared by three cells, a point index past the
+    POINTS list, a degenerate cell and a truncated file are DG_ERR_MESH."""
+    # unit square split into two triangles, one listed clockwise; its boundary
+    # lies in the inner box of the classification (wall, type 2)
+    pts = [(0, 0), (0.5, 0), (0.5, 0.5), (0, 0.5)]
+    p = tmp_path / "sq.vtk"
+    _write_vtk(p, pts, [(0, 1, 2), (0, 3, 2)])
+    m = dg.read_vtk(str(p))
+    assert m.ncells == 2 and np.all(_areas(m.cell_xy) > 0)
+    assert np.all(m.bedge_type == 2)
+    dg.mesh_check(*_mesh_arrays(m))
+
+    def bad(name, pts, tris, words, text=None):
+        q = tmp_path / name
+        if text is None:
+            _write_vtk(q, pts, tris)
+        else:
+            q.write_text(text)
+        with pytest.raises(dg.DGError) as ei:
+            dg.read_vtk(str(q))
+        assert ei.value.code == -5 and all(w in str(ei.value) for w in words), str(ei.value)
+    # a slanted boundary edge outside the inner box: no rule classifies it
+    bad("slant.vtk", [(0, 0), (3, 0.5), (0, 2)], [(0, 1, 2)], ["unclassified boundary edge"])
+    bad("fan.vtk", [(0, 0), (1, 0), (0.5, 1), (0.5, -1), (2, 0.2)], [(0, 1, 2), (1, 0, 3), (0, 1, 4)],
+        ["more than two cells"])
+    bad("range.vtk", pts, [(0, 1, 7)], ["names point 7"])
+    bad("flat.vtk", [(0, 0), (1, 0), (2, 0)], [(0, 1, 2)], ["zero area"])
+    bad("trunc.vtk", None, None, ["truncated"],
+        text="# vtk DataFile Version 2.0\nt\nASCII\nDATASET UNSTRUCTURED_GRID\nPOINTS 4 double\n0 0 0\n1 0 0\n")

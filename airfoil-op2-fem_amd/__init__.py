@@ -15,6 +15,6 @@ The directory name is not an identifier; import it with
 """
 from ._lib import DGError, lib, lib_path  # noqa: F401
 from .dg import (  # noqa: F401
-    BC_FARFIELD, BC_INFLOW, BC_OUTFLOW, BC_WALL, GEOM_CELL, GEOM_NODE, MODE_FUSED, MODE_REFERENCE,
-    BatchSolver, Context, Mesh, Sensitivity, Solver, generate_operators, naca0012_ogrid, periodic_grid,
+    BC_EXACT, BC_FARFIELD, BC_INFLOW, BC_OUTFLOW, BC_WALL, GEOM_CELL, GEOM_NODE, MODE_FUSED, MODE_REFERENCE,
+    BatchSolver, Context, Mesh, Sensitivity, Solver, generate_operators, naca0012_ogrid, periodic_grid, rect_grid,
     mesh_check, read_vtk, sweep_cases)

@@ -244,6 +244,9 @@ bool find_const(dg_ctx *c, const char *name, ConstSlot &s) {
       {"s", &c->s, nullptr, Np},            {"Dr", &c->Dr, nullptr, Np * Np},
       {"Ds", &c->Ds, nullptr, Np * Np},     {"Drw", &c->Drw, nullptr, Np * Np},
       {"Dsw", &c->Dsw, nullptr, Np * Np},   {"LIFT", &c->LIFT, nullptr, Np * NF},
+      // the vortex app's constants (vortex/src/vortex.cpp:171-174)
+      {"vortex_x0", nullptr, &c->vortex.x0, 1}, {"vortex_y0", nullptr, &c->vortex.y0, 1},
+      {"vortex_beta", nullptr, &c->vortex.beta, 1},
   };
   for (auto &t : tab)
     if (std::strcmp(t.n, name) == 0) {
@@ -339,6 +342,8 @@ int dg_decl_mesh(dg_ctx *c, int ncells, const int *cell2nodes, int nedges, const
     HIPCHECK(c, hipMemcpy(c->d_btype, bedge_type, sizeof(int) * nbedges, hipMemcpyHostToDevice));
   }
   c->ncells = ncells; c->nedges = nedges; c->nbedges = nbedges;
+  c->has_exact = false;
+  for (int b = 0; b < nbedges; b++) c->has_exact |= bedge_type[b] == dg::kBcExact;
   return DG_OK;
 }
 
@@ -393,6 +398,22 @@ int dg_get_bedge_q(dg_ctx *c, const double *nx, const double *ny, const double *
   return DG_OK;
 }
 
+int dg_get_bedge_q_exact(dg_ctx *c, const double *x, const double *y, double t, double *ext) {
+  NEED_MESH(c, "dg_get_bedge_q_exact");
+  if (c->nbedges > 0 && (!x || !y || !ext)) return fail(c, DG_ERR_ARG, "dg_get_bedge_q_exact: null array");
+  HIPCHECK(c, dg::launch_exact_bc(launch_of(c), c->nbedges, c->d_b2c, c->d_bnum, nullptr, x, y, c->vortex, t,
+                                  nullptr, nullptr, 0.0, nullptr, 0.0, true, ext));
+  return DG_OK;
+}
+
+int dg_set_exact_state(dg_ctx *c, double uinf, double beta, double x0, double y0, double period_x) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, "dg_set_exact_state: null context");
+  if (!(period_x >= 0.0)) return fail(c, DG_ERR_ARG, "dg_set_exact_state: period_x must be >= 0");
+  c->vortex.uinf = uinf; c->vortex.beta = beta; c->vortex.x0 = x0; c->vortex.y0 = y0;
+  c->vortex.period = period_x;
+  return DG_OK;
+}
+
 int dg_euler_rhs(dg_ctx *c, const double *w, double *ext, const double *rx, const double *ry,
                  const double *sx, const double *sy, const double *fscale, const double *nx,
                  const double *ny, double *qRHS) {
@@ -415,6 +436,10 @@ int dg_rhs_fused(dg_ctx *c, const double *w, const double *rx, const double *ry,
   if (c->ncells > 0 && (!w || !rx || !ry || !sx || !sy || !fscale || !nx || !ny || !qRHS))
     return fail(c, DG_ERR_ARG, "dg_rhs_fused: null array");
   if (!al16(w) || !al16(qRHS)) return fail(c, DG_ERR_ARG, "dg_rhs_fused: state arrays must be 16-byte aligned");
+  if (c->has_exact)
+    return fail(c, DG_ERR_ARG, "dg_rhs_fused: the mesh has exact-state boundaries (type 4), whose states come "
+                               "from dg_get_bedge_q_exact; use dg_get_neighbour_q, dg_get_bedge_q, "
+                               "dg_get_bedge_q_exact and dg_euler_rhs");
   dg::StageArgs a{};
   a.w = w; a.rx = rx; a.ry = ry; a.sx = sx; a.sy = sy; a.fscale = fscale; a.nx = nx; a.ny = ny;
   a.fmap = c->d_fmap; a.out = qRHS; a.ncells = c->ncells;
@@ -455,6 +480,23 @@ int dg_calc_dt(dg_ctx *c, const double *q, const double *fscale, double *dt1) {
 }
 
 /* ---------------------------- meshes ---------------------------- */
+
+int dg_mesh_rect(int nx, int ny, double x0, double y0, double Lx, double Ly, uint64_t seed, double jitter,
+                 int bc_type, dg_mesh **out) {
+  if (!out) return fail(nullptr, DG_ERR_ARG, "dg_mesh_rect: null output");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || !(Lx > 0) || !(Ly > 0) || !(jitter >= 0.0) || jitter >= 0.5 || bc_type < 0 ||
+      bc_type > 4)
+    return fail(nullptr, DG_ERR_ARG, "dg_mesh_rect: need nx, ny >= 1, L > 0, 0 <= jitter < 0.5, type in [0, 4]");
+  dg_mesh *m = new dg_mesh();
+  std::string err;
+  if (!dg::gen_rect(nx, ny, x0, y0, Lx, Ly, seed, jitter, bc_type, m->m, err)) {
+    delete m;
+    return fail(nullptr, DG_ERR_MESH, "dg_mesh_rect: " + err);
+  }
+  *out = m;
+  return DG_OK;
+}
 
 int dg_mesh_periodic(int nx, int ny, double x0, double y0, double Lx, double Ly, uint64_t seed,
                      double jitter, dg_mesh **out) {

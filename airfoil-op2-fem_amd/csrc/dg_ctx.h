@@ -28,6 +28,9 @@ struct dg_ctx {
   int *d_fmap = nullptr, *d_b2c = nullptr, *d_bnum = nullptr, *d_btype = nullptr;
   unsigned long long *d_red = nullptr;
   int *d_node_slots = nullptr;  // [Np][2] face slots reading each node (adjoint)
+  // exact isentropic-vortex state of boundary type 4 (vortex/src/get_bedge_q.h)
+  dg::Vortex vortex;
+  bool has_exact = false;  // the declared mesh has boundary edges of type 4
 };
 
 struct dg_mesh {

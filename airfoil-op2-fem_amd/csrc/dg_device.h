@@ -176,6 +176,28 @@ __device__ __forceinline__ void farfield_state(const Phys &ph, const double qi[4
   qb[3] = pb / (g - 1.0) + 0.5 * rhob * (ub * ub + vb * vb);
 }
 
+// Isentropic vortex state at (x, y) and time t (vortex/src/set_ic.h:1-28,
+// with the advection by uinf in both velocity components and an optional
+// period in x).
+__device__ __forceinline__ void vortex_state(double gam, const Vortex &vx, double t, double x, double y,
+                                             double q[4]) {
+  const double pi = 3.14159265358979323846;
+  double dx = x - vx.x0 - vx.uinf * t;
+  if (vx.period > 0.0) dx -= vx.period * floor(dx / vx.period + 0.5);
+  const double dy = y - vx.y0;
+  const double r2 = dx * dx + dy * dy;
+  const double e1 = exp(1.0 - r2);
+  const double u = vx.uinf - vx.beta * e1 * (dy / (2 * pi));
+  const double v = vx.beta * e1 * (dx / (2 * pi));
+  double rho = 1.0 - ((gam - 1) / (16 * gam * pi * pi)) * vx.beta * vx.beta * exp(2 * (1.0 - r2));
+  rho = pow(rho, 1.0 / (gam - 1));
+  const double p = pow(rho, gam);
+  q[0] = rho;
+  q[1] = rho * u;
+  q[2] = rho * v;
+  q[3] = (p / (gam - 1)) + 0.5 * rho * (u * u + v * v);
+}
+
 // Exterior state on a boundary face node (src/get_bedge_q.h:24-50 + type 3).
 __device__ __forceinline__ void boundary_state(const Phys &ph, int type, const double mQ[4],
                                                double nx, double ny, double qb[4]) {

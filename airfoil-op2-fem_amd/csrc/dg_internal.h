@@ -11,6 +11,17 @@ struct Phys {
   double gam, bc_r, bc_u, bc_v, bc_e, bc_p;
 };
 
+// Boundary type whose exterior state is the exact isentropic vortex at the
+// face node and stage time (vortex/src/get_bedge_q.h:1-40).
+constexpr int kBcExact = 4;
+
+// Isentropic vortex (vortex/src/set_ic.h:1-28 with free-stream speed uinf,
+// advected with it, periodic in x with period `period` when it is > 0):
+// centre (x0, y0), strength beta.
+struct Vortex {
+  double uinf = 0.0, beta = 5.0, x0 = 5.0, y0 = 0.0, period = 0.0;
+};
+
 // Layout of the per-context operator buffer on the device (doubles):
 //   Drw[Np*Np] | Dsw[Np*Np] | LIFT[Np*NF] | Dr[Np*Np] | Ds[Np*Np] | r[Np] | s[Np]
 inline int ops_size(int Np, int NF) { return 4 * Np * Np + Np * NF + 2 * Np; }
@@ -78,6 +89,17 @@ constexpr int kMfmaTile = 16;
 enum Epi { EPI_RHS = 0, EPI_STAGE0 = 1, EPI_STAGE1 = 2, EPI_STAGE2 = 3 };
 
 hipError_t launch_rhs(const Launch &L, const StageArgs &a, bool gather, int epi);
+// Exact vortex states on boundary edges (vortex/src/get_bedge_q.h:1-40):
+// for boundary edges b in [0, nb) whose type is kBcExact (every edge when
+// btype is null), at each face node m of (b2c[b], bnum[b]) the vortex state
+// at (x, y) of node FMASK[m] and time t goes to ext[(cell * NF + m) * 4]:
+// added (accumulate) or stored.  t = t_host when t_dev is null, else
+// t_dev[0] + frac * dt with dt = dt_fixed[0] or cfl / dt1[0] (the stage time
+// of a fused step, read on the device).
+hipError_t launch_exact_bc(const Launch &L, int nb, const int *b2c, const int *bnum, const int *btype,
+                           const double *x, const double *y, const Vortex &v, double t_host,
+                           const double *t_dev, const unsigned long long *dt1, double cfl,
+                           const double *dt_fixed, double frac, bool accumulate, double *ext);
 hipError_t launch_rhs_mfma(const Launch &L, const StageArgs &a, bool gather, int epi);
 hipError_t launch_get_neighbour_q(const Launch &L, int ncells, const int *fmap,
                                   const double *w, double *ext);

@@ -62,11 +62,45 @@ __global__ void get_bedge_q_kernel(int nbedges, int Np, int Nfp, const int *__re
     const size_t c = b2c[b];
     const int f = bnum[b];
     const int m = f * Nfp + i;
+    if (btype[b] == kBcExact) continue;  // exact states: exact_bc_kernel
     const double *mq = w + ((size_t)c * Np + fmask[m]) * 4;
     double mQ[4] = {mq[0], mq[1], mq[2], mq[3]}, qb[4];
     boundary_state(ph, btype[b], mQ, nx[c * NF + m], ny[c * NF + m], qb);
     double *d = ext + (c * NF + m) * 4;
     d[0] += qb[0]; d[1] += qb[1]; d[2] += qb[2]; d[3] += qb[3];
+  }
+}
+
+// vortex/src/get_bedge_q.h:1-40: one lane per (boundary edge, face node);
+// the exact vortex state at the node's (x, y) and time t, added to or stored
+// in the face slot.  The time of a fused stage is read on the device.
+__global__ void exact_bc_kernel(int nb, int Np, int Nfp, const int *__restrict__ b2c,
+                                const int *__restrict__ bnum, const int *__restrict__ btype,
+                                const int *__restrict__ fmask, double gam, const double *__restrict__ x,
+                                const double *__restrict__ y, Vortex vx, double t_host,
+                                const double *__restrict__ t_dev, const unsigned long long *__restrict__ dt1,
+                                double cfl, const double *__restrict__ dt_fixed, double frac, int accumulate,
+                                double *__restrict__ ext) {
+  const int NF = 3 * Nfp;
+  double t = t_host;
+  if (t_dev) t = t_dev[0] + frac * (dt_fixed ? dt_fixed[0] : cfl / __longlong_as_double((long long)dt1[0]));
+  const size_t n = (size_t)nb * Nfp;
+  for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < n;
+       it += (size_t)gridDim.x * blockDim.x) {
+    const size_t b = it / Nfp;
+    if (btype && btype[b] != kBcExact) continue;
+    const int i = (int)(it - b * Nfp);
+    const size_t c = b2c[b];
+    const int m = bnum[b] * Nfp + i;
+    const size_t node = c * Np + fmask[m];
+    double qb[4];
+    vortex_state(gam, vx, t, x[node], y[node], qb);
+    double *d = ext + (c * NF + m) * 4;
+    if (accumulate) {
+      d[0] += qb[0]; d[1] += qb[1]; d[2] += qb[2]; d[3] += qb[3];
+    } else {
+      d[0] = qb[0]; d[1] = qb[1]; d[2] = qb[2]; d[3] = qb[3];
+    }
   }
 }
 
@@ -205,20 +239,12 @@ __global__ void set_ic_kernel(size_t nnodes, Phys ph, double *q, double *w) {
 __global__ void set_vortex_kernel(size_t nnodes, double gam, const double *__restrict__ x,
                                   const double *__restrict__ y, double uinf, double beta,
                                   double x0, double y0, double period_x, double *q, double *w) {
-  const double pi = 3.14159265358979323846;
+  Vortex vx;
+  vx.uinf = uinf; vx.beta = beta; vx.x0 = x0; vx.y0 = y0; vx.period = period_x;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nnodes;
        i += (size_t)gridDim.x * blockDim.x) {
-    double dx = x[i] - x0;
-    if (period_x > 0.0) dx -= period_x * floor(dx / period_x + 0.5);
-    double dy = y[i] - y0;
-    double r2 = dx * dx + dy * dy;
-    double e1 = exp(1.0 - r2);
-    double u = uinf - beta * e1 * (dy / (2 * pi));
-    double v = beta * e1 * (dx / (2 * pi));
-    double rho = 1.0 - ((gam - 1) / (16 * gam * pi * pi)) * beta * beta * exp(2 * (1.0 - r2));
-    rho = pow(rho, 1.0 / (gam - 1));
-    double p = pow(rho, gam);
-    double qv[4] = {rho, rho * u, rho * v, (p / (gam - 1)) + 0.5 * rho * (u * u + v * v)};
+    double qv[4];
+    vortex_state(gam, vx, 0.0, x[i], y[i], qv);
     for (int k = 0; k < 4; k++) { q[4 * i + k] = qv[k]; w[4 * i + k] = qv[k]; }
   }
 }
@@ -366,6 +392,18 @@ hipError_t launch_set_ic(const Launch &L, int ncells, double *q, double *w) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(set_ic_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, L.stream, n, L.ph,
                      q, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_exact_bc(const Launch &L, int nb, const int *b2c, const int *bnum, const int *btype,
+                           const double *x, const double *y, const Vortex &v, double t_host,
+                           const double *t_dev, const unsigned long long *dt1, double cfl,
+                           const double *dt_fixed, double frac, bool accumulate, double *ext) {
+  const size_t n = (size_t)nb * L.Nfp;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(exact_bc_kernel, dim3(grid_for(n, 128, 4096)), dim3(128), 0, L.stream, nb, L.Np, L.Nfp,
+                     b2c, bnum, btype, L.fmask, L.ph.gam, x, y, v, t_host, t_dev, dt1, cfl, dt_fixed, frac,
+                     accumulate ? 1 : 0, ext);
   return hipGetLastError();
 }
 

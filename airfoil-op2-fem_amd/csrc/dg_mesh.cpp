@@ -63,6 +63,55 @@ bool gen_periodic(int nx, int ny, double x0, double y0, double Lx, double Ly,
   return build_edges(m, err);
 }
 
+bool gen_rect(int nx, int ny, double x0, double y0, double Lx, double Ly, uint64_t seed, double jitter,
+              int bc_type, Mesh &m, std::string &err) {
+  if (nx < 1 || ny < 1) {
+    err = "rectangle grid needs at least 1x1 quads";
+    return false;
+  }
+  if ((long long)nx * ny * 2 > (1ll << 29)) {
+    err = "rectangle grid too large";
+    return false;
+  }
+  if (bc_type < 0 || bc_type > 4) {
+    err = "boundary type must be in [0, 4]";
+    return false;
+  }
+  const double hx = Lx / nx, hy = Ly / ny;
+  const int nx1 = nx + 1, ny1 = ny + 1;
+  m = Mesh();
+  m.nnodes = nx1 * ny1;
+  m.coords.resize(2 * (size_t)m.nnodes);
+  uint64_t st = seed;
+  for (int j = 0; j < ny1; j++)
+    for (int i = 0; i < nx1; i++) {
+      double dx = (2.0 * uniform01(st) - 1.0) * jitter * hx;
+      double dy = (2.0 * uniform01(st) - 1.0) * jitter * hy;
+      const bool edge = i == 0 || j == 0 || i == nx || j == ny;
+      size_t n = (size_t)j * nx1 + i;
+      m.coords[2 * n] = x0 + i * hx + (edge ? 0.0 : dx);
+      m.coords[2 * n + 1] = y0 + j * hy + (edge ? 0.0 : dy);
+    }
+  m.ncells = 2 * nx * ny;
+  m.cells.resize(3 * (size_t)m.ncells);
+  m.cell_xy.resize(6 * (size_t)m.ncells);
+  auto put = [&](size_t k, int i, int j, int slot) {
+    const int n = j * nx1 + i;
+    m.cells[3 * k + slot] = n;
+    m.cell_xy[6 * k + 2 * slot] = m.coords[2 * (size_t)n];
+    m.cell_xy[6 * k + 2 * slot + 1] = m.coords[2 * (size_t)n + 1];
+  };
+  size_t k = 0;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      put(k, i, j, 0); put(k, i + 1, j, 1); put(k, i + 1, j + 1, 2); k++;
+      put(k, i, j, 0); put(k, i + 1, j + 1, 1); put(k, i, j + 1, 2); k++;
+    }
+  if (!build_edges(m, err)) return false;
+  for (int &t : m.bedge_type) t = bc_type;
+  return true;
+}
+
 double naca0012_thickness(double x) {
   if (x <= 0.0) return 0.0;
   return 0.594689181 * (0.298222773 * std::sqrt(x) - 0.127125232 * x -
@@ -249,7 +298,7 @@ bool build_face_map(int ncells, int nedges, const int *edge2cells, const int *ed
   }
   for (int b = 0; b < nbedges; b++) {
     int c = bedge2cells[b], f = bedgeNum[b], t = bedge_type ? bedge_type[b] : 0;
-    if (c < 0 || c >= ncells || f < 0 || f > 2 || t < 0 || t > 3) {
+    if (c < 0 || c >= ncells || f < 0 || f > 2 || t < 0 || t > 4) {
       err = "boundary edge " + std::to_string(b) + " has an out-of-range cell, edge or type";
       return false;
     }

@@ -52,6 +52,13 @@ bool gen_periodic(int nx, int ny, double x0, double y0, double Lx, double Ly,
 bool gen_ogrid(int ncirc, int nrad, double stretch, double rfar, Mesh &m,
                std::string &err);
 
+// Jittered nx x ny quad grid over [x0, x0+Lx] x [y0, y0+Ly] (not periodic),
+// each quad split SW->NE into two CCW triangles; vertices off the boundary
+// jittered by U(-jitter*h, jitter*h) from splitmix64(seed); every boundary
+// edge gets type bc_type.
+bool gen_rect(int nx, int ny, double x0, double y0, double Lx, double Ly, uint64_t seed, double jitter,
+              int bc_type, Mesh &m, std::string &err);
+
 // The same O-grid with the surface points of line k moved by surf_dy[k] in
 // y; interior nodes of line k move by (1 - rho_j) surf_dy[k] (rho_j the
 // radial blending parameter), the far-field ring stays fixed.

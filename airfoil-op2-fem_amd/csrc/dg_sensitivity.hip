@@ -960,6 +960,8 @@ int dg_rhs_tangent(dg_ctx *c, const double *w, const double *wd, const double *r
   if (c->ncells < 0) return fail(c, DG_ERR_STATE, "dg_rhs_tangent: no mesh declared");
   if (c->ncells > 0 && (!w || !wd || !rx || !ry || !sx || !sy || !fscale || !nx || !ny || !scratch || !out))
     return fail(c, DG_ERR_ARG, "dg_rhs_tangent: null array");
+  if (c->has_exact)
+    return fail(c, DG_ERR_ARG, "dg_rhs_tangent: exact-state boundaries (type 4) have no linearisation here");
   dg::TlmArgs t{};
   t.ncells = c->ncells; t.w = w; t.wd = wd;
   t.rx = rx; t.ry = ry; t.sx = sx; t.sy = sy; t.nx = nx; t.ny = ny; t.fscale = fscale;
@@ -983,6 +985,8 @@ int rhs_adjoint_impl(dg_ctx *c, const char *fn, const double *w, const double *m
                      const double *fscale, const double *nx, const double *ny, double *slots,
                      double *acc, double *grx, double *gry, double *gsx, double *gsy, double *gfs,
                      double *gnx, double *gny, double *gbcu, double *gbcv) {
+  if (c->has_exact)
+    return fail(c, DG_ERR_ARG, std::string(fn) + ": exact-state boundaries (type 4) have no adjoint here");
   if (c->ncells > 0 && (!w || !mu || !rx || !ry || !sx || !sy || !fscale || !nx || !ny || !slots ||
                         !acc || !grx || !gry || !gsx || !gsy || !gfs || !gnx || !gny || !gbcu || !gbcv))
     return fail(c, DG_ERR_ARG, std::string(fn) + ": null array");

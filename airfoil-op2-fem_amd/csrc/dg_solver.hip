@@ -41,6 +41,9 @@ struct dg_solver {
   // concatenated connectivity (global cell indices)
   int *d_fmap = nullptr, *d_b2c = nullptr, *d_bnum = nullptr, *d_btype = nullptr;
   int nbedges = 0;
+  // boundary edges of type 4 (exact vortex state, single-case solvers): each
+  // fused stage first stores their states at the stage time in exteriorQ
+  bool has_exact = false;
   std::map<std::string, double *> dats;
   double *q = nullptr, *w = nullptr, *rk[3] = {nullptr, nullptr, nullptr}, *ext = nullptr;
   unsigned long long *dt1 = nullptr;  // [2][ncases]
@@ -173,20 +176,35 @@ dg::StageArgs stage_args(dg_solver *s, double cfl) {
   return a;
 }
 
-// One fused SSPRK3 step (three launches) for the step parity p.
+// Exact vortex states of the type-4 boundary slots at the fused stage's time
+// t + frac dt (t and dt read on the device), stored in exteriorQ.
+hipError_t exact_states(dg_solver *s, const dg::Launch &L, const dg::StageArgs &a, double frac) {
+  if (!s->has_exact) return hipSuccess;
+  return dg::launch_exact_bc(L, s->nbedges, s->d_b2c, s->d_bnum, s->d_btype, dat(s, "x"), dat(s, "y"),
+                             s->ctx->vortex, 0.0, s->t_dev, a.dt1_cur, a.cfl, nullptr, frac, false, s->ext);
+}
+
+// One fused SSPRK3 step (three launches, plus one small one per stage on
+// meshes with exact-state boundaries) for the step parity p.  The stages
+// evaluate the RHS at t, t + dt and t + dt/2 (vortex/src/vortex.cpp:228).
 hipError_t fused_step(dg_solver *s, int p, double cfl) {
   const dg::Launch L = launch_s(s);
   dg::StageArgs a = stage_args(s, cfl);
   a.dt1_cur = s->dt1 + (size_t)p * s->ncases;
   a.dt1_next = s->dt1 + (size_t)(p ^ 1) * s->ncases;
+  if (s->has_exact) a.ext = s->ext;
   double *wA = s->w, *wB = s->rk[0];
   a.w = s->q; a.out = wA;
-  hipError_t e = timed(s, "rhs_stage0", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE0); });
+  hipError_t e = exact_states(s, L, a, 0.0);
+  if (e == hipSuccess) e = timed(s, "rhs_stage0", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE0); });
   if (e != hipSuccess) return e;
   a.w = wA; a.out = wB;
-  e = timed(s, "rhs_stage1", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE1); });
+  e = exact_states(s, L, a, 1.0);
+  if (e == hipSuccess) e = timed(s, "rhs_stage1", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE1); });
   if (e != hipSuccess) return e;
   a.w = wB; a.out = s->q;
+  e = exact_states(s, L, a, 0.5);
+  if (e != hipSuccess) return e;
   return timed(s, "rhs_stage2", [&] { return dg::launch_rhs(L, a, true, dg::EPI_STAGE2); });
 }
 
@@ -245,6 +263,7 @@ int dg_solver_create_batch(dg_ctx *c, int ncases, const dg_mesh *const *meshes, 
       b2c.push_back(M.bedge2cells[b] + off);
       bnum.push_back(M.bedgeNum[b]);
       btype.push_back(M.bedge_type[b]);
+      s->has_exact |= M.bedge_type[b] == dg::kBcExact;
       if (M.bedge_type[b] == DG_BC_WALL) {
         s->h_wcell.push_back(M.bedge2cells[b] + off);
         s->h_wface.push_back(M.bedgeNum[b]);
@@ -264,6 +283,11 @@ int dg_solver_create_batch(dg_ctx *c, int ncases, const dg_mesh *const *meshes, 
       s->phys.push_back(dg::launch_of(c).ph);
       s->alpha.push_back(c->bc_alpha);
     }
+  }
+  if (s->has_exact && ncases > 1) {
+    delete s;
+    return fail(c, DG_ERR_ARG, "dg_solver_create_batch: exact-state boundaries (type 4) run in single-case "
+                               "solvers");
   }
   s->K = (int)ktot;
   s->nbedges = (int)b2c.size();
@@ -369,8 +393,12 @@ int dg_solver_ncases(const dg_solver *s, int *ncases) {
 int dg_solver_set_vortex(dg_solver *s, double uinf, double beta, double x0, double y0,
                          double period_x) {
   if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_set_vortex: null solver");
+  if (!(period_x >= 0.0)) return fail(s->ctx, DG_ERR_ARG, "dg_solver_set_vortex: period_x must be >= 0");
   HIPCHECK(s->ctx, dg::launch_set_vortex(launch_s(s), s->K, dat(s, "x"), dat(s, "y"), uinf, beta,
                                          x0, y0, period_x, s->q, s->w));
+  // the same vortex is the exact state of type-4 boundaries
+  s->ctx->vortex.uinf = uinf; s->ctx->vortex.beta = beta; s->ctx->vortex.x0 = x0;
+  s->ctx->vortex.y0 = y0; s->ctx->vortex.period = period_x;
   s->dt_valid = false;
   return DG_OK;
 }
@@ -502,7 +530,10 @@ static int run_impl(dg_solver *s, int nsteps, double cfl, int mode, bool graph) 
     if (nsteps > 0) HIPCHECK(c, dg::launch_copy(L, nstate, s->q, s->w));
     return DG_OK;
   }
-  // DG_MODE_REFERENCE: the nine loops of src/airfoil.cpp:252-334, dt on the host
+  // DG_MODE_REFERENCE: the nine loops of src/airfoil.cpp:252-334, dt on the host.
+  // exteriorQ is zero between loops (euler_rhs zeroes it on exit); fused
+  // steps leave the exact states of type-4 slots in it, so clear it first.
+  if (s->has_exact) HIPCHECK(c, dg::launch_zero(L, (size_t)K * c->NF * 4, s->ext));
   unsigned long long *slot[2] = {s->dt1, s->dt1 + 1};
   double t = 0.0, dt1 = 0.0;
   unsigned long long bits = 0;
@@ -512,6 +543,7 @@ static int run_impl(dg_solver *s, int nsteps, double cfl, int mode, bool graph) 
   std::memcpy(&dt1, &bits, sizeof(dt1));
   double dt = cfl / dt1;
   double *fscale = dat(s, "fscale");
+  const double frac[3] = {0.0, 1.0, 0.5};  // stage times (vortex/src/vortex.cpp:228,248-258)
   for (int n = 0; n < nsteps; n++) {
     for (int j = 0; j < 3; j++) {
       HIPCHECK(c, timed(s, "get_neighbour_q", [&] { return dg::launch_get_neighbour_q(L, K, s->d_fmap, s->w, s->ext); }));
@@ -519,6 +551,12 @@ static int run_impl(dg_solver *s, int nsteps, double cfl, int mode, bool graph) 
         return dg::launch_get_bedge_q(L, s->nbedges, s->d_b2c, s->d_bnum, s->d_btype, dat(s, "nx"),
                                       dat(s, "ny"), s->w, s->ext);
       }));
+      if (s->has_exact)
+        HIPCHECK(c, timed(s, "get_bedge_q_exact", [&] {
+          return dg::launch_exact_bc(L, s->nbedges, s->d_b2c, s->d_bnum, s->d_btype, dat(s, "x"), dat(s, "y"),
+                                     c->vortex, t + frac[j] * dt, nullptr, nullptr, 0.0, nullptr, 0.0, true,
+                                     s->ext);
+        }));
       dg::StageArgs a{};
       a.w = s->w; a.ext = s->ext; a.rx = dat(s, "rx"); a.ry = dat(s, "ry"); a.sx = dat(s, "sx");
       a.sy = dat(s, "sy"); a.fscale = fscale; a.nx = dat(s, "nx"); a.ny = dat(s, "ny");

@@ -302,14 +302,16 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         }
         // own and exterior trace loads issued together, before any branch:
         // a boundary slot reads its own record as a stand-in exterior, which
-        // boundary_state then replaces
+        // boundary_state then replaces; an exact-state slot reads the state
+        // exact_bc_kernel stored in its exteriorQ slot for this stage
         const size_t own = cell * Np + fmask_of(m);
         const bool interior = GATHER && code[j] >= 0;
+        const bool exact = GATHER && code[j] == -(1 + kBcExact);
         const size_t nrec =
             interior ? (size_t)(code[j] >> 2) * Np + fmask_of((code[j] & 3) * Nfp + (Nfp - 1 - i)) : own;
         const double2 *ms = reinterpret_cast<const double2 *>(a.w + own * 4);
-        const double2 *ps = GATHER ? reinterpret_cast<const double2 *>(a.w + nrec * 4)
-                                   : reinterpret_cast<const double2 *>(a.ext + gm * 4);
+        const double2 *ps = (GATHER && !exact) ? reinterpret_cast<const double2 *>(a.w + nrec * 4)
+                                               : reinterpret_cast<const double2 *>(a.ext + gm * 4);
         const double2 m01 = ms[0], m23 = ms[1];
         const double2 p01 = ps[0], p23 = ps[1];
         double mQ[4] = {m01.x, m01.y, m23.x, m23.y};
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
           double2 *z = reinterpret_cast<double2 *>(a.ext + gm * 4);
           z[0] = make_double2(0.0, 0.0);  // OP_RW: euler_rhs zeroes exteriorQ
           z[1] = make_double2(0.0, 0.0);
-        } else if (!interior) {
+        } else if (!interior && !exact) {
           boundary_state(ph, -code[j] - 1, mQ, nxv, nyv, pQ);
         }
         double fl[4];

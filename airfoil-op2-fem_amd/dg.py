@@ -13,7 +13,7 @@ import numpy as np
 
 from ._lib import DGError, P, c_double, c_int, c_void_p, check, lib
 
-BC_INFLOW, BC_OUTFLOW, BC_WALL, BC_FARFIELD = 0, 1, 2, 3
+BC_INFLOW, BC_OUTFLOW, BC_WALL, BC_FARFIELD, BC_EXACT = 0, 1, 2, 3, 4
 MODE_FUSED, MODE_REFERENCE = 0, 1
 GEOM_NODE, GEOM_CELL = 0, 1
 
@@ -88,6 +88,15 @@ def periodic_grid(nx, ny, x0=0.0, y0=-5.0, lx=10.0, ly=10.0, seed=0, jitter=0.1)
     return Mesh(h.value)
 
 
+def rect_grid(nx, ny, x0=0.0, y0=-5.0, lx=10.0, ly=10.0, seed=0, jitter=0.1, bc_type=BC_EXACT):
+    """Seeded jittered nx*ny quad grid (not periodic), 2*nx*ny CCW triangles,
+    every boundary edge of type bc_type (default: exact vortex state)."""
+    h = c_void_p()
+    check(lib.dg_mesh_rect(int(nx), int(ny), float(x0), float(y0), float(lx), float(ly), int(seed),
+                           float(jitter), int(bc_type), ctypes.byref(h)))
+    return Mesh(h.value)
+
+
 def naca0012_ogrid(ncirc, nrad, stretch=1.08, rfar=20.0):
     """O-grid around the NACA0012 profile: 2*ncirc*nrad CCW triangles."""
     h = c_void_p()
@@ -154,7 +163,8 @@ def _dev(t, name, numel=None):
 class Context:
     """op_init + op_decl_const + op_decl_map, and the op_par_loop drop-ins."""
 
-    _DBL1 = ("gam", "bc_mach", "bc_alpha", "bc_p", "bc_r", "bc_u", "bc_v", "bc_e")
+    _DBL1 = ("gam", "bc_mach", "bc_alpha", "bc_p", "bc_r", "bc_u", "bc_v", "bc_e", "vortex_x0",
+             "vortex_y0", "vortex_beta")
 
     def __init__(self, order=3, stream=None):
         h = c_void_p()
@@ -262,6 +272,17 @@ class Context:
         self._chk(lib.dg_get_bedge_q(self._h, _dev(nx, "nx", K * NF), _dev(ny, "ny", K * NF),
                                      _dev(workingQ, "workingQ", self.state_numel()),
                                      _dev(exteriorQ, "exteriorQ", K * NF * 4)))
+
+    def get_bedge_q_exact(self, x, y, t, exteriorQ):
+        """vortex/src/get_bedge_q.h over every boundary edge at time t."""
+        K, Np, NF = self.ncells, self.Np, self.NF
+        self._chk(lib.dg_get_bedge_q_exact(self._h, _dev(x, "x", K * Np), _dev(y, "y", K * Np), float(t),
+                                           _dev(exteriorQ, "exteriorQ", K * NF * 4)))
+
+    def set_exact_state(self, uinf, beta=5.0, x0=5.0, y0=0.0, period_x=0.0):
+        """Vortex of the exact-state boundaries and get_bedge_q_exact."""
+        self._chk(lib.dg_set_exact_state(self._h, float(uinf), float(beta), float(x0), float(y0),
+                                         float(period_x)))
 
     def euler_rhs(self, workingQ, exteriorQ, rx, ry, sx, sy, fscale, nx, ny, qRHS):
         K, Np, NF, n = self.ncells, self.Np, self.NF, self.state_numel()

@@ -50,8 +50,11 @@ enum {
   DG_ERR_NONFINITE = -6  /* non-finite or non-positive state detected */
 };
 
-/* Boundary-condition types of bedge_type (src/load_mesh.cpp:147-172) */
-enum { DG_BC_INFLOW = 0, DG_BC_OUTFLOW = 1, DG_BC_WALL = 2, DG_BC_FARFIELD = 3 };
+/* Boundary-condition types of bedge_type: 0-2 of src/load_mesh.cpp:147-172;
+ * 3 a characteristic far field (no reference counterpart); 4 the exact
+ * isentropic-vortex state at the face node and stage time, the boundary of
+ * the reference's vortex case (vortex/src/get_bedge_q.h:1-40). */
+enum { DG_BC_INFLOW = 0, DG_BC_OUTFLOW = 1, DG_BC_WALL = 2, DG_BC_FARFIELD = 3, DG_BC_EXACT = 4 };
 
 typedef struct dg_ctx dg_ctx;
 typedef struct dg_mesh dg_mesh;
@@ -89,8 +92,9 @@ int dg_generate_operators(int order, double *r, double *s, double *Dr, double *D
 /* Replaces op_decl_const2 -> op_decl_const_char (src/airfoil_op.cpp:283-299,
  * src/cuda/airfoil_kernels.cu:33-107).  Names: gam, bc_mach, bc_alpha, bc_p,
  * bc_r, bc_u, bc_v, bc_e (dim 1, double); ones, r, s (Np); Dr, Ds, Drw, Dsw
- * (Np*Np); LIFT (Np*3*Nfp); FMASK (3*Nfp, int).  dim must match.  Unknown
- * names return DG_ERR_NAME instead of exiting. */
+ * (Np*Np); LIFT (Np*3*Nfp); FMASK (3*Nfp, int); and the vortex case's
+ * vortex_x0, vortex_y0, vortex_beta (dim 1, vortex/src/vortex.cpp:171-174).
+ * dim must match.  Unknown names return DG_ERR_NAME instead of exiting. */
 int dg_set_const(dg_ctx *ctx, const char *name, int dim, const void *host_data);
 int dg_get_const(const dg_ctx *ctx, const char *name, int dim, void *host_out);
 /* Free-stream set-up of src/airfoil.cpp:79-99: bc_u, bc_v from Mach and the
@@ -138,9 +142,24 @@ int dg_neighbour_zero(dg_ctx *ctx, double *d_exteriorQ);
  * declared topology (always reversed on counter-clockwise meshes). */
 int dg_get_neighbour_q(dg_ctx *ctx, const double *d_nodeX, const double *d_nodeY,
                        const double *d_workingQ, double *d_exteriorQ);
-/* src/get_bedge_q.h:1-51 over boundary edges (OP_INC), types 0-3. */
+/* src/get_bedge_q.h:1-51 over boundary edges (OP_INC), types 0-3 (edges of
+ * type 4 are left to dg_get_bedge_q_exact). */
 int dg_get_bedge_q(dg_ctx *ctx, const double *d_nx, const double *d_ny,
                    const double *d_workingQ, double *d_exteriorQ);
+/* The vortex case's boundary loop (vortex/src/get_bedge_q.h:1-40) over every
+ * boundary edge (OP_INC): exteriorQ += the exact isentropic-vortex state at
+ * the face nodes' coordinates (d_x, d_y: init_grid's [ncells][Np]) and time
+ * t, the stage time t + {0, 1, 1/2} dt of vortex/src/vortex.cpp:228,248-258.
+ * The vortex is dg_set_exact_state's (u_inf advects it in both components;
+ * the reference's own formula hardcodes u_inf = 1 and drops the -t in v). */
+int dg_get_bedge_q_exact(dg_ctx *ctx, const double *d_x, const double *d_y, double t,
+                         double *d_exteriorQ);
+/* Isentropic vortex of the exact-state boundaries (type 4) and
+ * dg_get_bedge_q_exact: free-stream speed uinf in x, strength beta, centre
+ * (x0, y0) at t = 0, periodic in x with period_x (0: not periodic).
+ * Defaults: 0, 5, 5, 0, 0; dg_solver_set_vortex sets the same values. */
+int dg_set_exact_state(dg_ctx *ctx, double uinf, double beta, double x0, double y0,
+                       double period_x);
 /* src/euler_rhs.h:222-291 over cells: volume term, Roe surface flux, LIFT;
  * zeroes exteriorQ on exit (OP_RW) and writes qRHS (OP_WRITE). */
 int dg_euler_rhs(dg_ctx *ctx, const double *d_workingQ, double *d_exteriorQ,
@@ -178,6 +197,12 @@ int dg_rhs_fused(dg_ctx *ctx, const double *d_workingQ, const double *d_rx,
  * by U(-jitter*h, jitter*h) from splitmix64(seed). */
 int dg_mesh_periodic(int nx, int ny, double x0, double y0, double Lx, double Ly,
                      uint64_t seed, double jitter, dg_mesh **out);
+/* Seeded jittered nx*ny quad grid over [x0,x0+Lx]x[y0,y0+Ly], not periodic,
+ * quads split SW->NE into two CCW triangles, vertices off the boundary
+ * jittered as dg_mesh_periodic's; every boundary edge gets type bc_type
+ * (DG_BC_EXACT for the vortex case with exact-state boundaries). */
+int dg_mesh_rect(int nx, int ny, double x0, double y0, double Lx, double Ly, uint64_t seed,
+                 double jitter, int bc_type, dg_mesh **out);
 /* O-grid around NACA0012 (naca0012.geo:14 profile); wall on the profile,
  * characteristic far field on the outer circle. */
 int dg_mesh_ogrid(int ncirc, int nrad, double stretch, double rfar, dg_mesh **out);

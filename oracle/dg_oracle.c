@@ -283,6 +283,23 @@ void or_get_bedge_q(const or_consts *c, const int *bedge_type, const int *bedgeN
   }
 }
 
+/* vortex/src/get_bedge_q.h:1-40 (the vortex formula as or_vortex_state's) */
+void or_get_bedge_q_exact(const or_consts *c, const int *bedgeNum, const double *x,
+                          const double *y, double t, const double *exact, double *exteriorQ) {
+  const int Nfp = c->Nfp;
+  int exInd = *bedgeNum * 4 * Nfp;
+  const int *fmask = c->FMASK + *bedgeNum * Nfp;
+  for (int i = 0; i < Nfp; i++) {
+    double qb[4];
+    or_vortex_state(c->gam, exact[0], exact[1], exact[2], exact[3], t, x[fmask[i]], y[fmask[i]],
+                    exact[4], qb);
+    exteriorQ[exInd + i * 4] += qb[0];
+    exteriorQ[exInd + i * 4 + 1] += qb[1];
+    exteriorQ[exInd + i * 4 + 2] += qb[2];
+    exteriorQ[exInd + i * 4 + 3] += qb[3];
+  }
+}
+
 /* src/set_workingQ.h:1-12 */
 void or_set_workingQ(const or_consts *c, const double *dt, const int *stage,
                      const double *q, const double *k1, const double *k2,
@@ -457,8 +474,9 @@ static void edge_body(const or_consts *c, const or_mesh *m, or_state *st, int e)
 }
 
 void or_ssprk3_step_timed(const or_consts *c, const or_mesh *m, or_state *st,
-                          double dt, int nthreads, const int *cptr, const int *clist,
+                          double t, double dt, int nthreads, const int *cptr, const int *clist,
                           int ncol, double *kt) {
+  const double frac[3] = {0.0, 1.0, 0.5};
   const int Np = c->Np, nf = 3 * c->Nfp;
   const int K = m->ncells;
   double *rk[3] = {st->rk1, st->rk2, st->rk3};
@@ -477,8 +495,12 @@ void or_ssprk3_step_timed(const or_consts *c, const or_mesh *m, or_state *st,
     double t1 = wtime();
     for (int b = 0; b < m->nbedges; b++) {
       int k = m->bedge2cells[b];
-      or_get_bedge_q(c, m->bedge_type + b, m->bedgeNum + b, m->nx + nf * k, m->ny + nf * k,
-                     st->workingQ + 4 * Np * k, st->exteriorQ + 4 * nf * k);
+      if (m->bedge_type[b] == 4)
+        or_get_bedge_q_exact(c, m->bedgeNum + b, m->x + Np * k, m->y + Np * k, t + frac[j] * dt,
+                             m->exact, st->exteriorQ + 4 * nf * k);
+      else
+        or_get_bedge_q(c, m->bedge_type + b, m->bedgeNum + b, m->nx + nf * k, m->ny + nf * k,
+                       st->workingQ + 4 * Np * k, st->exteriorQ + 4 * nf * k);
     }
     double t2 = wtime();
     double *out = rk[j];
@@ -507,7 +529,7 @@ void or_ssprk3_step_timed(const or_consts *c, const or_mesh *m, or_state *st,
 
 void or_ssprk3_step(const or_consts *c, const or_mesh *m, or_state *st, double dt,
                     int nthreads, const int *cptr, const int *clist, int ncol) {
-  or_ssprk3_step_timed(c, m, st, dt, nthreads, cptr, clist, ncol, 0);
+  or_ssprk3_step_timed(c, m, st, 0.0, dt, nthreads, cptr, clist, ncol, 0);
 }
 
 int or_colour_edges(int ncells, int nedges, const int *edge2cells, int *colour_ptr,
@@ -552,7 +574,7 @@ double or_run(const or_consts *c, const or_mesh *m, or_state *st, int nsteps,
   double dt = cfl / dt1;
   double t = 0.0;
   for (int i = 0; i < nsteps; i++) {
-    or_ssprk3_step_timed(c, m, st, dt, nthreads, cptr, clist, ncol, ktime);
+    or_ssprk3_step_timed(c, m, st, t, dt, nthreads, cptr, clist, ncol, ktime);
     t += dt;
     t6 = wtime();
     dt1 = or_calc_dt_all(c, m, st->q, 0.0, nthreads);
@@ -562,6 +584,24 @@ double or_run(const or_consts *c, const or_mesh *m, or_state *st, int nsteps,
   free(cptr);
   free(clist);
   if (dt_out) *dt_out = dt;
+  return t;
+}
+
+double or_run_dt(const or_consts *c, const or_mesh *m, or_state *st, int nsteps, double dt,
+                 int nthreads) {
+  int *cptr = 0, *clist = 0, ncol = 0;
+  if (nthreads > 1) {
+    cptr = (int *)malloc(9 * sizeof(int));
+    clist = (int *)malloc(((size_t)m->nedges + 1) * sizeof(int));
+    ncol = or_colour_edges(m->ncells, m->nedges, m->edge2cells, cptr, clist);
+  }
+  double t = 0.0;
+  for (int i = 0; i < nsteps; i++) {
+    or_ssprk3_step_timed(c, m, st, t, dt, nthreads, cptr, clist, ncol, 0);
+    t += dt;
+  }
+  free(cptr);
+  free(clist);
   return t;
 }
 

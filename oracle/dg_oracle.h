@@ -45,6 +45,11 @@ void or_get_neighbour_q(const or_consts *c, const int *edgeNum, const double *xL
 void or_get_bedge_q(const or_consts *c, const int *bedge_type, const int *bedgeNum,
                     const double *nx, const double *ny, const double *q,
                     double *exteriorQ);
+/* vortex/src/get_bedge_q.h:1-40 for one boundary edge: exteriorQ += the exact
+ * isentropic-vortex state (or_vortex_state, exact = {uinf, beta, x0, y0,
+ * period_x}) at the face nodes' coordinates x, y [Np] and time t. */
+void or_get_bedge_q_exact(const or_consts *c, const int *bedgeNum, const double *x,
+                          const double *y, double t, const double *exact, double *exteriorQ);
 /* src/set_workingQ.h:1-12 */
 void or_set_workingQ(const or_consts *c, const double *dt, const int *stage,
                      const double *q, const double *k1, const double *k2,
@@ -77,6 +82,11 @@ typedef struct {
   const double *nodeX, *nodeY;                 /* [K][3]      */
   const double *rx, *ry, *sx, *sy;             /* [K][Np]     */
   const double *nx, *ny, *fscale;              /* [K][3*Nfp]  */
+  /* boundary type 4 (exact isentropic-vortex state): node coordinates
+   * [K][Np] and the vortex {uinf, beta, x0, y0, period_x}; may be NULL when
+   * no boundary edge has type 4 */
+  const double *x, *y;
+  const double *exact;
 } or_mesh;
 
 typedef struct {
@@ -94,8 +104,10 @@ void or_init_grid_all(const or_consts *c, int ncells, const double *cell_xy,
 double or_calc_dt_all(const or_consts *c, const or_mesh *m, const double *q,
                       double dt1_in, int nthreads);
 
-/* One SSPRK3 stage-sequence step, as src/airfoil.cpp:253-317.  dt is the
- * step size for this step.  nthreads<=1 runs the seq loop order; >1 runs the
+/* One SSPRK3 stage-sequence step, as src/airfoil.cpp:253-317, starting at
+ * time 0 (or_run passes the running time to the exact-state boundaries,
+ * which evaluate at t + {0, 1, 1/2} dt, vortex/src/vortex.cpp:228,248-258).
+ * dt is the step size for this step.  nthreads<=1 runs the seq loop order; >1 runs the
  * openmp structure (contiguous chunks for direct loops, coloured edge loops,
  * per-thread partials for the max). */
 void or_ssprk3_step(const or_consts *c, const or_mesh *m, or_state *st, double dt,
@@ -109,6 +121,11 @@ void or_ssprk3_step(const or_consts *c, const or_mesh *m, or_state *st, double d
  * update_Q, calc_dt} when non-NULL. */
 double or_run(const or_consts *c, const or_mesh *m, or_state *st, int nsteps,
               double cfl, int nthreads, double *dt_out, double *ktime);
+
+/* nsteps steps of the fixed size dt from time 0 (the step sequence of or_run
+ * without the CFL recomputation); returns the final time. */
+double or_run_dt(const or_consts *c, const or_mesh *m, or_state *st, int nsteps, double dt,
+                 int nthreads);
 
 /* Greedy colouring of edges so that no two edges of one colour share a cell
  * (the race-freedom contract of OP2's plan colouring).  colour_ptr has

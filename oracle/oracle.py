@@ -47,7 +47,8 @@ class MeshS(ctypes.Structure):
     _fields_ = [("ncells", I), ("nedges", I), ("nbedges", I),
                 ("edge2cells", VP), ("edgeNum", VP), ("bedge2cells", VP), ("bedgeNum", VP),
                 ("bedge_type", VP), ("nodeX", VP), ("nodeY", VP), ("rx", VP), ("ry", VP),
-                ("sx", VP), ("sy", VP), ("nx", VP), ("ny", VP), ("fscale", VP)]
+                ("sx", VP), ("sy", VP), ("nx", VP), ("ny", VP), ("fscale", VP),
+                ("x", VP), ("y", VP), ("exact", VP)]
 
 
 class StateS(ctypes.Structure):
@@ -86,6 +87,7 @@ class Oracle:
         self.phys = dict(freestream())
         if phys:
             self.phys.update(phys)
+        self.exact = None
         self._mk()
 
     def _mk(self):
@@ -235,9 +237,26 @@ class Oracle:
         m.ncells, m.nedges, m.nbedges = case["K"], case["edge2cells"].shape[0], case["bedge2cells"].size
         m.edge2cells, m.edgeNum = _p(case["edge2cells"]), _p(case["edgeNum"])
         m.bedge2cells, m.bedgeNum, m.bedge_type = _p(case["bedge2cells"]), _p(case["bedgeNum"]), _p(case["bedge_type"])
-        for k in ("nodeX", "nodeY", "rx", "ry", "sx", "sy", "nx", "ny", "fscale"):
+        for k in ("nodeX", "nodeY", "rx", "ry", "sx", "sy", "nx", "ny", "fscale", "x", "y"):
             setattr(m, k, _p(g[k]))
+        if np.any(case["bedge_type"] == 4):
+            if self.exact is None:
+                raise ValueError("boundary type 4 needs set_exact_state()")
+            m.exact = _p(self.exact)
         return m
+
+    def set_exact_state(self, uinf, beta=5.0, x0=5.0, y0=0.0, period_x=0.0):
+        """Vortex of the exact-state boundaries (type 4, vortex/src/get_bedge_q.h)."""
+        self.exact = np.array([uinf, beta, x0, y0, period_x], dtype=np.float64)
+
+    def get_bedge_q_exact(self, bedgeNum, x, y, t, exteriorQ):
+        """vortex/src/get_bedge_q.h:1-40 on one boundary edge; returns the new exteriorQ."""
+        ext = np.ascontiguousarray(exteriorQ, dtype=np.float64).copy()
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        yy = np.ascontiguousarray(y, dtype=np.float64)
+        lib.or_get_bedge_q_exact(ctypes.byref(self.c), ctypes.byref(I(int(bedgeNum))), _p(xx), _p(yy), D(t),
+                                 _p(self.exact), _p(ext))
+        return ext
 
     def new_state(self, case, q0):
         K, Np, NF = case["K"], self.Np, self.NF
@@ -259,14 +278,24 @@ class Oracle:
                        int(nthreads), ctypes.byref(dtn), _p(kt))
         return t, dtn.value
 
+    def run_dt(self, case, state, nsteps, dt, nthreads=1):
+        """nsteps SSPRK3 steps of fixed size dt from time 0; returns the final time."""
+        m = self._mesh_struct(case)
+        s = StateS(*[_p(state[k]) for k in ("q", "workingQ", "exteriorQ", "rk1", "rk2", "rk3")])
+        lib.or_run_dt.restype = D
+        return lib.or_run_dt(ctypes.byref(self.c), ctypes.byref(m), ctypes.byref(s), int(nsteps), D(dt),
+                             int(nthreads))
+
     def calc_dt_all(self, case, q, nthreads=1):
         m = self._mesh_struct(case)
         qq = np.ascontiguousarray(q, dtype=np.float64)
         lib.or_calc_dt_all.restype = D
         return lib.or_calc_dt_all(ctypes.byref(self.c), ctypes.byref(m), _p(qq), D(0.0), int(nthreads))
 
-    def rhs_all(self, case, w):
-        """One pass of get_neighbour_q + get_bedge_q + euler_rhs over the mesh."""
+    def rhs_all(self, case, w, t=0.0):
+        """One pass of get_neighbour_q + get_bedge_q (and the exact-state loop
+        of vortex/src/get_bedge_q.h at time t for type-4 edges) + euler_rhs
+        over the mesh."""
         K, Np, NF = case["K"], self.Np, self.NF
         g = case["geo"]
         w = np.ascontiguousarray(w, dtype=np.float64).reshape(K, Np, 4)
@@ -278,6 +307,10 @@ class Oracle:
                 en[e], g["nodeX"][L], g["nodeY"][L], g["nodeX"][R], g["nodeY"][R], w[L], w[R], ext[L], ext[R])]
         for b in range(case["bedge2cells"].size):
             c = case["bedge2cells"][b]
+            if case["bedge_type"][b] == 4:
+                ext[c] = self.get_bedge_q_exact(case["bedgeNum"][b], g["x"][c], g["y"][c], t,
+                                                ext[c]).reshape(NF, 4)
+                continue
             ext[c] = self.get_bedge_q(case["bedge_type"][b], case["bedgeNum"][b], g["nx"][c], g["ny"][c],
                                       w[c], ext[c]).reshape(NF, 4)
         rhs = np.zeros((K, Np, 4))

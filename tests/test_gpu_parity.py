@@ -30,6 +30,21 @@ def oracle_threads():
     return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n) or n)))
 
 
+def mass_l2(J, M, d):
+    """sqrt(sum_k J_k d_k^T M d_k) per variable of d [K][Np][4]."""
+    d = np.asarray(d).reshape(len(J), M.shape[0], 4)
+    return np.sqrt(np.einsum("k,kiv,ij,kjv->v", J, d, M, d))
+
+
+def assert_l2_errors_agree(J, M, q_gpu, q_orc, e_gpu, e_orc):
+    """The two L2 errors against the analytic state differ by at most the L2
+    norm of the two states' difference (triangle inequality in the same
+    mass-weighted norm), up to the rounding of the sums."""
+    bound = mass_l2(J, M, np.asarray(q_gpu) - np.asarray(q_orc))
+    diff = np.abs(np.asarray(e_gpu) - np.asarray(e_orc))
+    assert np.all(diff <= bound * (1 + 1e-9) + 1e-14 * np.asarray(e_orc)), (diff, bound, e_gpu, e_orc)
+
+
 def rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -237,10 +252,10 @@ def test_fused_run_is_deterministic_and_resumable(dg, vortex64):
 
 
 def test_vortex_error_matches_oracle_and_converges(dg, orc):
-    """L2 error against the analytic vortex, GPU vs oracle, at two mesh sizes."""
-    import os
+    """L2 error against the analytic vortex, GPU vs oracle, at 16x16, 32x32
+    and the 64x64 mesh of the CPU-sized benchmark case."""
     errs = []
-    for n in (16, 32):
+    for n in (16, 32, 64):
         mesh, ctx, s = _vortex_solver(dg, n)
         q0 = s.get_q()
         steps = n  # same final time scale: dt ~ h
@@ -255,12 +270,16 @@ def test_vortex_error_matches_oracle_and_converges(dg, orc):
         area = 0.5 * ((g["nodeX"][:, 1] - g["nodeX"][:, 0]) * (g["nodeY"][:, 2] - g["nodeY"][:, 0])
                       - (g["nodeX"][:, 2] - g["nodeX"][:, 0]) * (g["nodeY"][:, 1] - g["nodeY"][:, 0]))
         M = dg.generate_operators(3)["mass"]
-        e_gpu = o.vortex_l2(g["x"], g["y"], area / 2, M, s.get_q(), UINF, 5.0, 5.0, 0.0, t, 10.0)
+        qg = s.get_q()
+        assert abs(t - to) <= 1e-12 * to
+        e_gpu = o.vortex_l2(g["x"], g["y"], area / 2, M, qg, UINF, 5.0, 5.0, 0.0, t, 10.0)
         e_orc = o.vortex_l2(g["x"], g["y"], area / 2, M, st["q"], UINF, 5.0, 5.0, 0.0, to, 10.0)
-        assert np.allclose(e_gpu, e_orc, rtol=1e-9), (e_gpu, e_orc)
+        print(f"vortex {n}x{n}, {steps} steps: L2 error vs analytic GPU {e_gpu}, oracle {e_orc}", flush=True)
+        assert_l2_errors_agree(area / 2, M, qg, st["q"], e_gpu, e_orc)
         errs.append(e_gpu)
     ratio = np.array(errs[0]) / np.array(errs[1])
     assert np.all(ratio > 8.0), ratio  # observed order > 3 at N=3
+    assert np.all(np.array(errs[2]) < np.array(errs[1])), errs
 
 
 def test_full_size_vortex_matches_oracle_and_conserves(dg, orc):
@@ -289,7 +308,13 @@ def test_full_size_vortex_matches_oracle_and_conserves(dg, orc):
     g = case["geo"]
     X, Y = g["nodeX"], g["nodeY"]
     area = 0.5 * ((X[:, 1] - X[:, 0]) * (Y[:, 2] - Y[:, 0]) - (X[:, 2] - X[:, 0]) * (Y[:, 1] - Y[:, 0]))
-    w = dg.generate_operators(3)["mass"].sum(axis=1)
+    M = dg.generate_operators(3)["mass"]
+    # L2 error against the analytic vortex, GPU state vs oracle state
+    e_gpu = o.vortex_l2(g["x"], g["y"], area / 2, M, q, UINF, 5.0, 5.0, 0.0, t, 10.0)
+    e_orc = o.vortex_l2(g["x"], g["y"], area / 2, M, st["q"], UINF, 5.0, 5.0, 0.0, t, 10.0)
+    print(f"724x724 vortex L2 error vs analytic: GPU {e_gpu}, oracle {e_orc}", flush=True)
+    assert_l2_errors_agree(area / 2, M, q, st["q"], e_gpu, e_orc)
+    w = M.sum(axis=1)
 
     def totals(x):
         return np.einsum("k,i,kiv->v", area / 2, w, np.asarray(x).reshape(-1, 10, 4))
@@ -455,3 +480,66 @@ def test_vertex_export_matches_oracle(dg, orc, tmp_path):
     k = tok.index("Density")
     dens = np.array(tok[k + 5:k + 5 + mesh.nnodes], dtype=float)
     assert np.array_equal(dens, rho)
+
+
+def _exact_box(dg, orc, n):
+    """Jittered box [0,10] x [-5,5] whose boundary edges all carry the exact
+    vortex state; the vortex moves at u_inf from (5, -4), one unit above the
+    bottom edge, so the boundary cuts through its flow."""
+    mesh = dg.rect_grid(n, n, 0.0, -5.0, 10.0, 10.0, seed=4, jitter=0.1)
+    ctx = dg.Context(3)
+    ctx.set_freestream(GAM, 0.5, 0.0, 1.0, 1.0)
+    o = orc.Oracle(dg.generate_operators(3), orc.freestream(GAM, 0.5, 0.0))
+    o.set_exact_state(UINF, 5.0, 5.0, -4.0, 0.0)
+    case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
+                       mesh.bedgeNum, mesh.bedge_type)
+    return mesh, ctx, o, case
+
+
+def test_exact_boundary_loop_matches_oracle(dg, orc, dev):
+    """dg_get_bedge_q_exact (vortex/src/get_bedge_q.h over every boundary
+    edge, OP_INC) against the oracle's loop at a stage time."""
+    mesh, ctx, o, case = _exact_box(dg, orc, 12)
+    ctx.decl_mesh(mesh)
+    ctx.set_exact_state(UINF, 5.0, 5.0, -4.0, 0.0)
+    K, NF = mesh.ncells, ctx.NF
+    g = case["geo"]
+    rng = np.random.default_rng(5)
+    ext0 = rng.normal(size=(K, NF, 4))
+    ext = T(ext0, dev)
+    ctx.get_bedge_q_exact(T(g["x"], dev), T(g["y"], dev), 0.37, ext)
+    ref = ext0.copy()
+    for b in range(mesh.nbedges):
+        c = case["bedge2cells"][b]
+        ref[c] = o.get_bedge_q_exact(case["bedgeNum"][b], g["x"][c], g["y"][c], 0.37, ref[c]).reshape(NF, 4)
+    got = ext.cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_exact_boundary_vortex_run_matches_oracle(dg, orc, mode):
+    """The vortex case with exact-state boundaries (vortex/src/vortex.cpp:
+    228-317): 100 SSPRK3 steps at CFL 0.5 in the fused solver (hipGraph
+    replay, exact states written per stage at t, t + dt, t + dt/2) and in the
+    nine-loop sequence, against the oracle: <= 1e-12 relative L2 per
+    variable, and the same L2 error against the analytic vortex."""
+    mesh, ctx, o, case = _exact_box(dg, orc, 32)
+    s = dg.Solver(ctx, mesh)
+    s.set_vortex(UINF, 5.0, 5.0, -4.0, 0.0)
+    q0 = s.get_q()
+    s.run(100, cfl=0.5, mode=mode, graph=(mode == 0))
+    assert s.check() == 0
+    q = s.get_q()
+    st = o.new_state(case, q0)
+    t, _ = o.run(case, st, 100, cfl=0.5, nthreads=oracle_threads())
+    errs = rel_per_var(q, st["q"])
+    print(f"exact-boundary vortex 32x32, mode {mode}, 100 steps: rel L2 per variable {errs}", flush=True)
+    assert max(errs) <= 1e-12, errs
+    assert abs(s.time()[0] - t) <= 1e-12 * t
+    g = case["geo"]
+    X, Y = g["nodeX"], g["nodeY"]
+    area = 0.5 * ((X[:, 1] - X[:, 0]) * (Y[:, 2] - Y[:, 0]) - (X[:, 2] - X[:, 0]) * (Y[:, 1] - Y[:, 0]))
+    M = dg.generate_operators(3)["mass"]
+    e_gpu = o.vortex_l2(g["x"], g["y"], area / 2, M, q, UINF, 5.0, 5.0, -4.0, t)
+    e_orc = o.vortex_l2(g["x"], g["y"], area / 2, M, st["q"], UINF, 5.0, 5.0, -4.0, t)
+    assert_l2_errors_agree(area / 2, M, q, st["q"], e_gpu, e_orc)

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -433,4 +434,61 @@ void op_par_loop_update_Q(char const *name, op_set cells, op_arg a0, op_arg a1, 
                     dptr(a4, cells, 4 * np, name, 4), dptr(a5, cells, 4 * np, name, 5)),
         name);
   sync_and_time(name, t0);
+}
+
+// src/airfoil.cpp's op_decl_const(dim, type, &var): the name is the variable.
+void op_decl_const_expr(int dim, char const *type, int type_size, char *data, char const *expr) {
+  std::string e;
+  for (const char *p = expr ? expr : ""; *p; p++)
+    if (!std::isspace((unsigned char)*p)) e += *p;
+  while (!e.empty() && (e.front() == '&' || e.front() == '(')) e.erase(0, 1);
+  while (!e.empty() && e.back() == ')') e.pop_back();
+  bool ok = !e.empty() && (std::isalpha((unsigned char)e[0]) || e[0] == '_');
+  for (char ch : e) ok = ok && (std::isalnum((unsigned char)ch) || ch == '_');
+  if (!ok)
+    die(std::string("op_decl_const: cannot take a constant name from '") + (expr ? expr : "") +
+        "'; use op_decl_const2(name, dim, type, ptr)");
+  op_decl_const_char(dim, type, type_size, data, e.c_str());
+}
+
+// src/airfoil.cpp's op_par_loop(kernel, "name", set, args...): the loop by name.
+void op_par_loop_by_name(char const *name, op_set set, std::initializer_list<op_arg> args) {
+  const std::string n = name ? name : "";
+  const op_arg *a = args.begin();
+  const size_t na = args.size();
+  auto need = [&](size_t k) {
+    if (na != k)
+      die("op_par_loop(" + n + "): " + std::to_string(na) + " arguments, the loop takes " + std::to_string(k));
+  };
+  if (n == "init_grid") {
+    need(18);
+    op_par_loop_init_grid(name, set, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11],
+                          a[12], a[13], a[14], a[15], a[16], a[17]);
+  } else if (n == "set_ic") {
+    need(2);
+    op_par_loop_set_ic(name, set, a[0], a[1]);
+  } else if (n == "neighbour_zero") {
+    need(1);
+    op_par_loop_neighbour_zero(name, set, a[0]);
+  } else if (n == "calc_dt") {
+    need(3);
+    op_par_loop_calc_dt(name, set, a[0], a[1], a[2]);
+  } else if (n == "get_neighbour_q") {
+    need(9);
+    op_par_loop_get_neighbour_q(name, set, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8]);
+  } else if (n == "get_bedge_q") {
+    need(6);
+    op_par_loop_get_bedge_q(name, set, a[0], a[1], a[2], a[3], a[4], a[5]);
+  } else if (n == "euler_rhs") {
+    need(10);
+    op_par_loop_euler_rhs(name, set, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9]);
+  } else if (n == "set_workingQ") {
+    need(6);
+    op_par_loop_set_workingQ(name, set, a[0], a[1], a[2], a[3], a[4], a[5]);
+  } else if (n == "update_Q") {
+    need(6);
+    op_par_loop_update_Q(name, set, a[0], a[1], a[2], a[3], a[4], a[5]);
+  } else {
+    die("op_par_loop: no loop named '" + n + "' in this interface");
+  }
 }

@@ -1,12 +1,19 @@
 /* OP2-compatible host interface for the DG Euler loops of the airfoil
  * driver, on top of the dg_euler.h C ABI (C++ only).
  *
- * Covers the subset of OP2 that src/airfoil.cpp uses (op_init, op_decl_set,
- * op_decl_map, op_decl_dat, op_decl_const2, op_arg_dat, op_arg_gbl, the nine
- * op_par_loop_* host stubs declared in src/airfoil_op.cpp:17-98, op_timers,
- * op_get_size, op_fetch_data, op_timing_output, op_exit), so that a driver
- * written against the OP2 code generator's output compiles unchanged
- * against this header and runs every loop on the MI355X kernels.
+ * Covers the subset of OP2 that the airfoil driver uses, in both of its
+ * forms:
+ *   - the source form of src/airfoil.cpp (op_seq.h style):
+ *     op_decl_const(dim, type, &var) and op_par_loop(kernel, "name", set,
+ *     args...) -- the constant's name is taken from the pointer expression
+ *     and the loop from its name string, as OP2's translator does; the
+ *     kernel identifier is not evaluated, so the seq kernel headers
+ *     (src/euler_rhs.h etc., which need CBLAS) are not needed;
+ *   - the translated form of src/airfoil_op.cpp: op_decl_const2 and the
+ *     nine op_par_loop_* host stubs of src/airfoil_op.cpp:17-98;
+ * plus op_init, op_decl_set, op_decl_map, op_decl_dat, op_arg_dat,
+ * op_arg_gbl, op_timers, op_get_size, op_fetch_data, op_timing_output and
+ * op_exit.  Every loop runs on the MI355X kernels.
  *
  * Semantics follow OP2's device backends: op_decl_dat copies the user's
  * host array to a device mirror; par_loops run on the mirrors and return
@@ -28,6 +35,8 @@
 #ifndef __cplusplus
 #error "op2_dg.h is a C++ interface"
 #endif
+
+#include <initializer_list>
 
 struct op_set_core;
 struct op_map_core;
@@ -97,5 +106,20 @@ void op_par_loop_euler_rhs(char const *, op_set, op_arg, op_arg, op_arg, op_arg,
                            op_arg, op_arg, op_arg, op_arg);
 void op_par_loop_set_workingQ(char const *, op_set, op_arg, op_arg, op_arg, op_arg, op_arg, op_arg);
 void op_par_loop_update_Q(char const *, op_set, op_arg, op_arg, op_arg, op_arg, op_arg, op_arg);
+
+/* The source form of src/airfoil.cpp:183-334.  op_decl_const(dim, type,
+ * ptr): the constant's name is the pointer expression without its '&'
+ * ("&gam" -> gam, "Drw" -> Drw), as OP2's translator turns it into
+ * op_decl_const2("gam", ...); an expression that is not a plain name throws.
+ * op_par_loop(kernel, "name", set, args...): the loop named "name" (one of
+ * the nine above, with the same arguments). */
+void op_decl_const_expr(int dim, char const *type, int type_size, char *data, char const *expr);
+template <class T>
+void op_decl_const_named(int dim, char const *type, T *data, char const *expr) {
+  op_decl_const_expr(dim, type, (int)sizeof(T), (char *)data, expr);
+}
+void op_par_loop_by_name(char const *name, op_set set, std::initializer_list<op_arg> args);
+#define op_decl_const(dim, type, ptr) op_decl_const_named((dim), (type), (ptr), #ptr)
+#define op_par_loop(kernel, name, set, ...) op_par_loop_by_name((name), (set), {__VA_ARGS__})
 
 #endif /* OP2_DG_H */

@@ -1,6 +1,9 @@
-"""The OP2-compatible host interface (include/op2_dg.h): an src/airfoil.cpp-
-style driver built against it runs the reference's loop sequence on the GPU
-and reproduces the oracle's restatement of that sequence.
+"""The OP2-compatible host interface (include/op2_dg.h): the airfoil driver
+built against it, in the source form of src/airfoil.cpp (op_decl_const,
+op_par_loop(kernel, "name", ...)) and in the translated form of
+src/airfoil_op.cpp (op_decl_const2, op_par_loop_<kernel>), runs the
+reference's loop sequence on the GPU and reproduces the oracle's restatement
+of that sequence.
 
 Bar: relative L2 per conserved variable <= 1e-12 after 10 steps at the
 reference's order (N=4, its literal table size), dt = 1/dt1 every step.
@@ -14,7 +17,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DRIVER = os.path.join(ROOT, "tests", "op2", "airfoil_op2_driver")
+DRIVERS = {"translated": os.path.join(ROOT, "tests", "op2", "airfoil_op2_driver"),
+           "source": os.path.join(ROOT, "tests", "op2", "airfoil_seq_form_driver")}
 
 
 def rel_per_var(a, b):
@@ -23,12 +27,14 @@ def rel_per_var(a, b):
     return [np.linalg.norm(a[:, v] - b[:, v]) / np.linalg.norm(b[:, v]) for v in range(4)]
 
 
-def test_op2_driver_matches_oracle(dg, orc, tmp_path):
+@pytest.mark.parametrize("form", ["source", "translated"])
+def test_op2_driver_matches_oracle(dg, orc, tmp_path, form):
+    DRIVER = DRIVERS[form]
     if not os.path.exists(DRIVER):
-        pytest.fail("tests/op2/airfoil_op2_driver is not built (make -C airfoil-op2-fem_amd)")
+        pytest.fail(f"{DRIVER} is not built (make -C airfoil-op2-fem_amd)")
     ncirc, nrad, order, iters, alpha = 24, 6, 4, 10, 3.0
     out = tmp_path / "q.bin"
-    print(f"op2 driver: ogrid {ncirc}x{nrad} N={order} {iters} steps", flush=True)
+    print(f"op2 driver ({form} form): ogrid {ncirc}x{nrad} N={order} {iters} steps", flush=True)
     r = subprocess.run([DRIVER, "ogrid", str(ncirc), str(nrad), str(order), str(iters), str(alpha), str(out)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout

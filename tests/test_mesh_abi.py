@@ -145,9 +145,10 @@ def test_op2_interface_exports_every_declared_function(dg):
     hdr = open(os.path.join(ROOT, "include", "op2_dg.h")).read()
     names = set(re.findall(r"^(?:void|op_set|op_map|op_dat|op_arg|int)\s+(op_\w+)\s*\(", hdr, re.M))
     templates = set(re.findall(r"template <class T>\s*\n\s*\w+\s+(op_\w+)\s*\(", hdr))
-    assert templates == {"op_decl_dat", "op_decl_const2", "op_arg_gbl", "op_fetch_data"}
+    assert templates == {"op_decl_dat", "op_decl_const2", "op_arg_gbl", "op_fetch_data", "op_decl_const_named"}
     names = sorted(names - templates)
     assert "op_par_loop_euler_rhs" in names and len(names) >= 20
+    assert {"op_decl_const_expr", "op_par_loop_by_name"} <= set(names)
     nm = shutil.which("nm")
     if nm is None:
         pytest.skip("nm not available")
@@ -155,7 +156,20 @@ def test_op2_interface_exports_every_declared_function(dg):
     defined = set(re.findall(r"\s(op_\w+)\(", out))
     missing = [n for n in names if n not in defined]
     assert not missing, missing
-    assert os.path.exists(os.path.join(ROOT, "tests", "op2", "airfoil_op2_driver"))
+    for exe in ("airfoil_op2_driver", "airfoil_seq_form_driver"):
+        assert os.path.exists(os.path.join(ROOT, "tests", "op2", exe)), exe
+
+
+def test_op2_source_form_errors_on_the_host(dg):
+    """src/airfoil.cpp's source form through include/op2_dg.h: op_decl_const
+    takes the constant's name from its pointer expression and refuses
+    anything but a plain name; op_par_loop refuses an unknown loop name and a
+    wrong argument count before any loop runs (no device needed)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "op2", "op2_source_form_errors")
+    assert os.path.exists(exe), "make -C airfoil-op2-fem_amd builds it"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "source form checks ok" in r.stdout, r.stdout + r.stderr
 
 
 def test_bad_order_rejected_without_device(dg):

@@ -51,20 +51,21 @@ struct AdjArgs {
   const int *b2c, *bnum;
   // Face-pair form (compact only; null pairs: the per-slot form).  pairs
   // [npairs][2] lists each interior face once as (c * 3 + f) of its two
-  // sides, the first the lower cell.  psi [K][NF][4] holds -LIFT^T mu, the
-  // cotangent of the scaled fluxes: read by the face-pair pass, rewritten
-  // by the cell pass from its out for the next launch (whose mu is out).
+  // sides, the first the lower cell; cmet [K][4] holds (rx, ry, sx, sy) of
+  // each cell (its node-0 factors: a straight-sided cell has the same
+  // factors at every node).  The face-pair pass forms psi = -LIFT^T mu, the
+  // cotangent of the scaled fluxes, from mu itself; the cell pass runs on
+  // the matrix cores with the per-cell factors.  Needs x0 (and y0 with
+  // out2).
   const int *pairs;
   int npairs;
-  double *psi;
+  const double *cmet;
 };
 
 // Ld = L'(w) wd + dL/dgeo geod + dL/dbc (bcud, bcvd)
 hipError_t launch_rhs_tlm(const Launch &L, const TlmArgs &a, double bcud, double bcvd);
 // node_slots: [Np][2] face slots m with FMASK[m] == node (-1 when none)
 hipError_t launch_rhs_adj(const Launch &L, const AdjArgs &a, const int *node_slots);
-// a.psi = -LIFT^T a.mu (the face-pair form's psi of a first launch)
-hipError_t launch_adj_psi(const Launch &L, const AdjArgs &a);
 // z = a x + b y  (y may be null)
 hipError_t launch_lincomb(const Launch &L, size_t n, double a, const double *x, double b,
                           const double *y, double *z);
@@ -88,11 +89,12 @@ struct GeoCot {
 // factors are those of init_grid_cell_rel_t (as launch_init_grid_rel writes).
 hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_rel, const GeoCot &g,
                           double *gv);
-// rx, ry, sx, sy [K][Np], nx, ny, fs [K][NF] and J [K] (node 0) of the cells
-// cell_rel, in the translation-invariant form init_grid_cell_rel_t.
+// rx, ry, sx, sy [K][Np], nx, ny, fs [K][NF], J [K] (node 0) and, when not
+// null, cmet [K][4] (rx, ry, sx, sy of node 0) of the cells cell_rel, in the
+// translation-invariant form init_grid_cell_rel_t.
 hipError_t launch_init_grid_rel(const Launch &L, int ncells, const double *cell_rel, double *rx,
                                 double *ry, double *sx, double *sy, double *nx, double *ny,
-                                double *fs, double *J);
+                                double *fs, double *J, double *cmet);
 // partial[0..kSumBlocks) = block sums of x[i * stride + offset], i < n, in a
 // fixed order; the caller adds the partials in index order.
 constexpr int kSumBlocks = 256;

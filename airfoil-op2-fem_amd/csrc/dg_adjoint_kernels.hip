@@ -7,20 +7,26 @@
 //           functions evaluated in dual arithmetic.
 // Adjoint:  out = e0 x0 + scale * L'(w)^T mu (plus an optional second
 //           combination), geometry and bc accumulators += scale *
-//           (dL/dgeo)^T mu, (dL/dbc)^T mu.  Gather-only in two passes: the
-//           face pass evaluates the reverse Roe flux once per face slot and
-//           stores the interior-side (gmq) and exterior-side (gpq)
-//           cotangents; the cell pass applies the transposed weak
-//           derivatives, the reverse Euler fluxes, and gathers its own gmq and
-//           the neighbours' gpq of the slots that read its nodes.  Both passes
-//           give each workgroup whole cells, staging the cotangent mu, the
-//           metrics and the nodal fluxes of its cells in LDS.
+//           (dL/dgeo)^T mu, (dL/dbc)^T mu.  Gather-only in two forms.
+//           Per-slot form (the C-ABI entry dg_rhs_adjoint): the face pass
+//           evaluates the reverse Roe flux once per face slot and stores the
+//           interior-side (gmq) and exterior-side (gpq) cotangents, the
+//           boundary pass reverses the boundary states through their dual
+//           Jacobian, and the cell pass applies the transposed weak
+//           derivatives and the reverse Euler fluxes and gathers its own gmq
+//           and the neighbours' gpq.  Face-pair form (the gradient's reverse
+//           sweep): one reverse Roe flux per interior face node pair (and per
+//           boundary slot, with the hand-written reverse of the boundary
+//           state), then the cell pass on the matrix cores with per-cell
+//           factors (adj_cell_mfma_kernel).
 #include <hip/hip_runtime.h>
 
 #include "dg_adjoint.h"
+#include "dg_bc.h"
 #include "dg_device.h"
 #include "dg_dual.h"
 #include "dg_geom.h"
+#include "dg_mfma.h"
 
 namespace dg {
 
@@ -71,55 +77,6 @@ __device__ __forceinline__ void roe_t(double gam, const T mQ[4], const T eQ[4], 
   out[1] = (nx * f1 - ny * f2) * fs;
   out[2] = (ny * f1 + nx * f2) * fs;
   out[3] = f3 * fs;
-}
-
-// Boundary states of types 0-3 (src/get_bedge_q.h + characteristic far
-// field) with the free-stream velocity as an input.
-template <class T>
-__device__ __forceinline__ void boundary_t(const Phys &ph, int type, T bcu, T bcv, const T mQ[4],
-                                           T nx, T ny, T qb[4]) {
-  const double g = ph.gam;
-  T bce = ph.bc_p / (g - 1.0) + 0.5 * ph.bc_r * (bcu * bcu + bcv * bcv);
-  if (type == 0) {
-    qb[0] = T(ph.bc_r); qb[1] = ph.bc_r * bcu; qb[2] = ph.bc_r * bcv; qb[3] = bce;
-  } else if (type == 1) {
-    qb[0] = T(ph.bc_r); qb[1] = ph.bc_r * bcu; qb[2] = ph.bc_r * bcv; qb[3] = mQ[3];
-  } else if (type == 2) {
-    T vn = nx * mQ[1] + ny * mQ[2];
-    qb[0] = mQ[0]; qb[1] = mQ[1] - 2 * vn * nx; qb[2] = mQ[2] - 2 * vn * ny; qb[3] = mQ[3];
-  } else {
-    T rhoi = mQ[0], ui = mQ[1] / rhoi, vi = mQ[2] / rhoi;
-    T pi_ = (g - 1.0) * (mQ[3] - 0.5 * rhoi * (ui * ui + vi * vi));
-    T ci = sqrt(fabs(g * pi_ / rhoi));
-    T vni = ui * nx + vi * ny;
-    const double cinf = ::sqrt(g * ph.bc_p / ph.bc_r);
-    T vninf = bcu * nx + bcv * ny;
-    if (val(fabs(vni)) >= val(ci)) {
-      if (val(vni) < 0.0) {
-        qb[0] = T(ph.bc_r); qb[1] = ph.bc_r * bcu; qb[2] = ph.bc_r * bcv; qb[3] = bce;
-      } else {
-        for (int k = 0; k < 4; k++) qb[k] = mQ[k];
-      }
-      return;
-    }
-    T rp = vni + 2.0 * ci / (g - 1.0);
-    T rm = vninf - 2.0 * cinf / (g - 1.0);
-    T vnb = 0.5 * (rp + rm);
-    T cb = 0.25 * (g - 1.0) * (rp - rm);
-    T ent, ut, vt;
-    if (val(vnb) > 0.0) {
-      ent = pi_ / pow(rhoi, g);
-      ut = ui - vni * nx; vt = vi - vni * ny;
-    } else {
-      ent = T(ph.bc_p / ::pow(ph.bc_r, g));
-      ut = bcu - vninf * nx; vt = bcv - vninf * ny;
-    }
-    T rhob = pow(cb * cb / (g * ent), 1.0 / (g - 1.0));
-    T pb = rhob * cb * cb / g;
-    T ub = ut + vnb * nx, vb = vt + vnb * ny;
-    qb[0] = rhob; qb[1] = rhob * ub; qb[2] = rhob * vb;
-    qb[3] = pb / (g - 1.0) + 0.5 * rhob * (ub * ub + vb * vb);
-  }
 }
 
 // ------------------------------------------------------------------------
@@ -451,38 +408,6 @@ __global__ __launch_bounds__(256) void adj_boundary_kernel(AdjArgs a, Phys ph, c
   }
 }
 
-// psi = -LIFT^T mu per face slot, the cotangent of the scaled Roe fluxes
-// (one thread per slot, whole cells per workgroup, the cells' mu staged in
-// LDS): the face-pair form's psi for a first launch; the cell pass writes
-// it for the launches after.
-template <int N>
-__global__ __launch_bounds__(kAdjThreads) void adj_psi_kernel(AdjArgs a, const double *__restrict__ ops) {
-  constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
-  constexpr int CPB = kAdjThreads / NF;
-  __shared__ double sL[Np * NF];
-  __shared__ double sMu[4][CPB * Np];
-  for (int k = threadIdx.x; k < Np * NF; k += kAdjThreads) sL[k] = ops[2 * Np * Np + k];
-  const int t = threadIdx.x;
-  const int cl = t / NF, m = t - cl * NF;
-  for (size_t base = (size_t)blockIdx.x * CPB; base < (size_t)a.ncells; base += (size_t)gridDim.x * CPB) {
-    __syncthreads();
-    const int nc = (int)min((size_t)CPB, (size_t)a.ncells - base);
-    for (int k = t; k < nc * Np; k += kAdjThreads) {
-      const double4 mu = *reinterpret_cast<const double4 *>(a.mu + (base * Np + k) * 4);
-      sMu[0][k] = mu.x; sMu[1][k] = mu.y; sMu[2][k] = mu.z; sMu[3][k] = mu.w;
-    }
-    __syncthreads();
-    if (cl < nc) {
-      double p[4] = {0, 0, 0, 0};
-      for (int j = 0; j < Np; j++) {
-        const double l = sL[j * NF + m];
-        for (int v = 0; v < 4; v++) p[v] -= l * sMu[v][cl * Np + j];
-      }
-      *reinterpret_cast<double4 *>(a.psi + ((base + cl) * NF + m) * 4) = make_double4(p[0], p[1], p[2], p[3]);
-    }
-  }
-}
-
 // Face-pair form of the face and boundary passes.
 //
 // The Roe flux is antisymmetric, R(p, m, -n) = -R(m, p, n) (the flux of
@@ -502,62 +427,67 @@ __global__ __launch_bounds__(kAdjThreads) void adj_psi_kernel(AdjArgs a, const d
 // per-node rounding makes the two normals opposite only to ~1e-15 relative
 // times the cell's aspect, and moves the gradient by that much.
 //
-// Blocks [0, nbblk): the boundary slots, 8 threads each, one per direction
-// of the boundary state's dual Jacobian (the 4 interior variables, nx, ny,
-// bc_u, bc_v); each reverses the slot's Roe flux itself (a few thousand
-// slots) and its direction of the boundary state; per-slot outputs, each
-// written by one thread.  Blocks [nbblk, grid): one thread per interior
-// face node pair, Nfp consecutive threads per face, the per-face geometry
-// sums in node order.
+// -LIFT^T mu at slot m of cell c (LIFT [Np][NF] in LDS): the cotangent of
+// the cell's scaled flux there.
+template <int Np, int NF>
+__device__ __forceinline__ void psi_of_mu(const double *__restrict__ mu, size_t c, int m, const double *sL,
+                                          double p[4]) {
+  p[0] = p[1] = p[2] = p[3] = 0.0;
+#pragma unroll 5
+  for (int j = 0; j < Np; j++) {
+    const double4 u = *reinterpret_cast<const double4 *>(mu + (c * Np + j) * 4);
+    const double l = sL[j * NF + m];
+    p[0] -= l * u.x; p[1] -= l * u.y; p[2] -= l * u.z; p[3] -= l * u.w;
+  }
+}
+
+// Blocks [0, nbblk): the boundary slots, one thread each: the slot's
+// reverse Roe flux with the boundary state as exterior, whose cotangent the
+// hand-written reverse of the boundary state (boundary_vjp) carries to the
+// trace, the normal and the free-stream velocity; per-slot outputs.
+// Blocks [nbblk, grid): one thread per interior face node pair, Nfp
+// consecutive threads per face, the per-face geometry sums in node order.
+// psi, the cotangent of the scaled fluxes, is formed from mu here
+// (psi_of_mu); the cell pass before wrote mu as its out.
 template <int N>
-__global__ __launch_bounds__(kAdjThreads, 2) void adj_pair_kernel(AdjArgs a, Phys ph,
-                                                                const int *__restrict__ fmask, int nbblk) {
+__global__ __launch_bounds__(kAdjThreads, 3) void adj_pair_kernel(AdjArgs a, Phys ph,
+                                                                const int *__restrict__ fmask,
+                                                                const double *__restrict__ ops, int nbblk) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   constexpr int FPB = kAdjThreads / Nfp;  // faces per workgroup visit
   __shared__ double sG[4][kAdjThreads];
+  __shared__ double sL[Np * NF];
   __shared__ int sFm[NF];
   const int t = threadIdx.x;
   if (t < NF) sFm[t] = fmask[t];
+  for (int k = t; k < Np * NF; k += kAdjThreads) sL[k] = ops[2 * Np * Np + k];
   __syncthreads();
   if ((int)blockIdx.x < nbblk) {
-    const int n = a.nbedges * Nfp * 8;
-    for (int it = (int)blockIdx.x * kAdjThreads + t; it < n; it += nbblk * kAdjThreads) {
-      const int dir = it & 7, bi = it >> 3;
-      const int b = bi / Nfp, i = bi - b * Nfp;
-      const size_t c = a.b2c[b];
-      const int f = a.bnum[b], m = f * Nfp + i;
-      const int code = a.fmap[c * 3 + f];
-      if (code >= 0) continue;  // not a boundary face (never for a valid mesh)
-      const size_t slot = c * NF + m;
-      const double4 q4 = *reinterpret_cast<const double4 *>(a.w + (c * Np + sFm[m]) * 4);
-      const double4 p4 = *reinterpret_cast<const double4 *>(a.psi + slot * 4);
-      const double nx = a.nx[slot], ny = a.ny[slot], fs = a.fscale[slot];
-      double mQ[4] = {q4.x, q4.y, q4.z, q4.w}, pQ[4];
-      boundary_state(ph, -code - 1, mQ, nx, ny, pQ);  // the forward's exterior state
-      const double psi[4] = {p4.x, p4.y, p4.z, p4.w};
-      double bm[4] = {0, 0, 0, 0}, bp[4] = {0, 0, 0, 0}, bnx = 0, bny = 0, bfs = 0;
-      roe_vjp(ph.gam, mQ, pQ, nx, ny, fs, psi, bm, bp, bnx, bny, bfs);
-      Dual q[4], qb[4];
-      for (int v = 0; v < 4; v++) q[v] = Dual(mQ[v], dir == v ? 1.0 : 0.0);
-      const Dual dnx(nx, dir == 4 ? 1.0 : 0.0), dny(ny, dir == 5 ? 1.0 : 0.0);
-      const Dual du(ph.bc_u, dir == 6 ? 1.0 : 0.0), dv(ph.bc_v, dir == 7 ? 1.0 : 0.0);
-      boundary_t<Dual>(ph, -code - 1, du, dv, q, dnx, dny, qb);
-      double s = 0.0;
-      for (int v = 0; v < 4; v++) s += bp[v] * qb[v].d;
-      if (dir < 4) {
-        const double bmd = dir == 0 ? bm[0] : (dir == 1 ? bm[1] : (dir == 2 ? bm[2] : bm[3]));
-        a.gmq[slot * 4 + dir] = bmd + s;
-      } else if (dir == 4) {
-        a.g_nx[slot] += a.scale * (bnx + s);
-      } else if (dir == 5) {
-        a.g_ny[slot] += a.scale * (bny + s);
-      } else if (dir == 6) {
-        a.g_bcu[slot] += a.scale * s;
-        a.g_fs[slot] += a.scale * bfs;
-      } else {
-        a.g_bcv[slot] += a.scale * s;
-      }
-    }
+    const int bi = (int)blockIdx.x * kAdjThreads + t;
+    if (bi >= a.nbedges * Nfp) return;
+    const int b = bi / Nfp, i = bi - b * Nfp;
+    const size_t c = a.b2c[b];
+    const int f = a.bnum[b], m = f * Nfp + i;
+    const int code = a.fmap[c * 3 + f];
+    if (code >= 0) return;  // not a boundary face (never for a valid mesh)
+    const size_t slot = c * NF + m;
+    const double4 q4 = *reinterpret_cast<const double4 *>(a.w + (c * Np + sFm[m]) * 4);
+    double psi[4];
+    psi_of_mu<Np, NF>(a.mu, c, m, sL, psi);
+    const double nx = a.nx[slot], ny = a.ny[slot], fs = a.fscale[slot];
+    const double mQ[4] = {q4.x, q4.y, q4.z, q4.w};
+    double pQ[4];
+    boundary_state(ph, -code - 1, mQ, nx, ny, pQ);  // the forward's exterior state
+    double bm[4] = {0, 0, 0, 0}, bp[4] = {0, 0, 0, 0}, bnx = 0, bny = 0, bfs = 0;
+    roe_vjp(ph.gam, mQ, pQ, nx, ny, fs, psi, bm, bp, bnx, bny, bfs);
+    double bu = 0.0, bv = 0.0;
+    boundary_vjp(ph, -code - 1, ph.bc_u, ph.bc_v, mQ, nx, ny, bp, bm, bnx, bny, bu, bv);
+    *reinterpret_cast<double4 *>(a.gmq + slot * 4) = make_double4(bm[0], bm[1], bm[2], bm[3]);
+    a.g_nx[slot] += a.scale * bnx;
+    a.g_ny[slot] += a.scale * bny;
+    a.g_fs[slot] += a.scale * bfs;
+    a.g_bcu[slot] += a.scale * bu;
+    a.g_bcv[slot] += a.scale * bv;
     return;
   }
   const int fl = t / Nfp, i = t - fl * Nfp;
@@ -576,8 +506,14 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_pair_kernel(AdjArgs a, Phy
       const size_t so = (size_t)co * NF + mo, sp = (size_t)cp * NF + mp;
       const double4 q4 = *reinterpret_cast<const double4 *>(a.w + ((size_t)co * Np + sFm[mo]) * 4);
       const double4 e4 = *reinterpret_cast<const double4 *>(a.w + ((size_t)cp * Np + sFm[mp]) * 4);
-      const double4 po = *reinterpret_cast<const double4 *>(a.psi + so * 4);
-      const double4 pp = *reinterpret_cast<const double4 *>(a.psi + sp * 4);
+      double4 po, pp;
+      {
+        double p[4];
+        psi_of_mu<Np, NF>(a.mu, co, mo, sL, p);
+        po = make_double4(p[0], p[1], p[2], p[3]);
+        psi_of_mu<Np, NF>(a.mu, cp, mp, sL, p);
+        pp = make_double4(p[0], p[1], p[2], p[3]);
+      }
       const double nx = a.nx[so], ny = a.ny[so], fso = a.fscale[so], fsp = a.fscale[sp];
       const double mQ[4] = {q4.x, q4.y, q4.z, q4.w}, pQ[4] = {e4.x, e4.y, e4.z, e4.w};
       const double o[4] = {fso * po.x - fsp * pp.x, fso * po.y - fsp * pp.y, fso * po.z - fsp * pp.z,
@@ -607,22 +543,196 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_pair_kernel(AdjArgs a, Phy
   }
 }
 
+// Cell pass of the face-pair form on the matrix cores (per-cell metric
+// factors cmet, per-cell metric cotangents).  Each wave owns tiles of 16
+// consecutive cells, 4 groups of 4 cells, laid out for
+// v_mfma_f64_16x16x4f64 as in the stage kernels.  The tile's mu records and
+// its slot cotangents gmq (both sides of each face, written by the pair
+// passes) are copied into LDS by direct-to-LDS loads; the mu record layout
+// is already an A operand: row cell + 4 var of group g, column node k, at
+// ((4 g + cell) Np + k) 4 + var.  Per group, nu_r = Drw^T mu and
+// nu_s = Dsw^T mu on the matrix cores (lane: cell, node j, the 4 variables
+// in its accumulator registers); with the cell's factors phF = rx nu_r +
+// sx nu_s, phG = ry nu_r + sy nu_s, and lam = (dF/dw)^T phF + (dG/dw)^T phG
+// plus the slot cotangents of the node's slots; the metric cotangents
+// sum_i mu_i . (Drw F)_i = sum_j F_j . nu_r_j (etc.) per cell by shuffles.
+// scale lam replaces the group's mu records; the write-back forms out
+// (and out2) chunk by chunk, coalesced, and leaves out in the records, from
+// which psi = -LIFT^T out for the next launch comes on the matrix cores.
+// One batched memory round trip per tile: the LDS copies, the lane's
+// states, the epilogue's x0 / y0 and the running metric cotangents.
+// The per-cell factors are node 0's; the forward's per-node factors differ
+// from them by init_grid's rounding, which moves the gradient by that much.
+template <int N, bool OUT2>
+__global__ __launch_bounds__(64, 3) void adj_cell_mfma_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops,
+                                                             unsigned long long fmask4) {
+  constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
+  constexpr int KP = (Np + 3) / 4;  // K-steps over nodes
+  constexpr int TE = kMfmaTile, G = 4;
+  constexpr int RCH = TE * Np * 2, SCH = TE * NF * 2;  // 16-byte chunks of a tile's records
+  constexpr int RD = copy_to_lds_doubles<RCH>(), SD = copy_to_lds_doubles<SCH>();
+  constexpr int WB = (RCH + 63) / 64;  // write-back chunks per lane
+  static_assert(Np <= 16 && NF <= 16, "one 16-column output tile per cell group");
+  __shared__ double sMu[RD], sGq[SD], sMet[128], sCell[TE * 4];
+  auto fmask_of = [&](int m) -> int { return (int)((fmask4 >> (4 * m)) & 15ull); };
+  const int lane = threadIdx.x;
+  for (int i = lane; i < RD; i += 64) sMu[i] = 0.0;
+  for (int i = lane; i < SD; i += 64) sGq[i] = 0.0;
+  const double *Drw = ops, *Dsw = ops + Np * Np;
+  // B operands, lane l: B[k = 4 ks + l/16][j = l%16]; the lane index goes
+  // through lane_in_loop so that they are loaded where used
+  auto load_b = [&](double *b, const double *Mx, int ld, int ncol) {
+    const int l = lane_in_loop(lane), j = l & 15, k0 = l >> 4;
+#pragma unroll
+    for (int ks = 0; ks < KP; ks++) {
+      const int k = 4 * ks + k0;
+      const double v = Mx[(k < Np ? k : 0) * ld + (j < ncol ? j : 0)];
+      b[ks] = (k < Np && j < ncol) ? v : 0.0;
+    }
+  };
+  // A operand of K-step ks of group g from the record tile (zero for the
+  // padding columns k >= Np); ln = lane_in_loop(lane)
+  auto opA = [&](int ln, int g, int ks) -> double {
+    const int r = ln & 15, k = 4 * ks + (ln >> 4);
+    return (4 * ks + 3 < Np || k < Np) ? sMu[((4 * g + (r & 3)) * Np + k) * 4 + (r >> 2)] : 0.0;
+  };
+  const int ntiles = (a.ncells + TE - 1) / TE;
+  for (int it_t = 0;; it_t++) {
+    const int tile = xcd_tile(it_t, ntiles);
+    if (tile >= ntiles) break;
+    const int e0 = tile * TE, ne = min(TE, a.ncells - e0);
+    __syncthreads();  // the previous tile is done with the LDS tiles
+    // ---- the tile's loads, one round trip: mu, gmq and the factors into
+    // LDS; the lane's states, the epilogue's x0 (and y0) chunks and the
+    // running metric cotangents into registers
+    copy_to_lds<RCH>(a.mu + (size_t)e0 * Np * 4, sMu, ne * Np * 2, lane);
+    copy_to_lds<SCH>(a.gmq + (size_t)e0 * NF * 4, sGq, ne * NF * 2, lane);
+    copy_to_lds<TE * 2>(a.cmet + (size_t)e0 * 4, sMet, ne * 2, lane);
+    const int ln = lane_in_loop(lane), jj = ln & 15, ce = ln >> 4;
+    double2 wr[G][2];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const int e = min(4 * g + ce, ne - 1);
+      const double2 *ws = reinterpret_cast<const double2 *>(a.w + ((size_t)(e0 + e) * Np + min(jj, Np - 1)) * 4);
+      wr[g][0] = ws[0]; wr[g][1] = ws[1];
+    }
+    double bR[KP], bS[KP];
+    load_b(bR, Drw, Np, Np);
+    load_b(bS, Dsw, Np, Np);
+    // the write-back's x0 (and y0) chunks and the running metric cotangents
+    double2 xc[WB], yc[WB];
+    const int nch = ne * Np * 2;
+#pragma unroll
+    for (int q = 0; q < WB; q++) {
+      const size_t ch = (size_t)e0 * Np * 2 + min(ln + 64 * q, nch - 1);
+      xc[q] = reinterpret_cast<const double2 *>(a.x0)[ch];
+      if (OUT2) yc[q] = reinterpret_cast<const double2 *>(a.y0)[ch];
+    }
+    const size_t kc = (size_t)e0 + min(ln, ne - 1);
+    const double crx = a.g_rx[kc], csx = a.g_sx[kc], cry = a.g_ry[kc], csy = a.g_sy[kc];
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the LDS tiles are in place
+    // the face slots that read node jj (at most two)
+    int ms0 = -1, ms1 = -1;
+#pragma unroll
+    for (int m = 0; m < NF; m++)
+      if (fmask_of(m) == jj) {
+        if (ms0 < 0) ms0 = m;
+        else ms1 = m;
+      }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (4 * g >= ne) continue;
+      const int e = 4 * g + ce;
+      const bool ok = e < ne && jj < Np;
+      dbl4 nr = {0.0, 0.0, 0.0, 0.0}, ns = nr;
+#pragma unroll
+      for (int ks = 0; ks < KP; ks++) {
+        const double av = opA(ln, g, ks);
+        nr = mfma16x16x4(av, bR[ks], nr);
+        ns = mfma16x16x4(av, bS[ks], ns);
+      }
+      double grx = 0.0, gsx = 0.0, gry = 0.0, gsy = 0.0, lam[4] = {0, 0, 0, 0};
+      if (ok) {
+        const double2 *mt = reinterpret_cast<const double2 *>(sMet + e * 4);
+        const double2 rxy = mt[0], sxy = mt[1];
+        const double wv[4] = {wr[g][0].x, wr[g][0].y, wr[g][1].x, wr[g][1].y};
+        double phF[4], phG[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          phF[v] = rxy.x * nr[v] + sxy.x * ns[v];
+          phG[v] = rxy.y * nr[v] + sxy.y * ns[v];
+        }
+        euler_flux_vjp(ph.gam, wv, phF, phG, lam);
+        // slot cotangents of the node's slots
+        if (ms0 >= 0) {
+          const double2 *sl = reinterpret_cast<const double2 *>(sGq + (e * NF + ms0) * 4);
+          const double2 b01 = sl[0], b23 = sl[1];
+          lam[0] += b01.x; lam[1] += b01.y; lam[2] += b23.x; lam[3] += b23.y;
+        }
+        if (ms1 >= 0) {
+          const double2 *sl = reinterpret_cast<const double2 *>(sGq + (e * NF + ms1) * 4);
+          const double2 b01 = sl[0], b23 = sl[1];
+          lam[0] += b01.x; lam[1] += b01.y; lam[2] += b23.x; lam[3] += b23.y;
+        }
+        // metric cotangents at this node: F . nu_r, F . nu_s, G . nu_r, G . nu_s
+        double F[4], Gv[4];
+        euler_flux(ph.gam, wv, F, Gv);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          grx += F[v] * nr[v]; gsx += F[v] * ns[v]; gry += Gv[v] * nr[v]; gsy += Gv[v] * ns[v];
+        }
+      }
+      // per-cell sums over the cell's 16 lanes (fixed order), into the stash
+#pragma unroll
+      for (int d = 1; d < 16; d <<= 1) {
+        grx += __shfl_xor(grx, d, 64);
+        gsx += __shfl_xor(gsx, d, 64);
+        gry += __shfl_xor(gry, d, 64);
+        gsy += __shfl_xor(gsy, d, 64);
+      }
+      if (jj == 0) {
+        sCell[4 * e] = grx; sCell[4 * e + 1] = gsx; sCell[4 * e + 2] = gry; sCell[4 * e + 3] = gsy;
+      }
+      // scale lam into the group's mu records (the products above have read them)
+      if (ok) {
+        double2 *d = reinterpret_cast<double2 *>(sMu + (e * Np + jj) * 4);
+        d[0] = make_double2(a.scale * lam[0], a.scale * lam[1]);
+        d[1] = make_double2(a.scale * lam[2], a.scale * lam[3]);
+      }
+    }
+    __syncthreads();
+    // ---- write-back: out = e0 x0 + scale lam (and out2), coalesced
+#pragma unroll
+    for (int q = 0; q < WB; q++) {
+      const int c = ln + 64 * q;
+      if (c >= nch) continue;
+      const double2 l = reinterpret_cast<const double2 *>(sMu)[c];
+      const double2 o = make_double2(a.e0 * xc[q].x + l.x, a.e0 * xc[q].y + l.y);
+      reinterpret_cast<double2 *>(a.out)[(size_t)e0 * Np * 2 + c] = o;
+      if (OUT2)
+        reinterpret_cast<double2 *>(a.out2)[(size_t)e0 * Np * 2 + c] =
+            make_double2(a.f0 * yc[q].x + a.f1 * xc[q].x + a.f2 * o.x, a.f0 * yc[q].y + a.f1 * xc[q].y + a.f2 * o.y);
+    }
+    if (ln < ne) {
+      a.g_rx[kc] = crx + a.scale * sCell[4 * ln];
+      a.g_sx[kc] = csx + a.scale * sCell[4 * ln + 1];
+      a.g_ry[kc] = cry + a.scale * sCell[4 * ln + 2];
+      a.g_sy[kc] = csy + a.scale * sCell[4 * ln + 3];
+    }
+  }
+}
+
 // One thread per node, whole cells per workgroup; mu, the metrics and the
 // nodal Euler fluxes of the cells are staged in LDS, so each flux is
-// evaluated once per node.  PAIR (face-pair form): the face contributions
-// are the own slots' gmq alone, and the cells' psi for the next launch is
-// formed from out.
-template <int N, bool PAIR>
+// evaluated once per node (the per-slot form's cell pass).
+template <int N>
 __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops,
                                 const int *__restrict__ node_slots) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   constexpr int CPB = kAdjThreads / Np, NT = CPB * Np;
   __shared__ double sD[2 * Np * Np];
   __shared__ double sMu[4][NT], sMet[4][NT], sFG[8][NT];
-  __shared__ double sL[PAIR ? Np * NF : 1];
   for (int k = threadIdx.x; k < 2 * Np * Np; k += kAdjThreads) sD[k] = ops[k];
-  if (PAIR)
-    for (int k = threadIdx.x; k < Np * NF; k += kAdjThreads) sL[k] = ops[2 * Np * Np + k];
   const double *Drw = sD, *Dsw = sD + Np * Np;
   const int t = threadIdx.x;
   const int cl = t / Np, i = t - cl * Np;
@@ -665,15 +775,13 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phy
       double lam[4] = {0, 0, 0, 0};
       euler_flux_vjp(ph.gam, wi, phF, phG, lam);
       // face contributions that read this node: its own slots' interior
-      // side (gmq) and the facing slots' exterior side (gpq; in the
-      // face-pair form gmq already holds both)
+      // side (gmq) and the facing slots' exterior side (gpq)
       const int msl[2] = {ms0, ms1};
       for (int s = 0; s < 2; s++) {
         const int m = msl[s];
         if (m < 0) continue;
         const double4 gm = *reinterpret_cast<const double4 *>(a.gmq + (c * NF + m) * 4);
         lam[0] += gm.x; lam[1] += gm.y; lam[2] += gm.z; lam[3] += gm.w;
-        if (PAIR) continue;
         const int code = a.fmap[c * 3 + m / Nfp];
         if (code >= 0) {
           const size_t nb = code >> 2;
@@ -733,24 +841,6 @@ __global__ __launch_bounds__(kAdjThreads, 2) void adj_cell_kernel(AdjArgs a, Phy
         a.g_sy[c] += a.scale * s3;
       }
     }
-    if (PAIR) {
-      // psi of the next launch, whose mu is this launch's out: -LIFT^T out
-      // per face slot of the block's cells (as adj_psi_kernel)
-      __syncthreads();  // the metric cotangents are done with sMu
-      if (act) {
-        sMu[0][t] = o[0]; sMu[1][t] = o[1]; sMu[2][t] = o[2]; sMu[3][t] = o[3];
-      }
-      __syncthreads();
-      for (int k = t; k < nc * NF; k += kAdjThreads) {
-        const int cc = k / NF, m = k - cc * NF;
-        double p[4] = {0, 0, 0, 0};
-        for (int j = 0; j < Np; j++) {
-          const double l = sL[j * NF + m];
-          for (int v = 0; v < 4; v++) p[v] -= l * sMu[v][cc * Np + j];
-        }
-        *reinterpret_cast<double4 *>(a.psi + ((base + cc) * NF + m) * 4) = make_double4(p[0], p[1], p[2], p[3]);
-      }
-    }
   }
 }
 
@@ -774,7 +864,7 @@ __global__ __launch_bounds__(128) void init_grid_rel_kernel(int ncells, const do
                                                             const double *__restrict__ cell_rel,
                                                             double *rx, double *ry, double *sx,
                                                             double *sy, double *nx, double *ny,
-                                                            double *fs, double *J) {
+                                                            double *fs, double *J, double *cmet) {
   constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
   const double *Dr = ops + 2 * Np * Np + Np * NF;
   const double *Ds = Dr + Np * Np;
@@ -791,6 +881,9 @@ __global__ __launch_bounds__(128) void init_grid_rel_kernel(int ncells, const do
     }
     for (int m = 0; m < NF; m++) { nx[c * NF + m] = p[m]; ny[c * NF + m] = q[m]; fs[c * NF + m] = f[m]; }
     J[c] = Jc;
+    if (cmet) {
+      cmet[4 * c] = a[0]; cmet[4 * c + 1] = b[0]; cmet[4 * c + 2] = cc[0]; cmet[4 * c + 3] = dd[0];
+    }
   }
 }
 
@@ -915,16 +1008,28 @@ template <int N>
 static hipError_t adj_n(const Launch &L, const AdjArgs &a, const int *node_slots) {
   hipError_t e = hipSuccess;
   if (a.pairs) {
-    // boundary blocks first, so that their longer threads start early
-    const int nbblk = a.nbedges > 0 ? (a.nbedges * Ord<N>::Nfp * 8 + kAdjThreads - 1) / kAdjThreads : 0;
+    // face-pair form: the face-pair pass (boundary blocks first, so that
+    // their longer threads start early; psi from mu), then the cell pass on
+    // the matrix cores
+    if (!a.cmet || !a.x0 || (a.out2 && !a.y0)) return hipErrorInvalidValue;
+    const int nbblk = (a.nbedges * Ord<N>::Nfp + kAdjThreads - 1) / kAdjThreads;
     const int fpb = kAdjThreads / Ord<N>::Nfp;
     const int npb = (a.npairs + fpb - 1) / fpb;
     if (nbblk + npb > 0)
-      hipLaunchKernelGGL(adj_pair_kernel<N>, dim3(nbblk + npb), dim3(kAdjThreads), 0, L.stream, a, L.ph,
-                         L.fmask, nbblk);
+      hipLaunchKernelGGL(adj_pair_kernel<N>, dim3(nbblk + npb), dim3(kAdjThreads), 0, L.stream, a, L.ph, L.fmask,
+                         L.ops, nbblk);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL((adj_cell_kernel<N, true>), dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::Np)),
-                       dim3(kAdjThreads), 0, L.stream, a, L.ph, L.ops, node_slots);
+    // one-wave workgroups, one 16-cell tile each (the XCD-chunked walk gives
+    // each XCD a contiguous range of tiles)
+    int grid = (a.ncells + kMfmaTile - 1) / kMfmaTile;
+    if (L.max_blocks > 0 && grid > L.max_blocks) grid = L.max_blocks;
+    if (grid >= 8) grid += (8 - grid % 8) % 8;
+    if (a.out2)
+      hipLaunchKernelGGL((adj_cell_mfma_kernel<N, true>), dim3(grid), dim3(64), 0, L.stream, a, L.ph, L.ops,
+                         (unsigned long long)L.fmask4);
+    else
+      hipLaunchKernelGGL((adj_cell_mfma_kernel<N, false>), dim3(grid), dim3(64), 0, L.stream, a, L.ph, L.ops,
+                         (unsigned long long)L.fmask4);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(adj_face_kernel<N>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::NF)),
@@ -936,20 +1041,8 @@ static hipError_t adj_n(const Launch &L, const AdjArgs &a, const int *node_slots
     hipLaunchKernelGGL(adj_boundary_kernel<N>, dim3(nb), dim3(256), 0, L.stream, a, L.ph, L.fmask);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((adj_cell_kernel<N, false>), dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::Np)),
+  hipLaunchKernelGGL(adj_cell_kernel<N>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<N>::Np)),
                      dim3(kAdjThreads), 0, L.stream, a, L.ph, L.ops, node_slots);
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_psi(const Launch &L, const AdjArgs &a) {
-  if (a.ncells == 0) return hipSuccess;
-  switch (L.N) {
-    case 1: hipLaunchKernelGGL(adj_psi_kernel<1>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<1>::NF)), dim3(kAdjThreads), 0, L.stream, a, L.ops); break;
-    case 2: hipLaunchKernelGGL(adj_psi_kernel<2>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<2>::NF)), dim3(kAdjThreads), 0, L.stream, a, L.ops); break;
-    case 3: hipLaunchKernelGGL(adj_psi_kernel<3>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<3>::NF)), dim3(kAdjThreads), 0, L.stream, a, L.ops); break;
-    case 4: hipLaunchKernelGGL(adj_psi_kernel<4>, dim3(cell_blocks(a.ncells, kAdjThreads / Ord<4>::NF)), dim3(kAdjThreads), 0, L.stream, a, L.ops); break;
-    default: return hipErrorInvalidValue;
-  }
   return hipGetLastError();
 }
 
@@ -1004,12 +1097,12 @@ hipError_t launch_geo_vjp(const Launch &L, int ncells, const double *cell_rel, c
 
 hipError_t launch_init_grid_rel(const Launch &L, int ncells, const double *cell_rel, double *rx,
                                 double *ry, double *sx, double *sy, double *nx, double *ny,
-                                double *fs, double *J) {
+                                double *fs, double *J, double *cmet) {
   if (ncells == 0) return hipSuccess;
   size_t nb = ((size_t)ncells + 127) / 128;
   const int grid = nb > 65536 ? 65536 : (int)nb;
 #define DG_IGR(n) hipLaunchKernelGGL(init_grid_rel_kernel<n>, dim3(grid), dim3(128), 0, L.stream, ncells, \
-                                     L.ops, L.fmask, cell_rel, rx, ry, sx, sy, nx, ny, fs, J)
+                                     L.ops, L.fmask, cell_rel, rx, ry, sx, sy, nx, ny, fs, J, cmet)
   switch (L.N) {
     case 1: DG_IGR(1); break;
     case 2: DG_IGR(2); break;

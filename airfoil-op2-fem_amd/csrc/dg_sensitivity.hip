@@ -108,12 +108,12 @@ struct dg_sens {
   DevBuf buf;
   // device geometry and maps
   double *cell_rel, *rx, *ry, *sx, *sy, *nx, *ny, *fs, *J;
+  double *cmet;  // [K][4] (rx, ry, sx, sy) of node 0: the reverse sweep's per-cell factors
   int *fmap, *node_slots, *cell_off, *tile_off;
-  // interior faces once each, (c * 3 + f) of both sides, lower cell first,
-  // and the flux cotangents psi [K][NF][4] of the face-pair reverse passes
+  // interior faces once each, (c * 3 + f) of both sides, lower cell first
+  // (the face-pair reverse pass)
   int *pairs;
   int npairs = 0;
-  double *psi;
   dg::Phys *phys_d;
   unsigned long long *dt1;
   double *tdev;
@@ -213,7 +213,7 @@ int set_shape(dg_sens *s, const std::vector<double> &b) {
   HIPCHECK(c, hipMemcpyAsync(s->cell_rel, cr.data(), sizeof(double) * cr.size(), hipMemcpyHostToDevice,
                              s->stream));
   HIPCHECK(c, dg::launch_init_grid_rel(launch_a(s), s->K, s->cell_rel, s->rx, s->ry, s->sx, s->sy,
-                                       s->nx, s->ny, s->fs, s->J));
+                                       s->nx, s->ny, s->fs, s->J, s->cmet));
   HIPCHECK(c, hipStreamSynchronize(s->stream));
   return DG_OK;
 }
@@ -418,6 +418,7 @@ void dg_sens_destroy(dg_sens *s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->stream2) (void)hipStreamSynchronize(s->stream2);
+
   for (double *p : s->ckpt) (void)hipFree(p);
   for (double *p : s->seg) (void)hipFree(p);
   for (double *p : s->seg2) (void)hipFree(p);
@@ -427,6 +428,7 @@ void dg_sens_destroy(dg_sens *s) {
     if (s->ev_free[b]) (void)hipEventDestroy(s->ev_free[b]);
   }
   if (s->stream2) (void)hipStreamDestroy(s->stream2);
+
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -467,6 +469,7 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   const size_t K = s->K, Np = s->Np, NF = s->NF, ST = K * Np * 4, FT = K * NF * 4;
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking);
+
   for (int k = 0; k < 2 && e == hipSuccess; k++) {
     e = hipEventCreateWithFlags(&s->ev_ready[k], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_free[k], hipEventDisableTiming);
@@ -478,6 +481,7 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
   s->sx = b.alloc<double>(K * Np); s->sy = b.alloc<double>(K * Np);
   s->nx = b.alloc<double>(K * NF); s->ny = b.alloc<double>(K * NF); s->fs = b.alloc<double>(K * NF);
   s->J = b.alloc<double>(K);
+  s->cmet = b.alloc<double>(K * 4);
   s->fmap = b.alloc<int>(fmap.size());
   std::vector<int> pairs;
   for (size_t k = 0; k < K; k++)
@@ -492,7 +496,6 @@ int dg_sens_create(dg_ctx *c, int ncirc, int nrad, double stretch, double rfar, 
     }
   s->npairs = (int)(pairs.size() / 2);
   s->pairs = b.alloc<int>(pairs.size());
-  s->psi = b.alloc<double>(FT);
   s->node_slots = b.alloc<int>(2 * Np);
   s->cell_off = b.alloc<int>(2); s->tile_off = b.alloc<int>(2);
   s->phys_d = b.alloc<dg::Phys>(1);
@@ -866,13 +869,10 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   a.g_nxf = s->g_nxf; a.g_nyf = s->g_nyf; a.g_fsf = s->g_fsf;
   a.compact = 1;
   a.nbedges = s->mesh.nbedges; a.b2c = s->d_b2c; a.bnum = s->d_bnum;
-  // face-pair form: psi of the first launch from lam here, of every later
-  // launch from the cell pass before it
-  a.pairs = s->pairs; a.npairs = s->npairs; a.psi = s->psi;
-  if (a.pairs) {
-    a.mu = s->lam;
-    HIPCHECK(c, dg::launch_adj_psi(L, a));
-  }
+  // face-pair form: the face-pair pass forms psi from mu, the cell pass
+  // runs on the matrix cores with the per-cell factors
+  a.pairs = s->pairs; a.npairs = s->npairs; a.cmet = s->cmet;
+  auto rhs_adj = [&]() -> hipError_t { return dg::launch_rhs_adj(L, a, s->node_slots); };
   for (int sg = nck - 1; sg >= 0; sg--) {
     const int n0 = sg * ckpt_every, n1 = std::min(s->nsteps, n0 + ckpt_every);
     const int b = (nck - 1 - sg) & 1;
@@ -882,14 +882,14 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
       const int j = n - n0;
       a.w = seg_at(s, b, 3 * j + 2); a.mu = s->lam;
       a.x0 = s->lam; a.e0 = 2.0 / 3.0; a.scale = 2.0 / 3.0 * dt; a.out = s->A; a.out2 = nullptr;
-      HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
+      HIPCHECK(c, rhs_adj());
       a.w = seg_at(s, b, 3 * j + 1); a.mu = s->A;
       a.x0 = s->A; a.e0 = 0.25; a.scale = 0.25 * dt; a.out = s->B;
       a.out2 = s->lam; a.y0 = s->lam; a.f0 = 1.0 / 3.0; a.f1 = 0.75; a.f2 = 1.0;
-      HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
+      HIPCHECK(c, rhs_adj());
       a.w = seg_at(s, b, 3 * j); a.mu = s->B;
       a.x0 = s->lam; a.e0 = 1.0; a.scale = dt; a.out = s->lam; a.out2 = nullptr;
-      HIPCHECK(c, dg::launch_rhs_adj(L, a, s->node_slots));
+      HIPCHECK(c, rhs_adj());
     }
     HIPCHECK(c, hipEventRecord(s->ev_free[b], s->stream));
     if (sg >= 2) HIPCHECK(c, recompute(sg - 2));

@@ -122,7 +122,13 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   constexpr int GS = mfma_fg_group<N>();    // doubles per group in the F and G tiles
   constexpr int GL = mfma_fl_group<N>();    // doubles per group in the face-flux tile
   constexpr int CS = kColStride, CSL = kFaceColStride;
-  constexpr bool kItemsInLoop = EPI == EPI_STAGE2 || N >= 4;  // see lane_in_loop
+constexpr bool kItemsInLoop = EPI == EPI_STAGE1 || EPI == EPI_STAGE2 || N >= 4;  // see lane_in_loop
+  // Roe items one ahead: item j + 1's loads go out before item j's flux.
+  // Measured on the 724x724 vortex: stages 1 and 2 and the per-cell geometry
+  // gain (per-cell step 1.10 -> 1.03 ms); the per-node stage-0 kernel ran
+  // ~1.5% slower with it and loads each item just before its flux.  The
+  // drop-in euler_rhs kernel (GATHER false) loads like the fused RHS kernel.
+  constexpr bool kRoeAhead = CG || EPI != EPI_STAGE0;
   static_assert(Np <= 16, "one 16-row output tile per cell group");
   // per wave: F tiles [G][Np][CS] then G tiles [G][Np][CS] (column k of a
   // group holds its rows cell + 4*var; the padding columns k >= Np of the
@@ -243,26 +249,31 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     {
       const int nrt = ne * NF;
       const int lr = kItemsInLoop ? lane_in_loop(lane) : lane;
-#pragma unroll
-      for (int j = 0; j < NRT; j++) {
-        const int it = lr + 64 * j;
-        if (it >= nrt) continue;
+      // one item's loads: own trace, exterior trace and geometry go out
+      // together, before any branch; a boundary slot reads its own record as
+      // a stand-in exterior, which boundary_state then replaces; an
+      // exact-state slot reads the state exact_bc_kernel stored in its
+      // exteriorQ slot for this stage; the drop-in euler_rhs (GATHER false)
+      // reads exteriorQ.  An item past the tile's end re-reads item 0's data
+      // (its face-map code is 0), which nobody uses.
+      struct RoeIn {
+        double2 m01, m23, p01, p23;
+        double g[4];  // nx, ny, fscale; or the cell's rx, ry, sx, sy (CG)
+      };
+      auto load_item = [&](int j, RoeIn &r) {
+        const int it0 = lr + 64 * j;
+        const int it = it0 < nrt ? it0 : 0;
         const int e = it / NF, m = it - e * NF;
         const int i = m - (m / Nfp) * Nfp;
         const size_t cell = (size_t)(e0 + e);
         const size_t gm = cell * NF + m;
-        double nxv, nyv, fs;
         if constexpr (CG) {
           const double2 *g = reinterpret_cast<const double2 *>(a.cmet + cell * 4);
           const double2 rxy = g[0], sxy = g[1];
-          cell_face_geometry(rxy.x, rxy.y, sxy.x, sxy.y, m / Nfp, nxv, nyv, fs);
+          r.g[0] = rxy.x; r.g[1] = rxy.y; r.g[2] = sxy.x; r.g[3] = sxy.y;
         } else {
-          nxv = a.nx[gm]; nyv = a.ny[gm]; fs = a.fscale[gm];
+          r.g[0] = a.nx[gm]; r.g[1] = a.ny[gm]; r.g[2] = a.fscale[gm];
         }
-        // own and exterior trace loads issued together, before any branch:
-        // a boundary slot reads its own record as a stand-in exterior, which
-        // boundary_state then replaces; an exact-state slot reads the state
-        // exact_bc_kernel stored in its exteriorQ slot for this stage
         const size_t own = cell * Np + fmask_of(m);
         const bool interior = GATHER && code[j] >= 0;
         const bool exact = GATHER && code[j] == -(1 + kBcExact);
@@ -271,22 +282,40 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         const double2 *ms = reinterpret_cast<const double2 *>(a.w + own * 4);
         const double2 *ps = (GATHER && !exact) ? reinterpret_cast<const double2 *>(a.w + nrec * 4)
                                                : reinterpret_cast<const double2 *>(a.ext + gm * 4);
-        const double2 m01 = ms[0], m23 = ms[1];
-        const double2 p01 = ps[0], p23 = ps[1];
-        double mQ[4] = {m01.x, m01.y, m23.x, m23.y};
-        double pQ[4] = {p01.x, p01.y, p23.x, p23.y};
-        if (!GATHER) {
-          double2 *z = reinterpret_cast<double2 *>(a.ext + gm * 4);
-          z[0] = make_double2(0.0, 0.0);  // OP_RW: euler_rhs zeroes exteriorQ
-          z[1] = make_double2(0.0, 0.0);
-        } else if (!interior && !exact) {
-          boundary_state(ph, -code[j] - 1, mQ, nxv, nyv, pQ);
-        }
-        double fl[4];
-        roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
-        const int off = (e >> 2) * GL + CSL * m + (e & 3);
+        r.m01 = ms[0]; r.m23 = ms[1];
+        r.p01 = ps[0]; r.p23 = ps[1];
+      };
+      RoeIn cur, nxt;
+      if (kRoeAhead) load_item(0, cur);
 #pragma unroll
-        for (int v = 0; v < 4; v++) sL[off + 4 * v] = fl[v];
+      for (int j = 0; j < NRT; j++) {
+        if (!kRoeAhead) load_item(j, cur);
+        else if (j + 1 < NRT) load_item(j + 1, nxt);
+        const int it = lr + 64 * j;
+        if (it < nrt) {
+          const int e = it / NF, m = it - e * NF;
+          double nxv, nyv, fs;
+          if constexpr (CG) {
+            cell_face_geometry(cur.g[0], cur.g[1], cur.g[2], cur.g[3], m / Nfp, nxv, nyv, fs);
+          } else {
+            nxv = cur.g[0]; nyv = cur.g[1]; fs = cur.g[2];
+          }
+          double mQ[4] = {cur.m01.x, cur.m01.y, cur.m23.x, cur.m23.y};
+          double pQ[4] = {cur.p01.x, cur.p01.y, cur.p23.x, cur.p23.y};
+          if (!GATHER) {
+            double2 *z = reinterpret_cast<double2 *>(a.ext + ((size_t)(e0 + e) * NF + m) * 4);
+            z[0] = make_double2(0.0, 0.0);  // OP_RW: euler_rhs zeroes exteriorQ
+            z[1] = make_double2(0.0, 0.0);
+          } else if (code[j] < 0 && code[j] != -(1 + kBcExact)) {
+            boundary_state(ph, -code[j] - 1, mQ, nxv, nyv, pQ);
+          }
+          double fl[4];
+          roe_node(ph.gam, mQ, pQ, nxv, nyv, fs, fl);
+          const int off = (e >> 2) * GL + CSL * m + (e & 3);
+#pragma unroll
+          for (int v = 0; v < 4; v++) sL[off + 4 * v] = fl[v];
+        }
+        if (kRoeAhead && j + 1 < NRT) cur = nxt;
       }
       if (NF < 4 * KF) {  // zero padding columns (orders whose 3(N+1) is not a multiple of 4)
         constexpr int PADF = 4 * KF - NF;
@@ -373,19 +402,18 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       const int e = 4 * g + ce;
       const bool ok = e < ne && jj < Np;
       const size_t gn = (size_t)(e0 + (ok ? e : 0)) * Np + (ok ? jj : 0);
-      // This group's first-touch global loads (metric factors, base state
-      // q) go out before its products, so their latency overlaps the matrix
-      // work; the stage input w, which the Euler-flux phase has just brought
-      // into the cache, follows the products when q is needed too (fewer
-      // registers live across them).
-      constexpr bool early_w = !(EPI == EPI_STAGE1 || EPI == EPI_STAGE2);
+      // This group's global loads (metric factors, stage input w, base
+      // state q) go out before its products, so their latency overlaps the
+      // matrix work.  (Loading w after the products, which kept 8 fewer
+      // registers live across them, put a dependent round trip into every
+      // group's epilogue: stage 2 measured 432 -> 411 us with it moved up.)
       double rxv = 0.0, sxv = 0.0, ryv = 0.0, syv = 0.0;
       double2 w01 = make_double2(0.0, 0.0), w23 = w01, q01 = w01, q23 = w01;
       if (ok) {
         if constexpr (!CG) {
           rxv = a.rx[gn]; sxv = a.sx[gn]; ryv = a.ry[gn]; syv = a.sy[gn];
         }
-        if (EPI != EPI_RHS && early_w) {
+        if (EPI != EPI_RHS) {
           const double2 *ws = reinterpret_cast<const double2 *>(a.w + gn * 4);
           w01 = ws[0]; w23 = ws[1];
         }
@@ -425,10 +453,6 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         for (int r = 0; r < 4; r++) c[r] += ryv * d1[r] + syv * d2[r];
       }
       if (!ok) continue;
-      if (EPI != EPI_RHS && !early_w) {
-        const double2 *ws = reinterpret_cast<const double2 *>(a.w + gn * 4);
-        w01 = ws[0]; w23 = ws[1];
-      }
       double o[4];
       if (EPI == EPI_RHS) {
 #pragma unroll

@@ -150,7 +150,10 @@ def test_stage_kernels_match_oracle(dg, orc, dev, kind):
     assert torch.count_nonzero(ext).item() == 0  # euler_rhs zeroes exteriorQ (OP_RW)
     fused = torch.zeros_like(rhs)
     ctx.rhs_fused(wd, geo["rx"], geo["ry"], geo["sx"], geo["sy"], geo["fscale"], geo["nx"], geo["ny"], fused)
-    assert torch.equal(fused, rhs)  # same arithmetic, traces gathered instead of staged
+    # the same arithmetic with the traces gathered instead of staged; the two
+    # kernels are separate compilations, whose multiply-add contractions the
+    # compiler picks per kernel, so they agree to rounding, not bit for bit
+    assert max(rel_per_var(fused.cpu().numpy(), got)) <= 1e-13
 
 
 def test_stage_updates_and_calc_dt(dg, orc, dev):

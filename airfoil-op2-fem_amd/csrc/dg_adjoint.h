@@ -18,11 +18,12 @@ struct TlmArgs {
   double *out;    // [K][Np][4]: tangent of the RHS
 };
 
-// State epilogue of the adjoint RHS (element-wise, so out may alias x0 and
-// out2 may alias y0):
+// State epilogue of the adjoint RHS, per-slot form (element-wise, so out may
+// alias x0 and out2 may alias y0):
 //   out  = e0 x0 + scale L'(w)^T mu            (x0 may be null)
 //   out2 = f0 y0 + f1 x0 + f2 out              (when out2 is not null)
-// The reverse SSPRK3 step is three such launches with no separate
+// (the face-pair form's epilogue is described with its fields below).  The
+// reverse SSPRK3 step is three such launches with no separate
 // linear-combination passes.
 struct AdjArgs {
   int ncells;
@@ -49,23 +50,32 @@ struct AdjArgs {
   int compact;
   int nbedges;                          // boundary edges (cell, local face)
   const int *b2c, *bnum;
-  // Face-pair form (compact only; null pairs: the per-slot form).  pairs
-  // [npairs][2] lists each interior face once as (c * 3 + f) of its two
-  // sides, the first the lower cell; cmet [K][4] holds (rx, ry, sx, sy) of
-  // each cell (its node-0 factors: a straight-sided cell has the same
-  // factors at every node).  The face-pair pass forms psi = -LIFT^T mu, the
-  // cotangent of the scaled fluxes, from mu itself; the cell pass runs on
-  // the matrix cores with the per-cell factors.  Needs x0 (and y0 with
-  // out2).
+  // Face-pair form (the gradient's reverse sweep; compact only; null pairs:
+  // the per-slot form).  pairs [npairs][2] lists each interior face once as
+  // (c * 3 + f) of its two sides, the first the lower cell; cmet [K][4]
+  // holds (rx, ry, sx, sy) of each cell (its node-0 factors: a
+  // straight-sided cell has the same factors at every node).  The face-pair
+  // pass reads psi [K][NF][4] = -LIFT^T mu, the cotangent of the scaled
+  // fluxes (the previous launch's psi_out, or launch_adj_psi for the
+  // first); the cell pass runs on the matrix cores with the per-cell
+  // factors and writes psi_out = -LIFT^T out.  Its state epilogue is
+  //   out = em mu + scale L'(w)^T mu  (+ e0 x0 + f0 y0 when x0 is not null;
+  //   x0 and y0 both given; out may alias y0; out2 unused)
   const int *pairs;
   int npairs;
   const double *cmet;
+  double em;
+  const double *psi;
+  double *psi_out;
 };
 
 // Ld = L'(w) wd + dL/dgeo geod + dL/dbc (bcud, bcvd)
 hipError_t launch_rhs_tlm(const Launch &L, const TlmArgs &a, double bcud, double bcvd);
 // node_slots: [Np][2] face slots m with FMASK[m] == node (-1 when none)
 hipError_t launch_rhs_adj(const Launch &L, const AdjArgs &a, const int *node_slots);
+// psi [K][NF][4] = -LIFT^T mu [K][Np][4] on the matrix cores, with the
+// arithmetic of the fused reverse RHS
+hipError_t launch_adj_psi(const Launch &L, int ncells, const double *mu, double *psi);
 // z = a x + b y  (y may be null)
 hipError_t launch_lincomb(const Launch &L, size_t n, double a, const double *x, double b,
                           const double *y, double *z);

@@ -18,8 +18,11 @@
 //           sweep): one reverse Roe flux per interior face node pair (and per
 //           boundary slot, with the hand-written reverse of the boundary
 //           state), then the cell pass on the matrix cores with per-cell
-//           factors (adj_cell_mfma_kernel).
+//           factors (adj_cell_mfma_kernel), which also forms the next
+//           launch's psi = -LIFT^T out on the matrix cores.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "dg_adjoint.h"
 #include "dg_bc.h"
@@ -427,40 +430,23 @@ __global__ __launch_bounds__(256) void adj_boundary_kernel(AdjArgs a, Phys ph, c
 // per-node rounding makes the two normals opposite only to ~1e-15 relative
 // times the cell's aspect, and moves the gradient by that much.
 //
-// -LIFT^T mu at slot m of cell c (LIFT [Np][NF] in LDS): the cotangent of
-// the cell's scaled flux there.
-template <int Np, int NF>
-__device__ __forceinline__ void psi_of_mu(const double *__restrict__ mu, size_t c, int m, const double *sL,
-                                          double p[4]) {
-  p[0] = p[1] = p[2] = p[3] = 0.0;
-#pragma unroll 5
-  for (int j = 0; j < Np; j++) {
-    const double4 u = *reinterpret_cast<const double4 *>(mu + (c * Np + j) * 4);
-    const double l = sL[j * NF + m];
-    p[0] -= l * u.x; p[1] -= l * u.y; p[2] -= l * u.z; p[3] -= l * u.w;
-  }
-}
-
 // Blocks [0, nbblk): the boundary slots, one thread each: the slot's
 // reverse Roe flux with the boundary state as exterior, whose cotangent the
 // hand-written reverse of the boundary state (boundary_vjp) carries to the
 // trace, the normal and the free-stream velocity; per-slot outputs.
 // Blocks [nbblk, grid): one thread per interior face node pair, Nfp
 // consecutive threads per face, the per-face geometry sums in node order.
-// psi, the cotangent of the scaled fluxes, is formed from mu here
-// (psi_of_mu); the cell pass before wrote mu as its out.
+// psi, the cotangent of the scaled fluxes (-LIFT^T mu), is read from a.psi,
+// which the cell pass before (or launch_adj_psi) wrote on the matrix cores.
 template <int N>
 __global__ __launch_bounds__(kAdjThreads, 3) void adj_pair_kernel(AdjArgs a, Phys ph,
-                                                                const int *__restrict__ fmask,
-                                                                const double *__restrict__ ops, int nbblk) {
+                                                                const int *__restrict__ fmask, int nbblk) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
   constexpr int FPB = kAdjThreads / Nfp;  // faces per workgroup visit
   __shared__ double sG[4][kAdjThreads];
-  __shared__ double sL[Np * NF];
   __shared__ int sFm[NF];
   const int t = threadIdx.x;
   if (t < NF) sFm[t] = fmask[t];
-  for (int k = t; k < Np * NF; k += kAdjThreads) sL[k] = ops[2 * Np * Np + k];
   __syncthreads();
   if ((int)blockIdx.x < nbblk) {
     const int bi = (int)blockIdx.x * kAdjThreads + t;
@@ -472,8 +458,8 @@ __global__ __launch_bounds__(kAdjThreads, 3) void adj_pair_kernel(AdjArgs a, Phy
     if (code >= 0) return;  // not a boundary face (never for a valid mesh)
     const size_t slot = c * NF + m;
     const double4 q4 = *reinterpret_cast<const double4 *>(a.w + (c * Np + sFm[m]) * 4);
-    double psi[4];
-    psi_of_mu<Np, NF>(a.mu, c, m, sL, psi);
+    const double4 p4 = *reinterpret_cast<const double4 *>(a.psi + slot * 4);
+    const double psi[4] = {p4.x, p4.y, p4.z, p4.w};
     const double nx = a.nx[slot], ny = a.ny[slot], fs = a.fscale[slot];
     const double mQ[4] = {q4.x, q4.y, q4.z, q4.w};
     double pQ[4];
@@ -506,14 +492,8 @@ __global__ __launch_bounds__(kAdjThreads, 3) void adj_pair_kernel(AdjArgs a, Phy
       const size_t so = (size_t)co * NF + mo, sp = (size_t)cp * NF + mp;
       const double4 q4 = *reinterpret_cast<const double4 *>(a.w + ((size_t)co * Np + sFm[mo]) * 4);
       const double4 e4 = *reinterpret_cast<const double4 *>(a.w + ((size_t)cp * Np + sFm[mp]) * 4);
-      double4 po, pp;
-      {
-        double p[4];
-        psi_of_mu<Np, NF>(a.mu, co, mo, sL, p);
-        po = make_double4(p[0], p[1], p[2], p[3]);
-        psi_of_mu<Np, NF>(a.mu, cp, mp, sL, p);
-        pp = make_double4(p[0], p[1], p[2], p[3]);
-      }
+      const double4 po = *reinterpret_cast<const double4 *>(a.psi + so * 4);
+      const double4 pp = *reinterpret_cast<const double4 *>(a.psi + sp * 4);
       const double nx = a.nx[so], ny = a.ny[so], fso = a.fscale[so], fsp = a.fscale[sp];
       const double mQ[4] = {q4.x, q4.y, q4.z, q4.w}, pQ[4] = {e4.x, e4.y, e4.z, e4.w};
       const double o[4] = {fso * po.x - fsp * pp.x, fso * po.y - fsp * pp.y, fso * po.z - fsp * pp.z,
@@ -543,34 +523,115 @@ __global__ __launch_bounds__(kAdjThreads, 3) void adj_pair_kernel(AdjArgs a, Phy
   }
 }
 
+// ------------------------------------------------------------------------
+// psi = -LIFT^T mu on the matrix cores.  A tile of 16 consecutive cells'
+// records (4 groups of 4 cells) is an A operand of v_mfma_f64_16x16x4f64 as
+// it lies in memory: row cell + 4 var of group g, column node k at
+// ((4 g + cell) Np + k) 4 + var; with B = -LIFT [Np][NF] lane l ends with
+// cell l/16 of the group, slot l%16, the 4 variables.
+// ------------------------------------------------------------------------
+template <int N>
+struct AdjTile {
+  static constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF, KP = (Np + 3) / 4;
+  static constexpr int RCH = kMfmaTile * Np * 2;  // 16-byte chunks of a tile's records
+  static constexpr int RD = copy_to_lds_doubles<RCH>();
+};
+
+// A operand of K-step ks of group g from a record tile (zero for the
+// padding columns k >= Np); ln = lane_in_loop(lane)
+template <int N>
+__device__ __forceinline__ double adj_opA(const double *sR, int ln, int g, int ks) {
+  constexpr int Np = Ord<N>::Np;
+  const int r = ln & 15, k = 4 * ks + (ln >> 4);
+  return (4 * ks + 3 < Np || k < Np) ? sR[((4 * g + (r & 3)) * Np + k) * 4 + (r >> 2)] : 0.0;
+}
+
+// B operand rows of M [Np][ld] for columns j < ncol (lane l: B[4 ks + l/16][l%16]),
+// times sgn
+template <int N>
+__device__ __forceinline__ void adj_load_b(double *b, const double *M, int ld, int ncol, double sgn, int ln) {
+  constexpr int Np = Ord<N>::Np, KP = (Np + 3) / 4;
+  const int j = ln & 15, k0 = ln >> 4;
+#pragma unroll
+  for (int ks = 0; ks < KP; ks++) {
+    const int k = 4 * ks + k0;
+    const double v = M[(k < Np ? k : 0) * ld + (j < ncol ? j : 0)];
+    b[ks] = (k < Np && j < ncol) ? sgn * v : 0.0;
+  }
+}
+
+// psi = -LIFT^T of group g's records (bP: adj_load_b of LIFT, sign -1)
+template <int N>
+__device__ __forceinline__ dbl4 adj_psi_group(const double *sR, int ln, int g, const double *bP) {
+  constexpr int KP = AdjTile<N>::KP;
+  dbl4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < KP; ks++) d = mfma16x16x4(adj_opA<N>(sR, ln, g, ks), bP[ks], d);
+  return d;
+}
+
+// psi = -LIFT^T mu of every cell into psi [K][NF][4] (the first launch of a
+// sweep; later launches write it as psi_out), with the arithmetic of the
+// fused kernel.
+template <int N>
+__global__ __launch_bounds__(64) void adj_psi_mfma_kernel(int ncells, const double *__restrict__ mu,
+                                                         const double *__restrict__ ops, double *__restrict__ psi) {
+  constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF, KP = AdjTile<N>::KP;
+  constexpr int TE = kMfmaTile, RCH = AdjTile<N>::RCH, RD = AdjTile<N>::RD;
+  __shared__ double sMu[RD];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < RD; i += 64) sMu[i] = 0.0;
+  const int ntiles = (ncells + TE - 1) / TE;
+  for (int it_t = 0;; it_t++) {
+    const int tile = xcd_tile(it_t, ntiles);
+    if (tile >= ntiles) break;
+    const int e0 = tile * TE, ne = min(TE, ncells - e0);
+    __syncthreads();
+    copy_to_lds<RCH>(mu + (size_t)e0 * Np * 4, sMu, ne * Np * 2, lane);
+    const int ln = lane_in_loop(lane), jj = ln & 15, ce = ln >> 4;
+    double bP[KP];
+    adj_load_b<N>(bP, ops + 2 * Np * Np, NF, NF, -1.0, ln);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      if (4 * g >= ne) continue;
+      const dbl4 p = adj_psi_group<N>(sMu, ln, g, bP);
+      const int e = 4 * g + ce;
+      if (e < ne && jj < NF) {
+        double2 *d = reinterpret_cast<double2 *>(psi + ((size_t)(e0 + e) * NF + jj) * 4);
+        d[0] = make_double2(p[0], p[1]);
+        d[1] = make_double2(p[2], p[3]);
+      }
+    }
+  }
+}
+
 // Cell pass of the face-pair form on the matrix cores (per-cell metric
 // factors cmet, per-cell metric cotangents).  Each wave owns tiles of 16
 // consecutive cells, 4 groups of 4 cells, laid out for
 // v_mfma_f64_16x16x4f64 as in the stage kernels.  The tile's mu records and
 // its slot cotangents gmq (both sides of each face, written by the pair
-// passes) are copied into LDS by direct-to-LDS loads; the mu record layout
-// is already an A operand: row cell + 4 var of group g, column node k, at
-// ((4 g + cell) Np + k) 4 + var.  Per group, nu_r = Drw^T mu and
+// pass) are copied into LDS by direct-to-LDS loads; the mu record layout
+// is already an A operand (adj_opA).  Per group, nu_r = Drw^T mu and
 // nu_s = Dsw^T mu on the matrix cores (lane: cell, node j, the 4 variables
 // in its accumulator registers); with the cell's factors phF = rx nu_r +
 // sx nu_s, phG = ry nu_r + sy nu_s, and lam = (dF/dw)^T phF + (dG/dw)^T phG
 // plus the slot cotangents of the node's slots; the metric cotangents
 // sum_i mu_i . (Drw F)_i = sum_j F_j . nu_r_j (etc.) per cell by shuffles.
-// scale lam replaces the group's mu records; the write-back forms out
-// (and out2) chunk by chunk, coalesced, and leaves out in the records, from
-// which psi = -LIFT^T out for the next launch comes on the matrix cores.
-// One batched memory round trip per tile: the LDS copies, the lane's
-// states, the epilogue's x0 / y0 and the running metric cotangents.
+// scale lam goes to the cell's slot-cotangent rows (read by then), the
+// write-back forms out = em mu + scale lam (+ e0 x0 + f0 y0) chunk by chunk,
+// coalesced, with mu from the LDS copy, and leaves out in the record tile,
+// from which psi_out = -LIFT^T out (the next launch's psi: its mu is this
+// out) comes on the matrix cores.
 // The per-cell factors are node 0's; the forward's per-node factors differ
 // from them by init_grid's rounding, which moves the gradient by that much.
-template <int N, bool OUT2>
+template <int N, bool XY>
 __global__ __launch_bounds__(64, 3) void adj_cell_mfma_kernel(AdjArgs a, Phys ph, const double *__restrict__ ops,
                                                              unsigned long long fmask4) {
-  constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF;
-  constexpr int KP = (Np + 3) / 4;  // K-steps over nodes
+  constexpr int Np = Ord<N>::Np, NF = Ord<N>::NF, KP = AdjTile<N>::KP;
   constexpr int TE = kMfmaTile, G = 4;
-  constexpr int RCH = TE * Np * 2, SCH = TE * NF * 2;  // 16-byte chunks of a tile's records
-  constexpr int RD = copy_to_lds_doubles<RCH>(), SD = copy_to_lds_doubles<SCH>();
+  constexpr int RCH = AdjTile<N>::RCH, SCH = TE * NF * 2;  // 16-byte chunks of a tile's records
+  constexpr int RD = AdjTile<N>::RD, SD = copy_to_lds_doubles<SCH>();
   constexpr int WB = (RCH + 63) / 64;  // write-back chunks per lane
   static_assert(Np <= 16 && NF <= 16, "one 16-column output tile per cell group");
   __shared__ double sMu[RD], sGq[SD], sMet[128], sCell[TE * 4];
@@ -578,24 +639,7 @@ __global__ __launch_bounds__(64, 3) void adj_cell_mfma_kernel(AdjArgs a, Phys ph
   const int lane = threadIdx.x;
   for (int i = lane; i < RD; i += 64) sMu[i] = 0.0;
   for (int i = lane; i < SD; i += 64) sGq[i] = 0.0;
-  const double *Drw = ops, *Dsw = ops + Np * Np;
-  // B operands, lane l: B[k = 4 ks + l/16][j = l%16]; the lane index goes
-  // through lane_in_loop so that they are loaded where used
-  auto load_b = [&](double *b, const double *Mx, int ld, int ncol) {
-    const int l = lane_in_loop(lane), j = l & 15, k0 = l >> 4;
-#pragma unroll
-    for (int ks = 0; ks < KP; ks++) {
-      const int k = 4 * ks + k0;
-      const double v = Mx[(k < Np ? k : 0) * ld + (j < ncol ? j : 0)];
-      b[ks] = (k < Np && j < ncol) ? v : 0.0;
-    }
-  };
-  // A operand of K-step ks of group g from the record tile (zero for the
-  // padding columns k >= Np); ln = lane_in_loop(lane)
-  auto opA = [&](int ln, int g, int ks) -> double {
-    const int r = ln & 15, k = 4 * ks + (ln >> 4);
-    return (4 * ks + 3 < Np || k < Np) ? sMu[((4 * g + (r & 3)) * Np + k) * 4 + (r >> 2)] : 0.0;
-  };
+  const double *Drw = ops, *Dsw = ops + Np * Np, *LIFT = ops + 2 * Np * Np;
   const int ntiles = (a.ncells + TE - 1) / TE;
   for (int it_t = 0;; it_t++) {
     const int tile = xcd_tile(it_t, ntiles);
@@ -603,8 +647,9 @@ __global__ __launch_bounds__(64, 3) void adj_cell_mfma_kernel(AdjArgs a, Phys ph
     const int e0 = tile * TE, ne = min(TE, a.ncells - e0);
     __syncthreads();  // the previous tile is done with the LDS tiles
     // ---- the tile's loads, one round trip: mu, gmq and the factors into
-    // LDS; the lane's states, the epilogue's x0 (and y0) chunks and the
-    // running metric cotangents into registers
+    // LDS; the lane's states and the running metric cotangents into
+    // registers (the epilogue's x0 and y0 chunks load after the products,
+    // which would otherwise spill)
     copy_to_lds<RCH>(a.mu + (size_t)e0 * Np * 4, sMu, ne * Np * 2, lane);
     copy_to_lds<SCH>(a.gmq + (size_t)e0 * NF * 4, sGq, ne * NF * 2, lane);
     copy_to_lds<TE * 2>(a.cmet + (size_t)e0 * 4, sMet, ne * 2, lane);
@@ -617,17 +662,9 @@ __global__ __launch_bounds__(64, 3) void adj_cell_mfma_kernel(AdjArgs a, Phys ph
       wr[g][0] = ws[0]; wr[g][1] = ws[1];
     }
     double bR[KP], bS[KP];
-    load_b(bR, Drw, Np, Np);
-    load_b(bS, Dsw, Np, Np);
-    // the write-back's x0 (and y0) chunks and the running metric cotangents
-    double2 xc[WB], yc[WB];
+    adj_load_b<N>(bR, Drw, Np, Np, 1.0, ln);
+    adj_load_b<N>(bS, Dsw, Np, Np, 1.0, ln);
     const int nch = ne * Np * 2;
-#pragma unroll
-    for (int q = 0; q < WB; q++) {
-      const size_t ch = (size_t)e0 * Np * 2 + min(ln + 64 * q, nch - 1);
-      xc[q] = reinterpret_cast<const double2 *>(a.x0)[ch];
-      if (OUT2) yc[q] = reinterpret_cast<const double2 *>(a.y0)[ch];
-    }
     const size_t kc = (size_t)e0 + min(ln, ne - 1);
     const double crx = a.g_rx[kc], csx = a.g_sx[kc], cry = a.g_ry[kc], csy = a.g_sy[kc];
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the LDS tiles are in place
@@ -647,7 +684,7 @@ __global__ __launch_bounds__(64, 3) void adj_cell_mfma_kernel(AdjArgs a, Phys ph
       dbl4 nr = {0.0, 0.0, 0.0, 0.0}, ns = nr;
 #pragma unroll
       for (int ks = 0; ks < KP; ks++) {
-        const double av = opA(ln, g, ks);
+        const double av = adj_opA<N>(sMu, ln, g, ks);
         nr = mfma16x16x4(av, bR[ks], nr);
         ns = mfma16x16x4(av, bS[ks], ns);
       }
@@ -693,31 +730,62 @@ __global__ __launch_bounds__(64, 3) void adj_cell_mfma_kernel(AdjArgs a, Phys ph
       if (jj == 0) {
         sCell[4 * e] = grx; sCell[4 * e + 1] = gsx; sCell[4 * e + 2] = gry; sCell[4 * e + 3] = gsy;
       }
-      // scale lam into the group's mu records (the products above have read them)
+      // scale lam into the cell's slot-cotangent rows (the node reads above
+      // were this group's last use of them; Np <= NF, so they hold a record)
       if (ok) {
-        double2 *d = reinterpret_cast<double2 *>(sMu + (e * Np + jj) * 4);
+        double2 *d = reinterpret_cast<double2 *>(sGq + (e * NF + jj) * 4);
         d[0] = make_double2(a.scale * lam[0], a.scale * lam[1]);
         d[1] = make_double2(a.scale * lam[2], a.scale * lam[3]);
       }
     }
+    double2 xc[WB], yc[WB];
+    if (XY) {
+#pragma unroll
+      for (int q = 0; q < WB; q++) {
+        const size_t ch = (size_t)e0 * Np * 2 + min(ln + 64 * q, nch - 1);
+        xc[q] = reinterpret_cast<const double2 *>(a.x0)[ch];
+        yc[q] = reinterpret_cast<const double2 *>(a.y0)[ch];
+      }
+    }
     __syncthreads();
-    // ---- write-back: out = e0 x0 + scale lam (and out2), coalesced
+    // ---- write-back: out = em mu + scale lam (+ e0 x0 + f0 y0), coalesced;
+    // out replaces mu in the record tile
 #pragma unroll
     for (int q = 0; q < WB; q++) {
       const int c = ln + 64 * q;
       if (c >= nch) continue;
-      const double2 l = reinterpret_cast<const double2 *>(sMu)[c];
-      const double2 o = make_double2(a.e0 * xc[q].x + l.x, a.e0 * xc[q].y + l.y);
+      const int r = c >> 1, e = r / Np, k = r - e * Np;
+      const double2 l = *reinterpret_cast<const double2 *>(sGq + (e * NF + k) * 4 + 2 * (c & 1));
+      double2 *mr = reinterpret_cast<double2 *>(sMu) + c;
+      const double2 mu2 = *mr;
+      double2 o = make_double2(a.em * mu2.x + l.x, a.em * mu2.y + l.y);
+      if (XY) {
+        o.x += a.e0 * xc[q].x + a.f0 * yc[q].x;
+        o.y += a.e0 * xc[q].y + a.f0 * yc[q].y;
+      }
       reinterpret_cast<double2 *>(a.out)[(size_t)e0 * Np * 2 + c] = o;
-      if (OUT2)
-        reinterpret_cast<double2 *>(a.out2)[(size_t)e0 * Np * 2 + c] =
-            make_double2(a.f0 * yc[q].x + a.f1 * xc[q].x + a.f2 * o.x, a.f0 * yc[q].y + a.f1 * xc[q].y + a.f2 * o.y);
+      *mr = o;
     }
     if (ln < ne) {
       a.g_rx[kc] = crx + a.scale * sCell[4 * ln];
       a.g_sx[kc] = csx + a.scale * sCell[4 * ln + 1];
       a.g_ry[kc] = cry + a.scale * sCell[4 * ln + 2];
       a.g_sy[kc] = csy + a.scale * sCell[4 * ln + 3];
+    }
+    __syncthreads();
+    // ---- psi_out = -LIFT^T out of the tile's cells
+    double bP[KP];
+    adj_load_b<N>(bP, LIFT, NF, NF, -1.0, ln);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (4 * g >= ne) continue;
+      const dbl4 p = adj_psi_group<N>(sMu, ln, g, bP);
+      const int e = 4 * g + ce;
+      if (e < ne && jj < NF) {
+        double2 *d = reinterpret_cast<double2 *>(a.psi_out + ((size_t)(e0 + e) * NF + jj) * 4);
+        d[0] = make_double2(p[0], p[1]);
+        d[1] = make_double2(p[2], p[3]);
+      }
     }
   }
 }
@@ -1009,22 +1077,35 @@ static hipError_t adj_n(const Launch &L, const AdjArgs &a, const int *node_slots
   hipError_t e = hipSuccess;
   if (a.pairs) {
     // face-pair form: the face-pair pass (boundary blocks first, so that
-    // their longer threads start early; psi from mu), then the cell pass on
-    // the matrix cores
-    if (!a.cmet || !a.x0 || (a.out2 && !a.y0)) return hipErrorInvalidValue;
+    // their longer threads start early; psi read from a.psi), then the cell
+    // pass on the matrix cores, which writes psi_out
+    if (!a.cmet || !a.psi || !a.psi_out || a.out2 || (a.x0 && !a.y0)) return hipErrorInvalidValue;
     const int nbblk = (a.nbedges * Ord<N>::Nfp + kAdjThreads - 1) / kAdjThreads;
     const int fpb = kAdjThreads / Ord<N>::Nfp;
-    const int npb = (a.npairs + fpb - 1) / fpb;
+    int npb = (a.npairs + fpb - 1) / fpb;
+    // The pair workgroups (a grid-stride walk over the pairs) are capped at
+    // 2/3 of the pass's resident capacity: a grid larger than the device
+    // holds keeps workgroups pending, and those take every slot that frees
+    // up, so the forward recomputation launched beside the sweep on the
+    // second stream hardly runs until the pass drains.  Measured on the
+    // 65,536-cell O-grid: reverse sweep 86.8 ms uncapped, 84.3 ms at 2/3,
+    // 84.9 ms at the full capacity.
+    static int pair_per_cu = -1;
+    if (pair_per_cu < 0 &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pair_per_cu, adj_pair_kernel<N>, kAdjThreads, 0) != hipSuccess ||
+         pair_per_cu < 1))
+      pair_per_cu = 1;
+    if (L.cus > 0) npb = std::min(npb, std::max(1, 2 * L.cus * pair_per_cu / 3));
     if (nbblk + npb > 0)
       hipLaunchKernelGGL(adj_pair_kernel<N>, dim3(nbblk + npb), dim3(kAdjThreads), 0, L.stream, a, L.ph, L.fmask,
-                         L.ops, nbblk);
+                         nbblk);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // one-wave workgroups, one 16-cell tile each (the XCD-chunked walk gives
     // each XCD a contiguous range of tiles)
     int grid = (a.ncells + kMfmaTile - 1) / kMfmaTile;
     if (L.max_blocks > 0 && grid > L.max_blocks) grid = L.max_blocks;
     if (grid >= 8) grid += (8 - grid % 8) % 8;
-    if (a.out2)
+    if (a.x0)
       hipLaunchKernelGGL((adj_cell_mfma_kernel<N, true>), dim3(grid), dim3(64), 0, L.stream, a, L.ph, L.ops,
                          (unsigned long long)L.fmask4);
     else
@@ -1066,6 +1147,20 @@ hipError_t launch_rhs_adj(const Launch &L, const AdjArgs &a, const int *node_slo
     case 4: return adj_n<4>(L, a, node_slots);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj_psi(const Launch &L, int ncells, const double *mu, double *psi) {
+  if (ncells == 0) return hipSuccess;
+  int grid = (ncells + kMfmaTile - 1) / kMfmaTile;
+  if (grid >= 8) grid += (8 - grid % 8) % 8;
+  switch (L.N) {
+    case 1: hipLaunchKernelGGL(adj_psi_mfma_kernel<1>, dim3(grid), dim3(64), 0, L.stream, ncells, mu, L.ops, psi); break;
+    case 2: hipLaunchKernelGGL(adj_psi_mfma_kernel<2>, dim3(grid), dim3(64), 0, L.stream, ncells, mu, L.ops, psi); break;
+    case 3: hipLaunchKernelGGL(adj_psi_mfma_kernel<3>, dim3(grid), dim3(64), 0, L.stream, ncells, mu, L.ops, psi); break;
+    case 4: hipLaunchKernelGGL(adj_psi_mfma_kernel<4>, dim3(grid), dim3(64), 0, L.stream, ncells, mu, L.ops, psi); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_lincomb(const Launch &L, size_t n, double a, const double *x, double b,

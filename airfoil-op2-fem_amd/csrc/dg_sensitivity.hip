@@ -32,6 +32,9 @@
 // one shape to the next.)
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <cstdlib>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -265,11 +268,23 @@ hipError_t stages01(dg_sens *s, const double *q, int n) {
   return step3(s, q, s->wA, s->wB, nullptr, n, false);
 }
 
-double *seg_q(dg_sens *s, int j) { return s->seg[3 * j]; }
-double *seg_w1(dg_sens *s, int j) { return s->seg[3 * j + 1]; }
-double *seg_w2(dg_sens *s, int j) { return s->seg[3 * j + 2]; }
 // entry k = 3 j + {0, 1, 2} of segment store b
-double *seg_at(dg_sens *s, int b, int k) { return b ? s->seg2[k] : s->seg[k]; }
+double *seg_at(dg_sens *s, int b, int k) {
+  const std::vector<double *> &v = b ? s->seg2 : s->seg;
+  if (k < 0 || (size_t)k >= v.size()) {
+    std::fprintf(stderr, "dg_sens: segment store %d has no slot %d\n", b, k);
+    std::abort();
+  }
+  return v[k];
+}
+
+// Segments a checkpointed forward run leaves in the segment stores: the
+// last one, and the one before it when there is one and the second store is
+// allocated.
+int kept_segments(const dg_sens *s, int ckpt_every) {
+  const int nck = (s->nsteps + ckpt_every - 1) / ckpt_every;
+  return (nck > 1 && (int)s->seg2.size() >= 3 * ckpt_every) ? 2 : 1;
+}
 
 int set_ic(dg_sens *s, double *q) {
   HIPCHECK(s->ctx, dg::launch_set_ic(launch_a(s), s->K, q, s->wA));
@@ -311,20 +326,25 @@ int run_forward(dg_sens *s, bool record, int ckpt_every, double *cl) {
     HIPCHECK(c, hipMemsetAsync(s->dt1, 0, 2 * sizeof(unsigned long long), s->stream));
     HIPCHECK(c, dg::launch_calc_dt(launch_a(s), s->K, s->q, s->fs, s->dt1));
   }
-  // With checkpoints, the steps of the last segment keep q, w1, w2 in the
-  // segment store, so the reverse sweep need not recompute that segment.
+  // With checkpoints, the steps of the last segment (and of the one before
+  // it, when the second segment store exists) keep q, w1, w2 in the segment
+  // stores where the reverse sweep reads them (segment sg in store
+  // (nck - 1 - sg) & 1), so the sweep need not recompute those segments.
   const size_t bytes = sizeof(double) * (size_t)s->K * s->Np * 4;
-  const int last0 = ckpt_every > 0 ? ((s->nsteps - 1) / ckpt_every) * ckpt_every : s->nsteps;
+  const bool stores = ckpt_every > 0;
+  const int nck = stores ? (s->nsteps + ckpt_every - 1) / ckpt_every : 0;
+  const int kept0 = stores ? nck - kept_segments(s, ckpt_every) : 0;  // first kept segment
   for (int n = 0; n < s->nsteps; n++) {
-    if (ckpt_every > 0 && n % ckpt_every == 0)
+    if (stores && n % ckpt_every == 0)
       HIPCHECK(c, hipMemcpyAsync(s->ckpt[n / ckpt_every], s->q, bytes, hipMemcpyDeviceToDevice, s->stream));
-    if (n < last0) {
+    const int sg = stores ? n / ckpt_every : 0;
+    if (!stores || sg < kept0) {
       HIPCHECK(c, step(s, s->q, n, record));
     } else {
-      const int j = n - last0;
-      if (j == 0) HIPCHECK(c, hipMemcpyAsync(seg_q(s, 0), s->q, bytes, hipMemcpyDeviceToDevice, s->stream));
-      double *qout = n + 1 < s->nsteps ? seg_q(s, j + 1) : s->q;
-      HIPCHECK(c, step3(s, seg_q(s, j), seg_w1(s, j), seg_w2(s, j), qout, n, record));
+      const int b = (nck - 1 - sg) & 1, j = n - sg * ckpt_every;
+      if (j == 0) HIPCHECK(c, hipMemcpyAsync(seg_at(s, b, 0), s->q, bytes, hipMemcpyDeviceToDevice, s->stream));
+      double *qout = (j + 1 < ckpt_every && n + 1 < s->nsteps) ? seg_at(s, b, 3 * j + 3) : s->q;
+      HIPCHECK(c, step3(s, seg_at(s, b, 3 * j), seg_at(s, b, 3 * j + 1), seg_at(s, b, 3 * j + 2), qout, n, record));
     }
   }
   double fx, fy;
@@ -859,7 +879,11 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   };
   HIPCHECK(c, hipEventRecord(s->ev_free[0], s->stream));
   HIPCHECK(c, hipEventRecord(s->ev_free[1], s->stream));
-  if (nck > 1) HIPCHECK(c, recompute(nck - 2));
+  // the segments the forward run left in the stores are ready
+  const int kept = kept_segments(s, ckpt_every);
+  HIPCHECK(c, hipEventRecord(s->ev_ready[0], s->stream));
+  HIPCHECK(c, hipEventRecord(s->ev_ready[1], s->stream));
+  if (nck > 1 && kept < 2) HIPCHECK(c, recompute(nck - 2));
   dg::AdjArgs a{};
   a.ncells = s->K;
   a.rx = s->rx; a.ry = s->ry; a.sx = s->sx; a.sy = s->sy; a.nx = s->nx; a.ny = s->ny; a.fscale = s->fs;
@@ -869,10 +893,20 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
   a.g_nxf = s->g_nxf; a.g_nyf = s->g_nyf; a.g_fsf = s->g_fsf;
   a.compact = 1;
   a.nbedges = s->mesh.nbedges; a.b2c = s->d_b2c; a.bnum = s->d_bnum;
-  // face-pair form: the face-pair pass forms psi from mu, the cell pass
-  // runs on the matrix cores with the per-cell factors
+  // face-pair form: the face-pair pass reads psi = -LIFT^T mu from one of
+  // two buffers, the cell pass (matrix cores, per-cell factors) writes psi
+  // of its out into the other; the first psi comes from launch_adj_psi
   a.pairs = s->pairs; a.npairs = s->npairs; a.cmet = s->cmet;
-  auto rhs_adj = [&]() -> hipError_t { return dg::launch_rhs_adj(L, a, s->node_slots); };
+  double *psi[2] = {s->gpq, s->fluxd};
+  int pc = 0;
+  HIPCHECK(c, dg::launch_adj_psi(L, (int)K, s->lam, psi[0]));
+  auto rhs_adj = [&](const double *w, const double *mu, double em, double scale, const double *x0, double e0,
+                     const double *y0, double f0, double *out) -> hipError_t {
+    a.w = w; a.mu = mu; a.em = em; a.scale = scale; a.x0 = x0; a.e0 = e0; a.y0 = y0; a.f0 = f0; a.out = out;
+    a.psi = psi[pc]; a.psi_out = psi[pc ^ 1];
+    pc ^= 1;
+    return dg::launch_rhs_adj(L, a, s->node_slots);
+  };
   for (int sg = nck - 1; sg >= 0; sg--) {
     const int n0 = sg * ckpt_every, n1 = std::min(s->nsteps, n0 + ckpt_every);
     const int b = (nck - 1 - sg) & 1;
@@ -880,16 +914,9 @@ int dg_sens_gradient(dg_sens *s, int ckpt_every, double *cl, double *dcl_dalpha,
     for (int n = n1 - 1; n >= n0; n--) {
       const double dt = s->h_dts[n];
       const int j = n - n0;
-      a.w = seg_at(s, b, 3 * j + 2); a.mu = s->lam;
-      a.x0 = s->lam; a.e0 = 2.0 / 3.0; a.scale = 2.0 / 3.0 * dt; a.out = s->A; a.out2 = nullptr;
-      HIPCHECK(c, rhs_adj());
-      a.w = seg_at(s, b, 3 * j + 1); a.mu = s->A;
-      a.x0 = s->A; a.e0 = 0.25; a.scale = 0.25 * dt; a.out = s->B;
-      a.out2 = s->lam; a.y0 = s->lam; a.f0 = 1.0 / 3.0; a.f1 = 0.75; a.f2 = 1.0;
-      HIPCHECK(c, rhs_adj());
-      a.w = seg_at(s, b, 3 * j); a.mu = s->B;
-      a.x0 = s->lam; a.e0 = 1.0; a.scale = dt; a.out = s->lam; a.out2 = nullptr;
-      HIPCHECK(c, rhs_adj());
+      HIPCHECK(c, rhs_adj(seg_at(s, b, 3 * j + 2), s->lam, 2.0 / 3.0, 2.0 / 3.0 * dt, nullptr, 0.0, nullptr, 0.0, s->A));
+      HIPCHECK(c, rhs_adj(seg_at(s, b, 3 * j + 1), s->A, 0.25, 0.25 * dt, nullptr, 0.0, nullptr, 0.0, s->B));
+      HIPCHECK(c, rhs_adj(seg_at(s, b, 3 * j), s->B, 1.0, dt, s->A, 0.75, s->lam, 1.0 / 3.0, s->lam));
     }
     HIPCHECK(c, hipEventRecord(s->ev_free[b], s->stream));
     if (sg >= 2) HIPCHECK(c, recompute(sg - 2));

@@ -218,7 +218,7 @@ def adjoint_leg(order=3, steps=400, ckpt=20, checks=True):
 
 
 def adjoint_bench(args):
-    out = adjoint_leg(args.order, args.adj_steps, 20, checks=True)
+    out = adjoint_leg(args.order, args.adj_steps, 20, checks=not args.no_checks)
     print(json.dumps(dict({"metric": "adjoint gradient cost / forward cost", "higher_is_better": False,
                            "dtype": "f64", "config": {"workload": out["workload"]}}, **out)))
 
@@ -377,6 +377,8 @@ def main():
         ap.add_argument("--adjoint", action="store_true")
         ap.add_argument("--adj-steps", type=int, default=400)
         ap.add_argument("--order", type=int, default=3)
+        ap.add_argument("--no-checks", action="store_true",
+                        help="timings only (no tangent-linear or finite-difference runs)")
         return adjoint_bench(ap.parse_args())
     if len(sys.argv) > 2 and sys.argv[1] == "--summarize-trace":
         return summarize_trace(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 0)
@@ -396,9 +398,14 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-case batched-sweep figure")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the lift-gradient figure")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch", default=None, metavar="WPB,WALK",
+                    help="fixed stage-kernel launch shape instead of the autotuner's pick (waves per "
+                         "workgroup 1/2/4, walk 0 XCD-chunked / 1 round-robin), e.g. for counter runs")
     ap.add_argument("--geometry", choices=["node", "cell"], default="node",
                     help="geometry the stage kernels read: per-node factors (the reference's "
                          "op_dats) or one metric record per cell")
+    ap.add_argument("--stream", type=float, default=0.0, metavar="GIB",
+                    help="only measure device-copy and read-only bandwidth on GIB GiB buffers")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run plain device copies of GIB GiB (counter calibration)")
     args = ap.parse_args()
@@ -413,6 +420,25 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+    if args.stream > 0:
+        # practical HBM ceilings for comparison with the 8 TB/s peak: a device
+        # copy (half reads, half writes) and a read-only reduction of GIB GiB
+        n = int(args.stream * (1 << 30)) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        out = {"stream_bytes": 8 * n}
+        for name, fn, moved in (("copy", lambda: dst.copy_(src), 16 * n), ("read_sum", lambda: src.sum(), 8 * n)):
+            for _ in range(3):
+                fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            out[name + "_TBps"] = moved * 20 / (e0.elapsed_time(e1) * 1e-3) / 1e12
+        print(json.dumps(out))
+        return
     if args.calibrate_copy > 0:
         n = int(args.calibrate_copy * (1 << 30)) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -440,7 +466,10 @@ def main():
           file=sys.stderr, flush=True)
 
     tune = None
-    if not args.no_autotune:
+    if args.launch:
+        wpb, walk = (int(x) for x in args.launch.split(","))
+        solver.set_launch(wpb, walk, 0)
+    elif not args.no_autotune:
         table = solver.autotune(trial_steps=3, cfl=args.cfl)
         tune = dict(solver.launch_config(), table_ms_per_step={
             f"wpb{r['wpb']}_walk{r['walk']}_w{r['waves_per_cu']}": round(r["ms_per_step"], 4) for r in table})

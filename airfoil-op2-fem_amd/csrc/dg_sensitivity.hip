@@ -16,7 +16,8 @@
 //   M = 1/4 A;    B = M + dt L'(w1)^T M;  lam += B + dt L'(q)^T B,
 // with the geometry and free-stream cotangents accumulated alongside.  The
 // states q, w1, w2 of each step are recomputed from checkpoints stored every
-// ckpt steps (one level of revolve-style recomputation).
+// ckpt steps (one level of revolve-style recomputation); the forward run
+// leaves the last two segments' states in the two segment stores.
 //
 // Geometry chain: bump amplitudes -> surface offsets -> node displacement
 // (linear, the radial blend of gen_ogrid_shaped) -> cell edge vectors ->
@@ -151,9 +152,10 @@ struct dg_sens {
   double fwd_ms = 0.0, grad_ms = 0.0, tlm_ms = 0.0;
   double grad_fwd_ms = 0.0, grad_rev_ms = 0.0, grad_host_ms = 0.0;
   // checkpoints (state every ckpt steps) and two segment stores: q, w1, w2 of
-  // each step of one segment, [3 * j + {0, 1, 2}].  The reverse sweep reads
-  // one store while the second stream recomputes the next segment into the
-  // other; ready[b]: store b holds its segment, free[b]: its reverse is done.
+  // each step of one segment, [3 * j + {0, 1, 2}]; segment sg lives in store
+  // (nck - 1 - sg) & 1.  The reverse sweep reads one store while the second
+  // stream recomputes the next segment into the other; ready[b]: store b
+  // holds its segment, free[b]: its reverse is done.
   std::vector<double *> ckpt, seg, seg2;
   // launch shape of the stage kernels on this mesh, picked at the first
   // recording (see tune_shape)

@@ -32,9 +32,10 @@
 // per SIMD: the next tile's face-map codes are loaded one tile ahead (during
 // the volume products), a Roe item's own trace, exterior trace and geometry
 // go out together (a boundary slot reads its own record as a stand-in
-// exterior), and each group's first-touch epilogue loads go out before its
-// products.  Running the surface term first keeps the accumulators out of
-// the Roe phase, the phase with the most live values.
+// exterior), in stages 1 and 2 one item ahead of its flux, and each group's
+// epilogue loads (metric factors, w, q) go out before its products.
+// Running the surface term first keeps the accumulators out of the Roe
+// phase, the phase with the most live values.
 //
 // Compact geometry (CG): on straight-sided cells the metric factors are the
 // same at every node, so the kernel reads one (rx, ry, sx, sy) record per

@@ -71,6 +71,7 @@ struct StageArgs {
   // launch, no dt.
   int ncases;
   const int *cell_off, *tile_off;
+  const int *tile_case;   // case of each tile (tile_off[ncases] entries), or null: binary search of tile_off
   const Phys *phys;
   // Frozen time step (per case): when non-null the stage updates use
   // dt_fixed[case] instead of cfl / dt1 (sensitivity runs replay a recorded

@@ -36,7 +36,7 @@ struct dg_solver {
   std::vector<int> cell_off, tile_off, wall_off;
   std::vector<dg::Phys> phys;
   std::vector<double> alpha;
-  int *d_cell_off = nullptr, *d_tile_off = nullptr;
+  int *d_cell_off = nullptr, *d_tile_off = nullptr, *d_tile_case = nullptr;
   dg::Phys *d_phys = nullptr;
   // concatenated connectivity (global cell indices)
   int *d_fmap = nullptr, *d_b2c = nullptr, *d_bnum = nullptr, *d_btype = nullptr;
@@ -171,6 +171,7 @@ dg::StageArgs stage_args(dg_solver *s, double cfl) {
   a.ncases = s->ncases;
   a.cell_off = s->d_cell_off;
   a.tile_off = s->d_tile_off;
+  a.tile_case = s->d_tile_case;
   a.phys = s->d_phys;
   a.ntiles_hint = s->tile_off.back();
   return a;
@@ -217,7 +218,7 @@ void dg_solver_destroy(dg_solver *s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   drop_graph(s);
   for (auto &kv : s->dats) (void)hipFree(kv.second);
-  void *ptrs[] = {s->dt1, s->t_dev, s->flag, s->d_cell_off, s->d_tile_off, s->d_phys, s->d_fmap,
+  void *ptrs[] = {s->dt1, s->t_dev, s->flag, s->d_cell_off, s->d_tile_off, s->d_tile_case, s->d_phys, s->d_fmap,
                   s->d_b2c, s->d_bnum, s->d_btype, s->d_wcell, s->d_wface, s->d_facew,
                   s->d_wforce, s->d_wcp};
   for (void *p : ptrs) (void)hipFree(p);
@@ -322,6 +323,13 @@ int dg_solver_create_batch(dg_ctx *c, int ncases, const dg_mesh *const *meshes, 
   if (e == hipSuccess) e = hipMemset(s->flag, 0, sizeof(unsigned int));
   if (e == hipSuccess) e = upload(&s->d_cell_off, s->cell_off);
   if (e == hipSuccess) e = upload(&s->d_tile_off, s->tile_off);
+  if (e == hipSuccess) {
+    // each tile's case, so that a stage kernel's tile finds its case with one load
+    std::vector<int> tile_case(s->tile_off.back());
+    for (int k = 0; k < ncases; k++)
+      for (int t = s->tile_off[k]; t < s->tile_off[k + 1]; t++) tile_case[t] = k;
+    e = upload(&s->d_tile_case, tile_case);
+  }
   if (e == hipSuccess) e = upload(&s->d_phys, s->phys);
   if (e == hipSuccess) e = upload(&s->d_fmap, fmap_all);
   if (e == hipSuccess) e = upload(&s->d_b2c, b2c);

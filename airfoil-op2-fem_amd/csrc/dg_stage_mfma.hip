@@ -53,9 +53,8 @@
 namespace dg {
 
 // LDS per wave (doubles): the F and G tiles [4 groups][Np columns][16 rows]
-// (the face-flux tile [4][4 KF columns][16 rows] reuses them); the stage-2
-// kernel also keeps the fscale values its CFL quantity reads, [16 cells][Np].
-// A column holds its 16 rows in a slot of kColStride doubles and an F or G
+// (the face-flux tile [4][4 KF columns][16 rows] and the copy of the tile's
+// state records reuse them).  A column holds its 16 rows in a slot of kColStride doubles and an F or G
 // group starts kGroupSkew doubles past the end of the previous one.  The
 // flux writes put the nodes of one cell (same row, consecutive columns) on
 // neighbouring lanes; with 16-double columns those all fall in one bank pair
@@ -66,8 +65,8 @@ namespace dg {
 // 17-double slots and no skew (consecutive columns one bank pair apart, at
 // most 2-way conflicts between neighbouring cells), so that it and the copy
 // of the tile's state records fit beside each other within the F and G
-// tiles' footprint (at N=3 a stage-2 wave needs 12,928 bytes, twelve waves
-// per compute unit 152 KiB of its 160).
+// tiles' footprint (at N=3 a wave needs 11,648 bytes: twelve one-wave
+// workgroups per compute unit, the register limit, take 137 KiB of its 160).
 constexpr int kColStride = 18, kGroupSkew = 2, kFaceColStride = 17;
 template <int N>
 constexpr int mfma_fg_group() {  // doubles per group in the F and G tiles
@@ -123,7 +122,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   constexpr int GS = mfma_fg_group<N>();    // doubles per group in the F and G tiles
   constexpr int GL = mfma_fl_group<N>();    // doubles per group in the face-flux tile
   constexpr int CS = kColStride, CSL = kFaceColStride;
-constexpr bool kItemsInLoop = EPI == EPI_STAGE1 || EPI == EPI_STAGE2 || N >= 4;  // see lane_in_loop
+  constexpr bool kItemsInLoop = EPI == EPI_STAGE1 || EPI == EPI_STAGE2 || N >= 4;  // see lane_in_loop
   // Roe items one ahead: item j + 1's loads go out before item j's flux.
   // Measured on the 724x724 vortex: stages 1 and 2 and the per-cell geometry
   // gain (per-cell step 1.10 -> 1.03 ms); the per-node stage-0 kernel ran
@@ -195,9 +194,13 @@ constexpr bool kItemsInLoop = EPI == EPI_STAGE1 || EPI == EPI_STAGE2 || N >= 4; 
     if (tile >= ntiles) return 2;
     if (multi) {
       int lo = 0, hi = a.ncases - 1;  // case owning this tile
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
+      if (a.tile_case) {
+        lo = a.tile_case[tile];
+      } else {
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (a.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
+        }
       }
       cs_ = lo;
       e0_ = a.cell_off[cs_] + (tile - a.tile_off[cs_]) * TE;

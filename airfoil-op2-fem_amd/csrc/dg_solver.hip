@@ -171,7 +171,7 @@ dg::StageArgs stage_args(dg_solver *s, double cfl) {
   a.ncases = s->ncases;
   a.cell_off = s->d_cell_off;
   a.tile_off = s->d_tile_off;
-  a.tile_case = s->d_tile_case;
+  a.tile_case = s->ncases > 1 ? s->d_tile_case : nullptr;  // one case: no lookup at all
   a.phys = s->d_phys;
   a.ntiles_hint = s->tile_off.back();
   return a;

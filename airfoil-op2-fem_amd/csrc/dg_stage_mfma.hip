@@ -32,7 +32,7 @@
 // per SIMD: the next tile's face-map codes are loaded one tile ahead (during
 // the volume products), a Roe item's own trace, exterior trace and geometry
 // go out together (a boundary slot reads its own record as a stand-in
-// exterior), in stages 1 and 2 one item ahead of its flux, and each group's
+// exterior), one item ahead of its flux, and each group's
 // epilogue loads (metric factors, w, q) go out before its products.
 // Running the surface term first keeps the accumulators out of the Roe
 // phase, the phase with the most live values.
@@ -123,12 +123,6 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   constexpr int GL = mfma_fl_group<N>();    // doubles per group in the face-flux tile
   constexpr int CS = kColStride, CSL = kFaceColStride;
   constexpr bool kItemsInLoop = EPI == EPI_STAGE1 || EPI == EPI_STAGE2 || N >= 4;  // see lane_in_loop
-  // Roe items one ahead: item j + 1's loads go out before item j's flux.
-  // Measured on the 724x724 vortex: stages 1 and 2 and the per-cell geometry
-  // gain (per-cell step 1.10 -> 1.03 ms); the per-node stage-0 kernel ran
-  // ~1.5% slower with it and loads each item just before its flux.  The
-  // drop-in euler_rhs kernel (GATHER false) loads like the fused RHS kernel.
-  constexpr bool kRoeAhead = CG || EPI != EPI_STAGE0;
   static_assert(Np <= 16, "one 16-row output tile per cell group");
   // per wave: F tiles [G][Np][CS] then G tiles [G][Np][CS] (column k of a
   // group holds its rows cell + 4*var; the padding columns k >= Np of the
@@ -289,12 +283,14 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         r.m01 = ms[0]; r.m23 = ms[1];
         r.p01 = ps[0]; r.p23 = ps[1];
       };
+      // one item ahead: item j + 1's loads go out before item j's flux
+      // (724x724 vortex: stage 0 373 -> 367 us, stages 1 and 2 1-3%, the
+      // per-cell geometry step 1.10 -> 1.03 ms)
       RoeIn cur, nxt;
-      if (kRoeAhead) load_item(0, cur);
+      load_item(0, cur);
 #pragma unroll
       for (int j = 0; j < NRT; j++) {
-        if (!kRoeAhead) load_item(j, cur);
-        else if (j + 1 < NRT) load_item(j + 1, nxt);
+        if (j + 1 < NRT) load_item(j + 1, nxt);
         const int it = lr + 64 * j;
         if (it < nrt) {
           const int e = it / NF, m = it - e * NF;
@@ -319,7 +315,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
 #pragma unroll
           for (int v = 0; v < 4; v++) sL[off + 4 * v] = fl[v];
         }
-        if (kRoeAhead && j + 1 < NRT) cur = nxt;
+        if (j + 1 < NRT) cur = nxt;
       }
       if (NF < 4 * KF) {  // zero padding columns (orders whose 3(N+1) is not a multiple of 4)
         constexpr int PADF = 4 * KF - NF;

@@ -293,6 +293,10 @@ __global__ __launch_bounds__(256, 2) void tlm_cell_kernel(TlmArgs a, Phys ph, co
 }
 
 constexpr int kAdjThreads = 256;
+// Threads per workgroup of the face-pair pass: one wave, so that its
+// per-iteration barriers hold one wave rather than four in lockstep (the
+// reverse sweep on the 65,536-cell O-grid: 84.1 -> 82.6 ms against 256).
+constexpr int kPairThreads = 64;
 
 // One thread per face slot, whole cells per workgroup; the cells' mu and the
 // LIFT matrix are staged in LDS for psi = -LIFT^T mu.
@@ -439,17 +443,17 @@ __global__ __launch_bounds__(256) void adj_boundary_kernel(AdjArgs a, Phys ph, c
 // psi, the cotangent of the scaled fluxes (-LIFT^T mu), is read from a.psi,
 // which the cell pass before (or launch_adj_psi) wrote on the matrix cores.
 template <int N>
-__global__ __launch_bounds__(kAdjThreads, 3) void adj_pair_kernel(AdjArgs a, Phys ph,
+__global__ __launch_bounds__(kPairThreads, 3) void adj_pair_kernel(AdjArgs a, Phys ph,
                                                                 const int *__restrict__ fmask, int nbblk) {
   constexpr int Np = Ord<N>::Np, Nfp = Ord<N>::Nfp, NF = Ord<N>::NF;
-  constexpr int FPB = kAdjThreads / Nfp;  // faces per workgroup visit
-  __shared__ double sG[4][kAdjThreads];
+  constexpr int FPB = kPairThreads / Nfp;  // faces per workgroup visit
+  __shared__ double sG[4][kPairThreads];
   __shared__ int sFm[NF];
   const int t = threadIdx.x;
   if (t < NF) sFm[t] = fmask[t];
   __syncthreads();
   if ((int)blockIdx.x < nbblk) {
-    const int bi = (int)blockIdx.x * kAdjThreads + t;
+    const int bi = (int)blockIdx.x * kPairThreads + t;
     if (bi >= a.nbedges * Nfp) return;
     const int b = bi / Nfp, i = bi - b * Nfp;
     const size_t c = a.b2c[b];
@@ -1080,24 +1084,25 @@ static hipError_t adj_n(const Launch &L, const AdjArgs &a, const int *node_slots
     // their longer threads start early; psi read from a.psi), then the cell
     // pass on the matrix cores, which writes psi_out
     if (!a.cmet || !a.psi || !a.psi_out || a.out2 || (a.x0 && !a.y0)) return hipErrorInvalidValue;
-    const int nbblk = (a.nbedges * Ord<N>::Nfp + kAdjThreads - 1) / kAdjThreads;
-    const int fpb = kAdjThreads / Ord<N>::Nfp;
+    const int nbblk = (a.nbedges * Ord<N>::Nfp + kPairThreads - 1) / kPairThreads;
+    const int fpb = kPairThreads / Ord<N>::Nfp;
     int npb = (a.npairs + fpb - 1) / fpb;
     // The pair workgroups (a grid-stride walk over the pairs) are capped at
     // 2/3 of the pass's resident capacity: a grid larger than the device
     // holds keeps workgroups pending, and those take every slot that frees
     // up, so the forward recomputation launched beside the sweep on the
     // second stream hardly runs until the pass drains.  Measured on the
-    // 65,536-cell O-grid: reverse sweep 86.8 ms uncapped, 84.3 ms at 2/3,
-    // 84.9 ms at the full capacity.
+    // 65,536-cell O-grid (one-wave workgroups): reverse sweep 86.6 ms at 1/3
+    // of the capacity, 82.7 ms at 2/3, 83.6 ms at the full capacity (and
+    // 86.8 ms uncapped with four-wave workgroups).
     static int pair_per_cu = -1;
     if (pair_per_cu < 0 &&
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pair_per_cu, adj_pair_kernel<N>, kAdjThreads, 0) != hipSuccess ||
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pair_per_cu, adj_pair_kernel<N>, kPairThreads, 0) != hipSuccess ||
          pair_per_cu < 1))
       pair_per_cu = 1;
     if (L.cus > 0) npb = std::min(npb, std::max(1, 2 * L.cus * pair_per_cu / 3));
     if (nbblk + npb > 0)
-      hipLaunchKernelGGL(adj_pair_kernel<N>, dim3(nbblk + npb), dim3(kAdjThreads), 0, L.stream, a, L.ph, L.fmask,
+      hipLaunchKernelGGL(adj_pair_kernel<N>, dim3(nbblk + npb), dim3(kPairThreads), 0, L.stream, a, L.ph, L.fmask,
                          nbblk);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // one-wave workgroups, one 16-cell tile each (the XCD-chunked walk gives

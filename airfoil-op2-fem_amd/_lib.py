@@ -50,6 +50,8 @@ _SIGS = {
     "dg_get_bedge_q": ([c_void_p] + [c_void_p] * 4, c_int),
     "dg_get_bedge_q_exact": ([c_void_p, c_void_p, c_void_p, c_double, c_void_p], c_int),
     "dg_set_exact_state": ([c_void_p, c_double, c_double, c_double, c_double, c_double], c_int),
+    "dg_l2_error": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p], c_int),
+    "dg_l2_error_mass": ([c_void_p] + [c_void_p] * 7 + [c_double, c_void_p], c_int),
     "dg_mesh_rect": ([c_int, c_int, c_double, c_double, c_double, c_double, c_uint64, c_double, c_int,
                       P(c_void_p)], c_int),
     "dg_euler_rhs": ([c_void_p] + [c_void_p] * 10, c_int),

@@ -165,7 +165,7 @@ int dg_ctx_create(int order, dg_ctx **out) {
   dg_ctx *c = new dg_ctx();
   c->N = order; c->Np = ops.Np; c->Nfp = ops.Nfp; c->NF = 3 * ops.Nfp;
   c->r = ops.r; c->s = ops.s; c->Dr = ops.Dr; c->Ds = ops.Ds; c->Drw = ops.Drw;
-  c->Dsw = ops.Dsw; c->LIFT = ops.LIFT; c->FMASK = ops.FMASK;
+  c->Dsw = ops.Dsw; c->LIFT = ops.LIFT; c->FMASK = ops.FMASK; c->mass = ops.mass;
   c->ones.assign(c->Np, 1.0);
   hipError_t he = hipGetDevice(&c->device);
   if (he == hipSuccess) {
@@ -177,6 +177,9 @@ int dg_ctx_create(int order, dg_ctx **out) {
   if (he == hipSuccess) he = hipMalloc(&c->d_fmask, sizeof(int) * c->NF);
   if (he == hipSuccess) he = hipMalloc(&c->d_red, sizeof(unsigned long long) * 4);
   if (he == hipSuccess) he = hipMalloc(&c->d_node_slots, sizeof(int) * 2 * c->Np);
+  if (he == hipSuccess) he = hipMalloc(&c->d_l2, sizeof(double) * (c->Np * c->Np + 8 * dg::kL2Blocks));
+  if (he == hipSuccess)
+    he = hipMemcpy(c->d_l2, c->mass.data(), sizeof(double) * c->Np * c->Np, hipMemcpyHostToDevice);
   if (he == hipSuccess) he = scratch_lookup(c->device);
   if (he != hipSuccess) {
     std::string msg = std::string("dg_ctx_create: ") + hipGetErrorString(he);
@@ -201,7 +204,7 @@ void dg_ctx_destroy(dg_ctx *c) {
   if (!c) return;
   (void)hipFree(c->d_ops); (void)hipFree(c->d_fmask); (void)hipFree(c->d_red);
   (void)hipFree(c->d_fmap); (void)hipFree(c->d_b2c); (void)hipFree(c->d_bnum);
-  (void)hipFree(c->d_btype); (void)hipFree(c->d_node_slots);
+  (void)hipFree(c->d_btype); (void)hipFree(c->d_node_slots); (void)hipFree(c->d_l2);
   delete c;
 }
 
@@ -411,6 +414,55 @@ int dg_set_exact_state(dg_ctx *c, double uinf, double beta, double x0, double y0
   if (!(period_x >= 0.0)) return fail(c, DG_ERR_ARG, "dg_set_exact_state: period_x must be >= 0");
   c->vortex.uinf = uinf; c->vortex.beta = beta; c->vortex.x0 = x0; c->vortex.y0 = y0;
   c->vortex.period = period_x;
+  return DG_OK;
+}
+
+// Errors against the exact vortex: the squared nodal errors (optional) and
+// the workgroup partials, summed here in workgroup order.
+static int l2_errors(dg_ctx *c, const char *who, const double *x, const double *y, const double *xr,
+                     const double *yr, const double *xs, const double *ys, const double *q, double t,
+                     double *error, double sq[4], double wsum[4]) {
+  if (!c) return fail(nullptr, DG_ERR_ARG, std::string(who) + ": null context");
+  if (!mesh_declared(c)) return fail(c, DG_ERR_STATE, std::string(who) + ": no mesh declared");
+  if (c->ncells > 0 && (!x || !y || !q)) return fail(c, DG_ERR_ARG, std::string(who) + ": null array");
+  if (!al16(q) || (error && !al16(error)))
+    return fail(c, DG_ERR_ARG, std::string(who) + ": q and error must be 16-byte aligned");
+  const int g = dg::l2_error_blocks(c->Np, c->ncells);
+  for (int k = 0; k < 4; k++) sq[k] = wsum[k] = 0.0;
+  if (g == 0) return DG_OK;
+  double *partial = c->d_l2 + c->Np * c->Np;
+  HIPCHECK(c, dg::launch_l2_error(launch_of(c), c->ncells, c->vortex, t, x, y, q, xr, yr, xs, ys, c->d_l2,
+                                  error, partial));
+  std::vector<double> h(8 * (size_t)g);
+  HIPCHECK(c, hipMemcpyAsync(h.data(), partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (int b = 0; b < g; b++)
+    for (int k = 0; k < 4; k++) {
+      sq[k] += h[8 * b + k];
+      wsum[k] += h[8 * b + 4 + k];
+    }
+  return DG_OK;
+}
+
+int dg_l2_error(dg_ctx *c, const double *x, const double *y, const double *q, double t, double *error,
+                double err[4]) {
+  if (!err) return fail(c, DG_ERR_ARG, "dg_l2_error: null err");
+  double sq[4], ws[4];
+  int rc = l2_errors(c, "dg_l2_error", x, y, nullptr, nullptr, nullptr, nullptr, q, t, error, sq, ws);
+  if (rc != DG_OK) return rc;
+  for (int k = 0; k < 4; k++) err[k] += sq[k];
+  return DG_OK;
+}
+
+int dg_l2_error_mass(dg_ctx *c, const double *x, const double *y, const double *xr, const double *yr,
+                     const double *xs, const double *ys, const double *q, double t, double err[4]) {
+  if (!err) return fail(c, DG_ERR_ARG, "dg_l2_error_mass: null err");
+  if (c && c->ncells > 0 && (!xr || !yr || !xs || !ys))
+    return fail(c, DG_ERR_ARG, "dg_l2_error_mass: null metric array");
+  double sq[4], ws[4];
+  int rc = l2_errors(c, "dg_l2_error_mass", x, y, xr, yr, xs, ys, q, t, nullptr, sq, ws);
+  if (rc != DG_OK) return rc;
+  for (int k = 0; k < 4; k++) err[k] = std::sqrt(ws[k]);
   return DG_OK;
 }
 

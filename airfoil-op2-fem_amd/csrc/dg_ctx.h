@@ -15,7 +15,7 @@ struct dg_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   // host copies of the constants (op_decl_const names)
-  std::vector<double> r, s, Dr, Ds, Drw, Dsw, LIFT, ones;
+  std::vector<double> r, s, Dr, Ds, Drw, Dsw, LIFT, ones, mass;
   std::vector<int> FMASK;
   double gam = 1.4, bc_mach = 0.4, bc_alpha = 0.0, bc_p = 1.0, bc_r = 1.0, bc_u = 0.0,
          bc_v = 0.0, bc_e = 0.0;
@@ -27,6 +27,7 @@ struct dg_ctx {
   int ncells = -1, nedges = 0, nbedges = 0;
   int *d_fmap = nullptr, *d_b2c = nullptr, *d_bnum = nullptr, *d_btype = nullptr;
   unsigned long long *d_red = nullptr;
+  double *d_l2 = nullptr;  // [Np*Np] mass matrix, then [kL2Blocks][8] error partials
   int *d_node_slots = nullptr;  // [Np][2] face slots reading each node (adjoint)
   // exact isentropic-vortex state of boundary type 4 (vortex/src/get_bedge_q.h)
   dg::Vortex vortex;

@@ -102,6 +102,17 @@ hipError_t launch_exact_bc(const Launch &L, int nb, const int *b2c, const int *b
                            const double *t_dev, const unsigned long long *dt1, double cfl,
                            const double *dt_fixed, double frac, bool accumulate, double *ext);
 hipError_t launch_rhs_mfma(const Launch &L, const StageArgs &a, bool gather, int epi);
+// Errors against the exact vortex v at time t (vortex/src/l2_error.h:1-29):
+// error[i][k] = (q[i][k] - exact)^2 when error is set; partial[8 g + k] and
+// partial[8 g + 4 + k] (k < 4) get workgroup g's sum of those squares and,
+// when xr is set, of J_c e^T mass e over its cells (J_c = xr ys - xs yr at
+// the cell's node 0).  l2_error_blocks(Np, ncells) <= kL2Blocks workgroups.
+constexpr int kL2Blocks = 256;
+int l2_error_blocks(int Np, int ncells);
+hipError_t launch_l2_error(const Launch &L, int ncells, const Vortex &v, double t, const double *x,
+                           const double *y, const double *q, const double *xr, const double *yr,
+                           const double *xs, const double *ys, const double *mass, double *error,
+                           double *partial);
 hipError_t launch_get_neighbour_q(const Launch &L, int ncells, const int *fmap,
                                   const double *w, double *ext);
 hipError_t launch_get_bedge_q(const Launch &L, int nbedges, const int *b2c,

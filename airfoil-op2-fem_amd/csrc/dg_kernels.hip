@@ -249,6 +249,86 @@ __global__ void set_vortex_kernel(size_t nnodes, double gam, const double *__res
   }
 }
 
+// Cells per pass of l2_error_kernel: one thread per node, then one per
+// (cell, variable), within 256 threads.
+__host__ __device__ inline int l2_cells_per_pass(int Np) { return min(256 / Np, 64); }
+
+// Error against the exact vortex (vortex/src/l2_error.h:1-29).  Each
+// workgroup takes a contiguous run of cells, l2_cells_per_pass per pass: one
+// thread per node forms e = q - exact and writes e^2 to error (when set),
+// then one thread per (cell, variable) adds sum_i e_i^2 and, when the metric
+// factors are given, J e^T M e with J = xr ys - xs yr at node 0.  The
+// workgroup's two sums per variable go to partial[blockIdx][0..3] and
+// [4..7]; the grid depends on ncells only, so the summation order is fixed.
+__global__ __launch_bounds__(256) void l2_error_kernel(int ncells, int Np, double gam, Vortex vx, double t,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ y,
+                                                       const double *__restrict__ q,
+                                                       const double *__restrict__ xr,
+                                                       const double *__restrict__ yr,
+                                                       const double *__restrict__ xs,
+                                                       const double *__restrict__ ys,
+                                                       const double *__restrict__ mass, double *error,
+                                                       double *partial) {
+  __shared__ double sE[256 * 4];
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const int cb = l2_cells_per_pass(Np);
+  const int npass = (ncells + cb - 1) / cb;
+  const int per = (npass + gridDim.x - 1) / gridDim.x;
+  const int p0 = blockIdx.x * per, p1 = min(npass, p0 + per);
+  const bool weighted = xr != nullptr;
+  double acc_n = 0.0, acc_m = 0.0;  // thread (cell tid / 4, variable tid % 4)
+  for (int p = p0; p < p1; p++) {
+    const int c0 = p * cb, nc = min(cb, ncells - c0);
+    if (tid < nc * Np) {
+      const size_t i = (size_t)c0 * Np + tid;
+      double ex[4];
+      vortex_state(gam, vx, t, x[i], y[i], ex);
+      const double2 qa = *reinterpret_cast<const double2 *>(q + 4 * i);
+      const double2 qb = *reinterpret_cast<const double2 *>(q + 4 * i + 2);
+      const double e0 = qa.x - ex[0], e1 = qa.y - ex[1], e2 = qb.x - ex[2], e3 = qb.y - ex[3];
+      sE[4 * tid] = e0; sE[4 * tid + 1] = e1; sE[4 * tid + 2] = e2; sE[4 * tid + 3] = e3;
+      if (error) {
+        *reinterpret_cast<double2 *>(error + 4 * i) = make_double2(e0 * e0, e1 * e1);
+        *reinterpret_cast<double2 *>(error + 4 * i + 2) = make_double2(e2 * e2, e3 * e3);
+      }
+    }
+    __syncthreads();
+    if (tid < 4 * nc) {
+      const int cl = tid >> 2, v = tid & 3;
+      const double *e = sE + 4 * cl * Np + v;
+      double sn = 0.0, sm = 0.0;
+      for (int a = 0; a < Np; a++) {
+        const double ea = e[4 * a];
+        sn += ea * ea;
+        if (weighted) {
+          double row = 0.0;
+          for (int b = 0; b < Np; b++) row += mass[a * Np + b] * e[4 * b];
+          sm += ea * row;
+        }
+      }
+      acc_n += sn;
+      if (weighted) {
+        const size_t n0 = (size_t)(c0 + cl) * Np;
+        acc_m += (xr[n0] * ys[n0] - xs[n0] * yr[n0]) * sm;
+      }
+    }
+    __syncthreads();
+  }
+  // tree over the threads of one variable: every stride is a multiple of 4
+  for (int k = 0; k < 2; k++) {
+    red[tid] = k == 0 ? acc_n : acc_m;
+    __syncthreads();
+    for (int w = 128; w >= 4; w >>= 1) {
+      if (tid < w) red[tid] += red[tid + w];
+      __syncthreads();
+    }
+    if (tid < 4) partial[8 * blockIdx.x + 4 * k + tid] = red[tid];
+    __syncthreads();
+  }
+}
+
 __global__ void zero_kernel(size_t n, double *p) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -414,6 +494,23 @@ hipError_t launch_set_vortex(const Launch &L, int ncells, const double *x, const
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(set_vortex_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, L.stream, n,
                      L.ph.gam, x, y, uinf, beta, x0, y0, period_x, q, w);
+  return hipGetLastError();
+}
+
+int l2_error_blocks(int Np, int ncells) {
+  const int cb = l2_cells_per_pass(Np);
+  const int npass = (ncells + cb - 1) / cb;
+  return npass < kL2Blocks ? npass : kL2Blocks;
+}
+
+hipError_t launch_l2_error(const Launch &L, int ncells, const Vortex &v, double t, const double *x,
+                           const double *y, const double *q, const double *xr, const double *yr,
+                           const double *xs, const double *ys, const double *mass, double *error,
+                           double *partial) {
+  const int g = l2_error_blocks(L.Np, ncells);
+  if (g == 0) return hipSuccess;
+  hipLaunchKernelGGL(l2_error_kernel, dim3(g), dim3(256), 0, L.stream, ncells, L.Np, L.ph.gam, v, t, x, y, q,
+                     xr, yr, xs, ys, mass, error, partial);
   return hipGetLastError();
 }
 

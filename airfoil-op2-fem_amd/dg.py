@@ -284,6 +284,30 @@ class Context:
         self._chk(lib.dg_set_exact_state(self._h, float(uinf), float(beta), float(x0), float(y0),
                                          float(period_x)))
 
+    def l2_error(self, x, y, q, t, error=None, err=None):
+        """vortex/src/l2_error.h over cells (vortex/src/vortex.cpp:314-328):
+        error (optional) = (q - exact)^2 per node; returns err + the sums of
+        those squares over every node, per variable (r, ru, rv, ener)."""
+        K, Np = self.ncells, self.Np
+        out = np.zeros(4) if err is None else np.array(err, dtype=np.float64).reshape(4)
+        ep = None if error is None else _dev(error, "error", K * Np * 4)
+        self._chk(lib.dg_l2_error(self._h, _dev(x, "x", K * Np), _dev(y, "y", K * Np),
+                                  _dev(q, "q", self.state_numel()), float(t), ep,
+                                  out.ctypes.data_as(c_void_p)))
+        return out
+
+    def l2_error_mass(self, x, y, xr, yr, xs, ys, q, t):
+        """Mass-weighted L2 norm of q - exact per variable:
+        sqrt(sum_cells J e^T M e), J at each cell's node 0."""
+        K, Np = self.ncells, self.Np
+        out = np.zeros(4)
+        self._chk(lib.dg_l2_error_mass(self._h, _dev(x, "x", K * Np), _dev(y, "y", K * Np),
+                                       _dev(xr, "xr", K * Np), _dev(yr, "yr", K * Np),
+                                       _dev(xs, "xs", K * Np), _dev(ys, "ys", K * Np),
+                                       _dev(q, "q", self.state_numel()), float(t),
+                                       out.ctypes.data_as(c_void_p)))
+        return out
+
     def euler_rhs(self, workingQ, exteriorQ, rx, ry, sx, sy, fscale, nx, ny, qRHS):
         K, Np, NF, n = self.ncells, self.Np, self.NF, self.state_numel()
         self._chk(lib.dg_euler_rhs(self._h, _dev(workingQ, "workingQ", n),

@@ -160,6 +160,24 @@ int dg_get_bedge_q_exact(dg_ctx *ctx, const double *d_x, const double *d_y, doub
  * Defaults: 0, 5, 5, 0, 0; dg_solver_set_vortex sets the same values. */
 int dg_set_exact_state(dg_ctx *ctx, double uinf, double beta, double x0, double y0,
                        double period_x);
+/* The vortex case's error loop (vortex/src/l2_error.h:1-29, called at
+ * vortex/src/vortex.cpp:314-328) over cells: d_error ([ncells][Np][4],
+ * OP_WRITE; may be NULL) = (q - exact)^2 per node and variable, and
+ * err[k] += the sum of those squares over every node (the four OP_INC
+ * globals r, ru, rv, ener).  The exact state is dg_set_exact_state's vortex
+ * at time t and the node coordinates d_x, d_y (init_grid's); the reference's
+ * own formula hardcodes u_inf = 1 and drops the -t in v.  The sum is formed
+ * in a fixed order that depends on ncells only: repeated calls give the same
+ * bits.  Synchronises the context's stream. */
+int dg_l2_error(dg_ctx *ctx, const double *d_x, const double *d_y, const double *d_q, double t,
+                double *d_error, double err[4]);
+/* Mass-weighted L2 norm of the same errors (no reference counterpart; the
+ * norm a convergence study needs): err[k] = sqrt(sum_cells J e^T M e) with
+ * M the reference-element mass matrix and J = xr ys - xs yr at the cell's
+ * node 0 (exact for straight-sided cells).  err is overwritten. */
+int dg_l2_error_mass(dg_ctx *ctx, const double *d_x, const double *d_y, const double *d_xr,
+                     const double *d_yr, const double *d_xs, const double *d_ys,
+                     const double *d_q, double t, double err[4]);
 /* src/euler_rhs.h:222-291 over cells: volume term, Roe surface flux, LIFT;
  * zeroes exteriorQ on exit (OP_RW) and writes qRHS (OP_WRITE). */
 int dg_euler_rhs(dg_ctx *ctx, const double *d_workingQ, double *d_exteriorQ,

@@ -519,6 +519,62 @@ def test_exact_boundary_loop_matches_oracle(dg, orc, dev):
     assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("order,n", [(3, 12), (3, 64), (1, 9), (2, 7), (4, 8)])
+def test_l2_error_loop_matches_oracle(dg, orc, dev, order, n):
+    """dg_l2_error (vortex/src/l2_error.h over cells, vortex/src/vortex.cpp:
+    314-328): the squared nodal errors against the oracle's exact vortex and
+    their sums per variable added to the caller's accumulators (OP_INC);
+    dg_l2_error_mass against the oracle's mass-weighted norm.  The 64 x 64
+    box has more passes than workgroups, the other sizes one pass or fewer
+    per workgroup; repeated calls give the same bits."""
+    mesh = dg.rect_grid(n, n, 0.0, -5.0, 10.0, 10.0, seed=4, jitter=0.1)
+    ctx = dg.Context(order)
+    ctx.set_freestream(GAM, 0.5, 0.0, 1.0, 1.0)
+    ops = dg.generate_operators(order)
+    o = orc.Oracle(ops, orc.freestream(GAM, 0.5, 0.0))
+    case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
+                       mesh.bedgeNum, mesh.bedge_type)
+    ctx.decl_mesh(mesh)
+    ctx.set_exact_state(UINF, 5.0, 3.0, -1.0, 10.0)
+    g = case["geo"]
+    K, Np = mesh.ncells, ctx.Np
+    t = 0.83
+    ex = o.vortex_state(UINF, 5.0, 3.0, -1.0, t, g["x"], g["y"], 10.0)
+    rng = np.random.default_rng(order * 100 + n)
+    q = ex + 1e-3 * rng.normal(size=ex.shape)
+    sq = (q - ex) ** 2
+    x, y, qd = T(g["x"], dev), T(g["y"], dev), T(q, dev)
+    error = torch.full((K, Np, 4), -1.0, dtype=torch.float64, device=dev)
+    acc0 = np.array([1.0, 2.0, 3.0, 4.0])
+    got = ctx.l2_error(x, y, qd, t, error, acc0)
+    got_err = error.cpu().numpy()
+    # the two exact states differ by the device's and the host's exp / pow
+    # and by the multiply-adds the device contracts in x - x0 - uinf t and r^2
+    # (relative error up to about r^2 ulps in exp(1 - r^2)): bounded by 1e-13
+    # of the state's unit scale.  A difference d moves e^2 by at most
+    # 2|e|d + d^2.
+    tol = 1e-13 * (np.abs(ex) + 1.0)
+    dsq = 2 * np.sqrt(sq) * tol + tol ** 2 + 1e-15 * sq
+    worst = np.abs(got_err - sq) / dsq
+    assert worst.max() <= 1.0, (worst.max(), np.unravel_index(worst.argmax(), worst.shape))
+    ref = acc0 + sq.reshape(-1, 4).sum(axis=0)
+    bound = dsq.reshape(-1, 4).sum(axis=0) + 1e-13 * ref
+    assert np.all(np.abs(got - ref) <= bound), (got, ref, bound)
+    again = ctx.l2_error(x, y, qd, t, None, acc0)
+    assert np.array_equal(again, got)
+    geo = [T(g[k], dev) for k in ("xr", "yr", "xs", "ys")]
+    wl2 = ctx.l2_error_mass(x, y, *geo, qd, t)
+    M = ops["mass"]
+    e_orc = o.vortex_l2(g["x"], g["y"], g["J"], M, q, UINF, 5.0, 3.0, -1.0, t, 10.0)
+    # triangle inequality in the mass norm: |d|^T |M| |d| bounds d^T M d
+    bound = mass_l2(g["J"], np.abs(M), tol) + 1e-12 * e_orc
+    assert np.all(np.abs(wl2 - e_orc) <= bound), (wl2, e_orc, bound)
+    # the weighted norm of the errors' own closed form: at q = exact it is 0
+    zero = ctx.l2_error_mass(x, y, *geo, T(ex, dev), t)
+    assert np.all(zero <= mass_l2(g["J"], np.abs(M), tol)), zero
+    assert np.array_equal(ctx.l2_error_mass(x, y, *geo, qd, t), wl2)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_exact_boundary_vortex_run_matches_oracle(dg, orc, mode):
     """The vortex case with exact-state boundaries (vortex/src/vortex.cpp:

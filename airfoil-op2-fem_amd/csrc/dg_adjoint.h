@@ -79,8 +79,6 @@ hipError_t launch_adj_psi(const Launch &L, int ncells, const double *mu, double 
 // z = a x + b y  (y may be null)
 hipError_t launch_lincomb(const Launch &L, size_t n, double a, const double *x, double b,
                           const double *y, double *z);
-// *dst = cfl / dt1
-hipError_t launch_record_dt(const Launch &L, const unsigned long long *dt1, double cfl, double *dst);
 
 // Reverse of init_grid for the vertex coordinates: for every cell and each
 // of its 6 vertex coordinates d, gv[6 c + d] = sum over the cell's factors

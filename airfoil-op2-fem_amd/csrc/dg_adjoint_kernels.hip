@@ -923,10 +923,6 @@ __global__ void lincomb_kernel(size_t n, double a, const double *__restrict__ x,
     z[i] = a * x[i] + (y ? b * y[i] : 0.0);
 }
 
-__global__ void record_dt_kernel(const unsigned long long *dt1, double cfl, double *dst) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = cfl / __longlong_as_double((long long)*dt1);
-}
-
 // Geometry of the sensitivity path from the per-cell records cell_rel
 // [K][6] = (x0, y0, x1 - x0, y1 - y0, x2 - x0, y2 - y0): the factors of
 // init_grid_cell_rel_t (translation-invariant rounding), J at node 0.
@@ -1172,11 +1168,6 @@ hipError_t launch_lincomb(const Launch &L, size_t n, double a, const double *x, 
                           const double *y, double *z) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(lincomb_kernel, dim3(grid_of(n)), dim3(256), 0, L.stream, n, a, x, b, y, z);
-  return hipGetLastError();
-}
-
-hipError_t launch_record_dt(const Launch &L, const unsigned long long *dt1, double cfl, double *dst) {
-  hipLaunchKernelGGL(record_dt_kernel, dim3(1), dim3(64), 0, L.stream, dt1, cfl, dst);
   return hipGetLastError();
 }
 

@@ -77,6 +77,9 @@ struct StageArgs {
   // dt_fixed[case] instead of cfl / dt1 (sensitivity runs replay a recorded
   // dt sequence).
   const double *dt_fixed;
+  // Stage 0 of a recorded run: workgroup 0 stores the step's time step
+  // cfl / dt1_cur[0] here (case 0; sensitivity recording), or null.
+  double *dt_rec;
   // Host-side count of 16-cell tiles of a case table (0: unknown); lets the
   // launcher size the persistent grid so that every wave gets the same
   // number of tiles.

@@ -247,9 +247,8 @@ hipError_t step3(dg_sens *s, const double *q, double *w1out, double *w2out, doub
   a.dt1_cur = s->dt1 + p;
   a.dt1_next = s->dt1 + (p ^ 1);
   hipError_t e = hipSuccess;
-  if (record) e = dg::launch_record_dt(L, a.dt1_cur, s->cfl, s->dts + n);
+  if (record) a.dt_rec = s->dts + n;
   else a.dt_fixed = s->dts + n;
-  if (e != hipSuccess) return e;
   a.q = q;
   a.w = q; a.out = w1out;
   if ((e = dg::launch_rhs(L, a, true, dg::EPI_STAGE0)) != hipSuccess) return e;

@@ -158,8 +158,10 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   }
 
   const bool multi = a.ncases > 0;
-  if (EPI == EPI_STAGE0 && blockIdx.x == 0 && wave == 0)
+  if (EPI == EPI_STAGE0 && blockIdx.x == 0 && wave == 0) {
     for (int c = lane; c < a.ncases; c += 64) a.dt1_next[c] = 0ull;
+    if (a.dt_rec && lane == 0) a.dt_rec[0] = a.cfl / __longlong_as_double((long long)a.dt1_cur[0]);
+  }
   if (EPI == EPI_STAGE2 && blockIdx.x == 0 && wave == 0)
     for (int c = lane; c < a.ncases; c += 64)
       a.t_dev[c] += a.dt_fixed ? a.dt_fixed[c] : a.cfl / __longlong_as_double((long long)a.dt1_cur[c]);

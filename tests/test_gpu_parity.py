@@ -45,6 +45,26 @@ def assert_l2_errors_agree(J, M, q_gpu, q_orc, e_gpu, e_orc):
     assert np.all(diff <= bound * (1 + 1e-9) + 1e-14 * np.asarray(e_orc)), (diff, bound, e_gpu, e_orc)
 
 
+def device_l2(dg, mesh, g, q, t, dev, x0=5.0, y0=0.0, period=10.0):
+    """The library's own mass-weighted L2 error against the analytic vortex
+    (dg_l2_error_mass) of the state q on the oracle's geometry g."""
+    c = dg.Context(3)
+    c.set_freestream(GAM, 0.5, 0.0, 1.0, 1.0)
+    c.decl_mesh(mesh)
+    c.set_exact_state(UINF, 5.0, x0, y0, period)
+    d = [T(g[k], dev) for k in ("x", "y", "xr", "yr", "xs", "ys")]
+    return c.l2_error_mass(*d, T(q, dev), t)
+
+
+def assert_device_l2_agrees(J, M, q, e_dev, e_host):
+    """The device's error of q and the oracle's differ by the two exact
+    states' difference (exp / pow and contracted multiply-adds, within 1e-13
+    of the state's unit scale) in the mass norm, and by the sums' rounding."""
+    tol = 1e-13 * (np.abs(np.asarray(q)) + 1.0)
+    bound = mass_l2(J, np.abs(M), tol) + 1e-12 * np.asarray(e_host)
+    assert np.all(np.abs(np.asarray(e_dev) - np.asarray(e_host)) <= bound), (e_dev, e_host, bound)
+
+
 def rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -254,9 +274,10 @@ def test_fused_run_is_deterministic_and_resumable(dg, vortex64):
     assert a.time() == b.time()
 
 
-def test_vortex_error_matches_oracle_and_converges(dg, orc):
+def test_vortex_error_matches_oracle_and_converges(dg, orc, dev):
     """L2 error against the analytic vortex, GPU vs oracle, at 16x16, 32x32
-    and the 64x64 mesh of the CPU-sized benchmark case."""
+    and the 64x64 mesh of the CPU-sized benchmark case: the oracle's norm of
+    both states, and the library's own device norm of the GPU state."""
     errs = []
     for n in (16, 32, 64):
         mesh, ctx, s = _vortex_solver(dg, n)
@@ -277,15 +298,18 @@ def test_vortex_error_matches_oracle_and_converges(dg, orc):
         assert abs(t - to) <= 1e-12 * to
         e_gpu = o.vortex_l2(g["x"], g["y"], area / 2, M, qg, UINF, 5.0, 5.0, 0.0, t, 10.0)
         e_orc = o.vortex_l2(g["x"], g["y"], area / 2, M, st["q"], UINF, 5.0, 5.0, 0.0, to, 10.0)
-        print(f"vortex {n}x{n}, {steps} steps: L2 error vs analytic GPU {e_gpu}, oracle {e_orc}", flush=True)
+        e_dev = device_l2(dg, mesh, g, qg, t, dev)
+        print(f"vortex {n}x{n}, {steps} steps: L2 error vs analytic GPU {e_gpu} (device norm {e_dev}), "
+              f"oracle {e_orc}", flush=True)
         assert_l2_errors_agree(area / 2, M, qg, st["q"], e_gpu, e_orc)
+        assert_device_l2_agrees(area / 2, M, qg, e_dev, e_gpu)
         errs.append(e_gpu)
     ratio = np.array(errs[0]) / np.array(errs[1])
     assert np.all(ratio > 8.0), ratio  # observed order > 3 at N=3
     assert np.all(np.array(errs[2]) < np.array(errs[1])), errs
 
 
-def test_full_size_vortex_matches_oracle_and_conserves(dg, orc):
+def test_full_size_vortex_matches_oracle_and_conserves(dg, orc, dev):
     """The bench workload at its full size (724x724 quads, 1,048,352 cells,
     N=3, CFL 0.5): 100 fused SSPRK3 steps against the oracle's 100 steps on
     the host threads (<= 1e-12 relative L2 per conserved variable); the
@@ -315,8 +339,10 @@ def test_full_size_vortex_matches_oracle_and_conserves(dg, orc):
     # L2 error against the analytic vortex, GPU state vs oracle state
     e_gpu = o.vortex_l2(g["x"], g["y"], area / 2, M, q, UINF, 5.0, 5.0, 0.0, t, 10.0)
     e_orc = o.vortex_l2(g["x"], g["y"], area / 2, M, st["q"], UINF, 5.0, 5.0, 0.0, t, 10.0)
-    print(f"724x724 vortex L2 error vs analytic: GPU {e_gpu}, oracle {e_orc}", flush=True)
+    e_dev = device_l2(dg, mesh, g, q, t, dev)
+    print(f"724x724 vortex L2 error vs analytic: GPU {e_gpu} (device norm {e_dev}), oracle {e_orc}", flush=True)
     assert_l2_errors_agree(area / 2, M, q, st["q"], e_gpu, e_orc)
+    assert_device_l2_agrees(area / 2, M, q, e_dev, e_gpu)
     w = M.sum(axis=1)
 
     def totals(x):

@@ -153,13 +153,30 @@ def test_autograd_wrapper(dg):
 
 
 def test_adjoint_matches_central_differences(sens):
+    """Central differences of C_L with the recorded dt sequence.  C_L
+    carries a few hundred ulps of rounding noise from one perturbed forward
+    run to the next, which the plain central difference at h = 1e-6 divides
+    by 2h: ~2e-10 absolute, i.e. above 1e-6 relative on a component as small
+    as 2e-4.  So the plain central difference is held to 1e-6 in relative
+    L2 over the five components, and the fourth-order central difference at
+    h = 1e-5 (truncation O(h^4), noise ~5 times smaller) to 1e-6 relative
+    on every component, with no absolute slack."""
     s, b = sens["s"], sens["bumps"]
+
+    def cl_at(i, d):
+        if i == 0:
+            return s.eval(2.0 + d, b)
+        bb = b.copy()
+        bb[i - 1] += d
+        return s.eval(2.0, bb)
+    g = np.array([sens["da"]] + list(sens["db"]))
     h = 1e-6
-    fd = (s.eval(2.0 + h, b) - s.eval(2.0 - h, b)) / (2 * h)
-    assert abs(fd - sens["da"]) <= 1e-6 * abs(sens["da"])
-    for i in range(4):
-        bp, bm = b.copy(), b.copy()
-        bp[i] += h
-        bm[i] -= h
-        fd = (s.eval(2.0, bp) - s.eval(2.0, bm)) / (2 * h)
-        assert abs(fd - sens["db"][i]) <= 1e-6 * abs(sens["db"][i]) + 1e-10, (i, fd, sens["db"][i])
+    fd2 = np.array([(cl_at(i, h) - cl_at(i, -h)) / (2 * h) for i in range(5)])
+    h4 = 1e-5
+    fd4 = np.array([(-cl_at(i, 2 * h4) + 8 * cl_at(i, h4) - 8 * cl_at(i, -h4) + cl_at(i, -2 * h4)) / (12 * h4)
+                    for i in range(5)])
+    r2 = np.abs(fd2 - g) / np.abs(g)
+    r4 = np.abs(fd4 - g) / np.abs(g)
+    print(f"central h=1e-6: rel {r2.tolist()}; 4th order h=1e-5: rel {r4.tolist()}", flush=True)
+    assert np.all(r4 <= 1e-6), r4
+    assert np.linalg.norm(fd2 - g) <= 1e-6 * np.linalg.norm(g), r2

@@ -33,7 +33,8 @@
 // the volume products), a Roe item's own trace, exterior trace and geometry
 // go out together (a boundary slot reads its own record as a stand-in
 // exterior), one item ahead of its flux, and each group's
-// epilogue loads (metric factors, w, q) go out before its products.
+// epilogue loads (metric factors, w, q) go out before its products (the
+// stage-1 kernel's metric factors of all four groups before the first).
 // Running the surface term first keeps the accumulators out of the Roe
 // phase, the phase with the most live values.
 //
@@ -397,6 +398,23 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     int ncode[NRT];
     load_codes(e0n, nen, ncode);
 
+    // The stage-1 kernel loads the per-node metric factors of all four groups
+    // here, so that only w and q are left to each group's epilogue (724x724
+    // vortex: stage 1 401-405 -> 387-396 us; the same in stage 0, or rx and
+    // sx alone in stage 2, measured 1.5% and 5% slower, all four in stage 2
+    // spill).
+    constexpr bool kMetricsAhead = !CG && EPI == EPI_STAGE1;
+    double hrx[G], hsx[G], hry[G], hsy[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      hrx[g] = hsx[g] = hry[g] = hsy[g] = 0.0;
+      if (kMetricsAhead) {
+        const int e = 4 * g + ce;
+        const bool ok = e < ne && jj < Np;
+        const size_t gn = (size_t)(e0 + (ok ? e : 0)) * Np + (ok ? jj : 0);
+        if (ok) { hrx[g] = a.rx[gn]; hsx[g] = a.sx[gn]; hry[g] = a.ry[gn]; hsy[g] = a.sy[gn]; }
+      }
+    }
     // ---- volume products, metric terms and the epilogue, one 4-cell group at a time
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -404,15 +422,17 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       const int e = 4 * g + ce;
       const bool ok = e < ne && jj < Np;
       const size_t gn = (size_t)(e0 + (ok ? e : 0)) * Np + (ok ? jj : 0);
-      // This group's global loads (metric factors, stage input w, base
-      // state q) go out before its products, so their latency overlaps the
-      // matrix work.  (Loading w after the products, which kept 8 fewer
+      // This group's global loads (metric factors unless loaded above, stage
+      // input w, base state q) go out before its products, so their latency
+      // overlaps the matrix work.  (Loading w after the products, which kept 8 fewer
       // registers live across them, put a dependent round trip into every
       // group's epilogue: stage 2 measured 432 -> 411 us with it moved up.)
       double rxv = 0.0, sxv = 0.0, ryv = 0.0, syv = 0.0;
       double2 w01 = make_double2(0.0, 0.0), w23 = w01, q01 = w01, q23 = w01;
       if (ok) {
-        if constexpr (!CG) {
+        if constexpr (kMetricsAhead) {
+          rxv = hrx[g]; sxv = hsx[g]; ryv = hry[g]; syv = hsy[g];
+        } else if constexpr (!CG) {
           rxv = a.rx[gn]; sxv = a.sx[gn]; ryv = a.ry[gn]; syv = a.sy[gn];
         }
         if (EPI != EPI_RHS) {

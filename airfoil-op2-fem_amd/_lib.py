@@ -66,6 +66,7 @@ _SIGS = {
     "dg_mesh_destroy": ([c_void_p], None),
     "dg_mesh_sizes": ([c_void_p, P(c_int), P(c_int), P(c_int), P(c_int)], c_int),
     "dg_mesh_get": ([c_void_p, c_char_p, c_void_p], c_int),
+    "dg_mesh_set_bedge_types": ([c_void_p, c_int, c_void_p], c_int),
     "dg_solver_create": ([c_void_p, c_void_p, P(c_void_p)], c_int),
     "dg_solver_destroy": ([c_void_p], None),
     "dg_solver_set_vortex": ([c_void_p, c_double, c_double, c_double, c_double, c_double], c_int),

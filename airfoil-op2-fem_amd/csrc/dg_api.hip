@@ -617,6 +617,19 @@ int dg_mesh_sizes(const dg_mesh *m, int *nnodes, int *ncells, int *nedges, int *
   return DG_OK;
 }
 
+int dg_mesh_set_bedge_types(dg_mesh *m, int nbedges, const int *types) {
+  if (!m || (nbedges > 0 && !types)) return fail(nullptr, DG_ERR_ARG, "dg_mesh_set_bedge_types: null argument");
+  if (nbedges != m->m.nbedges)
+    return fail(nullptr, DG_ERR_ARG, "dg_mesh_set_bedge_types: nbedges must equal the mesh's " +
+                                         std::to_string(m->m.nbedges) + " boundary edges");
+  for (int b = 0; b < nbedges; b++)
+    if (types[b] < DG_BC_INFLOW || types[b] > DG_BC_EXACT)
+      return fail(nullptr, DG_ERR_ARG, "dg_mesh_set_bedge_types: type " + std::to_string(types[b]) +
+                                           " of boundary edge " + std::to_string(b) + " is not in [0, 4]");
+  std::copy(types, types + nbedges, m->m.bedge_type.begin());
+  return DG_OK;
+}
+
 int dg_mesh_get(const dg_mesh *m, const char *field, void *out) {
   if (!m || !field || !out) return fail(nullptr, DG_ERR_ARG, "dg_mesh_get: null argument");
   const dg::Mesh &M = m->m;

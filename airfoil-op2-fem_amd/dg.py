@@ -74,6 +74,13 @@ class Mesh:
             self._cache[field] = a
         return self._cache[field]
 
+    def set_bedge_types(self, types):
+        """Boundary class of every boundary edge (DG_BC_* values, one per
+        boundary edge), e.g. all BC_EXACT for the vortex case."""
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(types, dtype=np.int32), (self.nbedges,)))
+        check(lib.dg_mesh_set_bedge_types(self._h, self.nbedges, _np_ptr(t) if t.size else None))
+        self._cache.pop("bedge_type", None)
+
     def __getattr__(self, name):
         if name in Mesh._SHAPES:
             return self.get(name)

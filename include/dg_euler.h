@@ -234,6 +234,13 @@ int dg_mesh_sizes(const dg_mesh *m, int *nnodes, int *ncells, int *nedges, int *
  * "edgeNum", "edge2nodes" (int [nedges][2]), "bedge2cells", "bedgeNum",
  * "bedge_type" (int [nbedges]), "bedge2nodes" (int [nbedges][2]). */
 int dg_mesh_get(const dg_mesh *m, const char *field, void *host_out);
+/* Replaces the boundary class of every boundary edge (types: int [nbedges],
+ * each a DG_BC_* value; nbedges must be the mesh's count).  The vortex case
+ * classifies nothing: its loader reads no boundary types
+ * (vortex/src/load_mesh.cpp:122-145) and every boundary edge gets the exact
+ * state (vortex/src/vortex.cpp:250-260), i.e. DG_BC_EXACT for all of them
+ * after dg_mesh_read_vtk. */
+int dg_mesh_set_bedge_types(dg_mesh *m, int nbedges, const int *types);
 
 /* ------------------------------------------------------------------ */
 /* Solver: the SSPRK3 driver of src/airfoil.cpp:204-334 on device      */

@@ -30,16 +30,13 @@ def ops3(dg):
     return dg.generate_operators(3)
 
 
-@pytest.fixture(scope="session")
-def naca_vtk(tmp_path_factory):
-    """The reference's airfoil mesh (tests/golden/naca0012_mesh.npz, made by
-    make_naca0012_mesh.py from naca0012.vtk) written back out as a
-    triangle-only legacy-VTK file for the library's reader."""
-    d = np.load(os.path.join(ROOT, "tests", "golden", "naca0012_mesh.npz"))
+def _write_triangle_vtk(npz_name, path, title):
+    """A fixture mesh (points, triangles) written as a triangle-only
+    legacy-VTK file for the library's reader."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", npz_name))
     pts, tris = d["points"], d["triangles"]
-    path = tmp_path_factory.mktemp("mesh") / "naca0012.vtk"
     with open(path, "w") as f:
-        f.write("# vtk DataFile Version 2.0\nnaca0012\nASCII\nDATASET UNSTRUCTURED_GRID\n")
+        f.write(f"# vtk DataFile Version 2.0\n{title}\nASCII\nDATASET UNSTRUCTURED_GRID\n")
         f.write(f"POINTS {len(pts)} double\n")
         for p in pts:  # shortest round-trip decimal form: the same doubles back
             f.write(f"{float(p[0])!r} {float(p[1])!r} {float(p[2])!r}\n")
@@ -49,6 +46,22 @@ def naca_vtk(tmp_path_factory):
         f.write(f"CELL_TYPES {len(tris)}\n")
         f.write("5\n" * len(tris))
     return str(path)
+
+
+@pytest.fixture(scope="session")
+def naca_vtk(tmp_path_factory):
+    """The reference's airfoil mesh (tests/golden/naca0012_mesh.npz, made by
+    make_naca0012_mesh.py from naca0012.vtk) written back out as a
+    triangle-only legacy-VTK file for the library's reader."""
+    return _write_triangle_vtk("naca0012_mesh.npz", tmp_path_factory.mktemp("mesh") / "naca0012.vtk",
+                               "naca0012")
+
+
+@pytest.fixture(scope="session")
+def vortex_vtk(tmp_path_factory):
+    """The reference's vortex mesh (tests/golden/vortex_grid_mesh.npz, made
+    by make_vortex_grid_mesh.py from vortex/grid.vtk), likewise."""
+    return _write_triangle_vtk("vortex_grid_mesh.npz", tmp_path_factory.mktemp("mesh") / "grid.vtk", "grid")
 
 
 def rel_l2(a, b):

@@ -628,3 +628,94 @@ def test_exact_boundary_vortex_run_matches_oracle(dg, orc, mode):
     e_gpu = o.vortex_l2(g["x"], g["y"], area / 2, M, q, UINF, 5.0, 5.0, -4.0, t)
     e_orc = o.vortex_l2(g["x"], g["y"], area / 2, M, st["q"], UINF, 5.0, 5.0, -4.0, t)
     assert_l2_errors_agree(area / 2, M, q, st["q"], e_gpu, e_orc)
+
+
+def test_reference_vortex_scenario(dg, orc, dev, vortex_vtk):
+    """The reference's vortex run end to end (vortex/src/vortex.cpp:195-328)
+    on the library's per-loop entry points, driven in the reference's order:
+    its mesh (vortex/grid.vtk, 618 triangles, through the VTK reader, every
+    boundary edge the exact state), the literal N=4 tables, the vortex
+    (beta 5 at (5, 0), u_inf 1, gamma 1.4) as the initial state, dt = 0.1 /
+    dt1 from the initial state and fixed from then on (:213-222), and per
+    stage neighbour_zero, get_neighbour_q, the exact-state boundary at
+    t + {0, 1, 1/2} dt, euler_rhs and (stages 0, 1) set_workingQ, update_Q
+    per step (:230-307); 2000 steps.  Against the oracle's fixed-dt run:
+    <= 1e-12 relative L2 per variable, and the L2 errors against the
+    analytic vortex at the end (l2_error's role, :314-328), the library's
+    device norm and the oracle's, agree."""
+    import json
+    import os
+    tab = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "n4_tables.json")))
+    names = ("r", "s", "Dr", "Ds", "Drw", "Dsw", "LIFT", "FMASK")
+    mesh = dg.read_vtk(vortex_vtk)
+    mesh.set_bedge_types(dg.BC_EXACT)
+    uinf, beta, x0, y0 = 1.0, 5.0, 5.0, 0.0
+    mach = uinf / math.sqrt(GAM)  # p = rho = 1
+    ctx = dg.Context(4)
+    for k in names:
+        ctx.set_const(k, tab[k])
+    ctx.set_freestream(GAM, mach, 0.0, 1.0, 1.0)
+    ctx.decl_mesh(mesh)
+    ctx.set_exact_state(uinf, beta, x0, y0, 0.0)
+    K, Np, NF = mesh.ncells, ctx.Np, ctx.NF
+
+    def z(n):
+        return torch.zeros(n, dtype=torch.float64, device=dev)
+    geo = {k: z(K * 3) for k in ("nodeX", "nodeY")}
+    geo.update({k: z(K * Np) for k in ("x", "y", "xr", "yr", "xs", "ys", "rx", "ry", "sx", "sy")})
+    geo.update({k: z(K * NF) for k in ("nx", "ny", "fscale")})
+    ctx.init_grid(T(mesh.cell_xy, dev), geo)
+
+    ops = {k: np.array(tab[k]) for k in names}
+    ops["N"] = 4
+    o = orc.Oracle(ops, orc.freestream(GAM, mach, 0.0))
+    o.set_exact_state(uinf, beta, x0, y0, 0.0)
+    case = o.make_case(mesh.cells, mesh.cell_xy, mesh.edge2cells, mesh.edgeNum, mesh.bedge2cells,
+                       mesh.bedgeNum, mesh.bedge_type)
+    g = case["geo"]
+    for k in ("x", "y", "rx", "sy", "nx", "fscale"):
+        assert np.array_equal(geo[k].cpu().numpy(), g[k].ravel()), k  # init_grid: the oracle's bits
+
+    q0 = o.vortex_state(uinf, beta, x0, y0, 0.0, g["x"], g["y"])  # set_ic at t = 0
+    q, w = T(q0, dev).reshape(-1), T(q0, dev).reshape(-1)
+    ext = z(K * NF * 4)
+    rk = [z(K * Np * 4) for _ in range(3)]
+    dt1 = ctx.calc_dt(q, geo["fscale"])
+    dt1_o = o.calc_dt_all(case, q0)
+    assert abs(dt1 - dt1_o) <= 1e-14 * dt1_o, (dt1, dt1_o)
+    dt = 0.1 / dt1
+    nsteps, t = 2000, 0.0
+    frac = (0.0, 1.0, 0.5)
+    for _ in range(nsteps):
+        for j in range(3):
+            ctx.neighbour_zero(ext)
+            ctx.get_neighbour_q(geo["nodeX"], geo["nodeY"], w, ext)
+            ctx.get_bedge_q_exact(geo["x"], geo["y"], t + frac[j] * dt, ext)
+            ctx.euler_rhs(w, ext, geo["rx"], geo["ry"], geo["sx"], geo["sy"], geo["fscale"], geo["nx"],
+                          geo["ny"], rk[j])
+            if j != 2:
+                ctx.set_workingQ(dt, j, q, rk[0], rk[1], w)
+        ctx.update_Q(dt, q, rk[0], rk[1], rk[2], w)
+        t += dt
+    torch.cuda.synchronize()
+    qg = q.cpu().numpy().reshape(K, Np, 4)
+    assert np.all(np.isfinite(qg))
+
+    st = o.new_state(case, q0)
+    t_o = o.run_dt(case, st, nsteps, dt, nthreads=oracle_threads())
+    assert abs(t_o - t) <= 1e-13 * t
+    errs = rel_per_var(qg, st["q"])
+    print(f"vortex grid.vtk N=4, {nsteps} steps of dt {dt:.4e}: rel L2 per variable vs oracle {errs}",
+          flush=True)
+    assert max(errs) <= 1e-12, errs
+
+    X, Y = g["nodeX"], g["nodeY"]
+    area = 0.5 * ((X[:, 1] - X[:, 0]) * (Y[:, 2] - Y[:, 0]) - (X[:, 2] - X[:, 0]) * (Y[:, 1] - Y[:, 0]))
+    M = dg.generate_operators(4)["mass"]
+    e_gpu = o.vortex_l2(g["x"], g["y"], area / 2, M, qg, uinf, beta, x0, y0, t)
+    e_orc = o.vortex_l2(g["x"], g["y"], area / 2, M, st["q"], uinf, beta, x0, y0, t)
+    assert_l2_errors_agree(area / 2, M, qg, st["q"], e_gpu, e_orc)
+    e_dev = ctx.l2_error_mass(geo["x"], geo["y"], geo["xr"], geo["yr"], geo["xs"], geo["ys"], q, t)
+    assert_device_l2_agrees(area / 2, M, qg, e_dev, e_gpu)
+    print(f"vortex grid.vtk: L2 error vs the analytic vortex at t = {t:.4f}: GPU {e_gpu}, oracle {e_orc}, "
+          f"device norm {e_dev}", flush=True)

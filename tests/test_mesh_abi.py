@@ -95,6 +95,32 @@ def test_mesh_fixture_reads_like_the_reference_file(dg, naca_vtk):
             assert np.array_equal(getattr(m, k), getattr(r, k)), k
 
 
+def test_vortex_mesh_fixture_and_boundary_reclassification(dg, vortex_vtk):
+    """The reference's vortex mesh (vortex/grid.vtk: 618 triangles on the box
+    [0,10] x [-5,5]) through the VTK reader; every boundary edge lies on the
+    box and, reclassified as the vortex case does (vortex/src/vortex.cpp:
+    250-260), carries the exact state.  Wrong counts and unknown classes are
+    rejected and leave the classes unchanged."""
+    m = dg.read_vtk(vortex_vtk)
+    assert (m.ncells, m.nnodes) == (618, 342)
+    assert np.all(_areas(m.cell_xy.reshape(-1, 3, 2)) > 0)
+    # every boundary edge has both end points on one side of the box
+    be = m.coords[m.bedge2nodes]
+    on_x = np.all(np.isin(be[:, :, 0], [0.0, 10.0]), axis=1) & (be[:, 0, 0] == be[:, 1, 0])
+    on_y = np.all(np.isin(be[:, :, 1], [-5.0, 5.0]), axis=1) & (be[:, 0, 1] == be[:, 1, 1])
+    assert np.all(on_x | on_y)
+    # the box's perimeter, 40, is covered once
+    assert abs(np.linalg.norm(be[:, 1] - be[:, 0], axis=1).sum() - 40.0) < 1e-12
+    before = m.bedge_type.copy()
+    with pytest.raises(dg.DGError):
+        m.set_bedge_types(np.full(m.nbedges, 5))
+    zeros = np.zeros(m.nbedges, np.int32)
+    assert dg.lib.dg_mesh_set_bedge_types(m._h, m.nbedges - 1, zeros.ctypes.data) == -1  # DG_ERR_ARG
+    assert np.array_equal(m.bedge_type, before)
+    m.set_bedge_types(dg.BC_EXACT)
+    assert np.all(m.bedge_type == dg.BC_EXACT)
+
+
 def test_mesh_errors_are_reported(dg):
     with pytest.raises(dg.DGError):
         dg.periodic_grid(2, 2)

@@ -218,7 +218,7 @@ def adjoint_leg(order=3, steps=400, ckpt=20, checks=True):
 
 
 def adjoint_bench(args):
-    out = adjoint_leg(args.order, args.adj_steps, 20, checks=not args.no_checks)
+    out = adjoint_leg(args.order, args.adj_steps, args.ckpt, checks=not args.no_checks)
     print(json.dumps(dict({"metric": "adjoint gradient cost / forward cost", "higher_is_better": False,
                            "dtype": "f64", "config": {"workload": out["workload"]}}, **out)))
 
@@ -376,6 +376,9 @@ def main():
         ap = argparse.ArgumentParser()
         ap.add_argument("--adjoint", action="store_true")
         ap.add_argument("--adj-steps", type=int, default=400)
+        ap.add_argument("--ckpt", type=int, default=20,
+                        help="checkpoint interval in steps (--adj-steps: one segment, every stage state "
+                             "kept by the forward run, no recomputation)")
         ap.add_argument("--order", type=int, default=3)
         ap.add_argument("--no-checks", action="store_true",
                         help="timings only (no tangent-linear or finite-difference runs)")

@@ -653,7 +653,11 @@ __global__ __launch_bounds__(64, 3) void adj_cell_mfma_kernel(AdjArgs a, Phys ph
     // ---- the tile's loads, one round trip: mu, gmq and the factors into
     // LDS; the lane's states and the running metric cotangents into
     // registers (the epilogue's x0 and y0 chunks load after the products,
-    // which would otherwise spill)
+    // which would otherwise spill).  They are issued at a raised wave
+    // priority, as in the stage kernels (reverse sweep on the 65,536-cell
+    // O-grid 82.2 -> 81.5 ms over three alternating runs; the same in the
+    // face-pair pass gained nothing more)
+    __builtin_amdgcn_s_setprio(2);
     copy_to_lds<RCH>(a.mu + (size_t)e0 * Np * 4, sMu, ne * Np * 2, lane);
     copy_to_lds<SCH>(a.gmq + (size_t)e0 * NF * 4, sGq, ne * NF * 2, lane);
     copy_to_lds<TE * 2>(a.cmet + (size_t)e0 * 4, sMet, ne * 2, lane);
@@ -671,6 +675,7 @@ __global__ __launch_bounds__(64, 3) void adj_cell_mfma_kernel(AdjArgs a, Phys ph
     const int nch = ne * Np * 2;
     const size_t kc = (size_t)e0 + min(ln, ne - 1);
     const double crx = a.g_rx[kc], csx = a.g_sx[kc], cry = a.g_ry[kc], csy = a.g_sy[kc];
+    __builtin_amdgcn_s_setprio(0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the LDS tiles are in place
     // the face slots that read node jj (at most two)
     int ms0 = -1, ms1 = -1;

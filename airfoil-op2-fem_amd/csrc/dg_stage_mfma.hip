@@ -243,7 +243,15 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     __syncthreads();  // the previous tile's volume products are done with the LDS tiles
 
     // the tile's state records go to LDS while the face phase runs; the
-    // Euler-flux phase reads them there instead of waiting on its own loads
+    // Euler-flux phase reads them there instead of waiting on its own loads.
+    // Each stretch of code that issues a tile's loads (this copy and the
+    // first Roe item, each next Roe item, each group's epilogue loads) runs
+    // at a raised wave priority, so that a wave about to put loads in flight
+    // is picked before waves with arithmetic ready: more of the CU's 12
+    // waves keep loads outstanding (724x724 vortex: 1.147-1.160 -> 1.140-
+    // 1.147 ms per step over four alternating runs on two boxes; the copy
+    // alone or the epilogue alone measured no gain, priority 3 the same as 2)
+    __builtin_amdgcn_s_setprio(2);
     copy_to_lds<TE * Np * 2>(a.w + (size_t)e0 * Np * 4, sW, active ? ne * Np * 2 : 1, lane);
 
     // ---- Roe fluxes per (cell, face node), written in the A-operand layout
@@ -291,9 +299,14 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       // per-cell geometry step 1.10 -> 1.03 ms)
       RoeIn cur, nxt;
       load_item(0, cur);
+      __builtin_amdgcn_s_setprio(0);
 #pragma unroll
       for (int j = 0; j < NRT; j++) {
-        if (j + 1 < NRT) load_item(j + 1, nxt);
+        if (j + 1 < NRT) {
+          __builtin_amdgcn_s_setprio(2);
+          load_item(j + 1, nxt);
+          __builtin_amdgcn_s_setprio(0);
+        }
         const int it = lr + 64 * j;
         if (it < nrt) {
           const int e = it / NF, m = it - e * NF;
@@ -429,6 +442,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       // group's epilogue: stage 2 measured 432 -> 411 us with it moved up.)
       double rxv = 0.0, sxv = 0.0, ryv = 0.0, syv = 0.0;
       double2 w01 = make_double2(0.0, 0.0), w23 = w01, q01 = w01, q23 = w01;
+      __builtin_amdgcn_s_setprio(2);
       if (ok) {
         if constexpr (kMetricsAhead) {
           rxv = hrx[g]; sxv = hsx[g]; ryv = hry[g]; syv = hsy[g];
@@ -444,6 +458,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
           q01 = qs[0]; q23 = qs[1];
         }
       }
+      __builtin_amdgcn_s_setprio(0);
       dbl4 c = acc[g];
       if constexpr (CG) {
 #pragma unroll

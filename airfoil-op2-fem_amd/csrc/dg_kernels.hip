@@ -2,7 +2,7 @@
 // Euler path (face traces, boundary states, SSPRK3 stage updates, CFL
 // reduction), the set-up loops (init_grid, initial states) and the wall
 // forces.  FP64 throughout.  The fused stage kernel (Roe fluxes, volume and
-// LIFT products on the matrix cores, stage epilogues) is in dg_stage_mfma.hip.
+// LIFT products on the matrix cores, stage epilogues) is in dg_stage_kernel.h.
 #include <hip/hip_runtime.h>
 
 #include "dg_device.h"
@@ -11,7 +11,7 @@
 namespace dg {
 
 // ------------------------------------------------------------------------
-// Stage kernel: the matrix-core kernel of dg_stage_mfma.hip
+// Stage kernel: the matrix-core kernel of dg_stage_kernel.h
 // ------------------------------------------------------------------------
 
 hipError_t launch_rhs(const Launch &L, const StageArgs &a, bool gather, int epi) {

@@ -426,7 +426,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         const int e = 4 * g + ce;
         const bool ok = e < ne && jj < Np;
         const size_t gn = (size_t)(e0 + (ok ? e : 0)) * Np + (ok ? jj : 0);
-        if (ok) { hrx[g] = a.rx[gn]; hsx[g] = a.sx[gn]; hry[g] = a.ry[gn]; hsy[g] = a.sy[gn]; }
+        if (4 * g < ne) { hrx[g] = a.rx[gn]; hsx[g] = a.sx[gn]; hry[g] = a.ry[gn]; hsy[g] = a.sy[gn]; }
       }
     }
     // ---- volume products, metric terms and the epilogue, one 4-cell group at a time
@@ -444,7 +444,12 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       double rxv = 0.0, sxv = 0.0, ryv = 0.0, syv = 0.0;
       double2 w01 = make_double2(0.0, 0.0), w23 = w01, q01 = w01, q23 = w01;
       __builtin_amdgcn_s_setprio(2);
-      if (ok) {
+      // Every lane issues them: a lane past the group's cells or nodes reads
+      // the tile's node 0 (its values are not used).  The conditional form
+      // zeroed the destination registers first, and the compiler made each
+      // group wait there for loads still in flight (stage 1 388-391 ->
+      // 377-382 us with unconditional loads; stages 0 and 2 unchanged).
+      {
         if constexpr (kMetricsAhead) {
           rxv = hrx[g]; sxv = hsx[g]; ryv = hry[g]; syv = hsy[g];
         } else if constexpr (!CG) {

@@ -239,7 +239,9 @@ int dg_mesh_get(const dg_mesh *m, const char *field, void *host_out);
  * classifies nothing: its loader reads no boundary types
  * (vortex/src/load_mesh.cpp:122-145) and every boundary edge gets the exact
  * state (vortex/src/vortex.cpp:250-260), i.e. DG_BC_EXACT for all of them
- * after dg_mesh_read_vtk. */
+ * after dg_mesh_read_vtk.  Solvers and dg_decl_mesh copy the classes when
+ * they are created or called, so a change reaches only those created later.
+ * On an error the classes are left unchanged. */
 int dg_mesh_set_bedge_types(dg_mesh *m, int nbedges, const int *types);
 
 /* ------------------------------------------------------------------ */

@@ -45,6 +45,7 @@ _SIGS = {
                        c_void_p], c_int),
     "dg_init_grid": ([c_void_p] + [c_void_p] * 16, c_int),
     "dg_set_ic": ([c_void_p, c_void_p, c_void_p], c_int),
+    "dg_set_ic_vortex": ([c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "dg_neighbour_zero": ([c_void_p, c_void_p], c_int),
     "dg_get_neighbour_q": ([c_void_p] + [c_void_p] * 4, c_int),
     "dg_get_bedge_q": ([c_void_p] + [c_void_p] * 4, c_int),

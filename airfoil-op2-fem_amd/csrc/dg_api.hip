@@ -409,6 +409,14 @@ int dg_get_bedge_q_exact(dg_ctx *c, const double *x, const double *y, double t, 
   return DG_OK;
 }
 
+int dg_set_ic_vortex(dg_ctx *c, double t, const double *x, const double *y, double *q, double *w) {
+  NEED_MESH(c, "dg_set_ic_vortex");
+  if (c->ncells > 0 && (!x || !y || !q || !w)) return fail(c, DG_ERR_ARG, "dg_set_ic_vortex: null array");
+  const dg::Vortex &v = c->vortex;
+  HIPCHECK(c, dg::launch_set_vortex(launch_of(c), c->ncells, x, y, v.uinf, v.beta, v.x0, v.y0, v.period, t, q, w));
+  return DG_OK;
+}
+
 int dg_set_exact_state(dg_ctx *c, double uinf, double beta, double x0, double y0, double period_x) {
   if (!c) return fail(nullptr, DG_ERR_ARG, "dg_set_exact_state: null context");
   if (!(period_x >= 0.0)) return fail(c, DG_ERR_ARG, "dg_set_exact_state: period_x must be >= 0");

@@ -140,7 +140,7 @@ hipError_t launch_cell_metrics(const Launch &L, int ncells, const double *rx, co
 hipError_t launch_set_ic(const Launch &L, int ncells, double *q, double *w);
 hipError_t launch_set_vortex(const Launch &L, int ncells, const double *x,
                              const double *y, double uinf, double beta, double x0,
-                             double y0, double period_x, double *q, double *w);
+                             double y0, double period_x, double t, double *q, double *w);
 hipError_t launch_zero(const Launch &L, size_t n, double *p);
 hipError_t launch_copy(const Launch &L, size_t n, const double *src, double *dst);
 // out[i][0..4) = q node record rec[i] (zeros where rec[i] < 0)

@@ -235,16 +235,18 @@ __global__ void set_ic_kernel(size_t nnodes, Phys ph, double *q, double *w) {
   }
 }
 
-// Isentropic vortex (vortex/src/set_ic.h:1-28, with free-stream speed uinf).
+// Isentropic vortex at time t (vortex/src/set_ic.h:1-28, with free-stream
+// speed uinf).
 __global__ void set_vortex_kernel(size_t nnodes, double gam, const double *__restrict__ x,
                                   const double *__restrict__ y, double uinf, double beta,
-                                  double x0, double y0, double period_x, double *q, double *w) {
+                                  double x0, double y0, double period_x, double t, double *q,
+                                  double *w) {
   Vortex vx;
   vx.uinf = uinf; vx.beta = beta; vx.x0 = x0; vx.y0 = y0; vx.period = period_x;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nnodes;
        i += (size_t)gridDim.x * blockDim.x) {
     double qv[4];
-    vortex_state(gam, vx, 0.0, x[i], y[i], qv);
+    vortex_state(gam, vx, t, x[i], y[i], qv);
     for (int k = 0; k < 4; k++) { q[4 * i + k] = qv[k]; w[4 * i + k] = qv[k]; }
   }
 }
@@ -489,11 +491,11 @@ hipError_t launch_exact_bc(const Launch &L, int nb, const int *b2c, const int *b
 
 hipError_t launch_set_vortex(const Launch &L, int ncells, const double *x, const double *y,
                              double uinf, double beta, double x0, double y0, double period_x,
-                             double *q, double *w) {
+                             double t, double *q, double *w) {
   size_t n = (size_t)ncells * L.Np;
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(set_vortex_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, L.stream, n,
-                     L.ph.gam, x, y, uinf, beta, x0, y0, period_x, q, w);
+                     L.ph.gam, x, y, uinf, beta, x0, y0, period_x, t, q, w);
   return hipGetLastError();
 }
 

@@ -403,7 +403,7 @@ int dg_solver_set_vortex(dg_solver *s, double uinf, double beta, double x0, doub
   if (!s) return fail(nullptr, DG_ERR_ARG, "dg_solver_set_vortex: null solver");
   if (!(period_x >= 0.0)) return fail(s->ctx, DG_ERR_ARG, "dg_solver_set_vortex: period_x must be >= 0");
   HIPCHECK(s->ctx, dg::launch_set_vortex(launch_s(s), s->K, dat(s, "x"), dat(s, "y"), uinf, beta,
-                                         x0, y0, period_x, s->q, s->w));
+                                         x0, y0, period_x, 0.0, s->q, s->w));
   // the same vortex is the exact state of type-4 boundaries
   s->ctx->vortex.uinf = uinf; s->ctx->vortex.beta = beta; s->ctx->vortex.x0 = x0;
   s->ctx->vortex.y0 = y0; s->ctx->vortex.period = period_x;

@@ -134,11 +134,17 @@ void declare_mesh(op_set cells) {
   const int ne = s.e2c ? s.e2c->from->size : 0;
   const int nb = s.b2c ? s.b2c->from->size : 0;
   if (ne > 0 && !s.enum_) die("no int dat of dimension 2 on the edge set (edgeNum)");
-  if (nb > 0 && (!s.bnum || !s.btype)) die("boundary edges need int dats named bedgeNum and bedge_type");
+  if (nb > 0 && !s.bnum) die("boundary edges need an int dat named bedgeNum");
+  // Without a bedge_type dat every boundary edge carries the exact vortex
+  // state: the vortex case declares no boundary classes
+  // (vortex/src/vortex.cpp:123-168) and applies the exact state on every
+  // boundary edge (:250-260).
+  std::vector<int> exact;
+  if (nb > 0 && !s.btype) exact.assign((size_t)nb, DG_BC_EXACT);
   check(dg_decl_mesh(s.ctx, cells->size, s.c2n ? s.c2n->data.data() : nullptr, ne,
                      ne ? s.e2c->data.data() : nullptr, ne ? (const int *)s.enum_->host : nullptr, nb,
                      nb ? s.b2c->data.data() : nullptr, nb ? (const int *)s.bnum->host : nullptr,
-                     nb ? (const int *)s.btype->host : nullptr),
+                     nb ? (s.btype ? (const int *)s.btype->host : exact.data()) : nullptr),
         "op_par_loop: mesh");
   s.cells = cells;
   s.mesh = true;
@@ -436,6 +442,117 @@ void op_par_loop_update_Q(char const *name, op_set cells, op_arg a0, op_arg a1, 
   sync_and_time(name, t0);
 }
 
+// --------------------------------------------------------------------------
+// The vortex case's loop forms (vortex/src/vortex.cpp:195-328), reached
+// through op_par_loop by name and argument count
+// --------------------------------------------------------------------------
+namespace {
+
+// The exact vortex of the vortex case: the declared vortex_x0, vortex_y0,
+// vortex_beta (defaults vortex/src/constants/constant_util.h:6-8) and the
+// free-stream speed 1 that vortex/src/set_ic.h:8 hardcodes.
+void set_vortex_exact_state() {
+  State &s = st();
+  auto val = [&](const char *n, double d) {
+    auto it = s.consts.find(n);
+    if (it == s.consts.end()) return d;
+    if (it->second.type_size != (int)sizeof(double) || it->second.dim != 1)
+      die(std::string("op_decl_const: ") + n + " must be one double");
+    double v;
+    std::memcpy(&v, it->second.bytes.data(), sizeof(double));
+    return v;
+  };
+  check(dg_set_exact_state(s.ctx, 1.0, val("vortex_beta", 5.0), val("vortex_x0", 5.0), val("vortex_y0", 0.0), 0.0),
+        "vortex exact state");
+}
+
+double gbl_double(const op_arg &a, const char *loop, int k) {
+  if (!a.gbl || a.dim != 1) die(std::string(loop) + ": argument " + std::to_string(k) + " must be a global double");
+  return *(const double *)a.gbl;
+}
+
+// set_ic(t, x, y, q, workingQ) (vortex/src/vortex.cpp:208-213)
+void vortex_set_ic(char const *name, op_set cells, const op_arg *a) {
+  auto t0 = std::chrono::steady_clock::now();
+  ensure_mesh(cells);
+  set_vortex_exact_state();
+  const int np = Np();
+  const double t = gbl_double(a[0], "set_ic", 0);
+  for (int k = 1; k < 5; k++) need_direct(a[k], "set_ic", k);
+  check(dg_set_ic_vortex(st().ctx, t, dptr(a[1], cells, np, name, 1), dptr(a[2], cells, np, name, 2),
+                         dptr(a[3], cells, 4 * np, name, 3), dptr(a[4], cells, 4 * np, name, 4)),
+        name);
+  sync_and_time(name, t0);
+}
+
+// get_neighbour_q(edgeNum, node_coords[0..1] via edge2nodes, then the
+// airfoil form's eight arguments) (vortex/src/vortex.cpp:236-247): the node
+// coordinates only serve the reference's orientation test, which the face
+// map replaces
+void vortex_get_neighbour_q(char const *name, op_set edges, const op_arg *a) {
+  for (int k = 1; k < 3; k++) {
+    if (!a[k].map || a[k].map->from != edges || a[k].idx != k - 1 || a[k].dat != a[1].dat || a[k].map != a[1].map)
+      die(std::string("get_neighbour_q: argument ") + std::to_string(k) + " must be the node coordinates through the "
+          "edge-to-node map with index " + std::to_string(k - 1));
+  }
+  op_par_loop_get_neighbour_q(name, edges, a[0], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]);
+}
+
+// get_bedge_q(bedgeNum, node_coords[0..1] via bedge2nodes, nodeX, nodeY, x,
+// y via bedge2cells, t, exteriorQ) (vortex/src/vortex.cpp:250-260): the exact
+// state at time t on every boundary edge
+void vortex_get_bedge_q(char const *name, op_set bedges, const op_arg *a) {
+  auto t0 = std::chrono::steady_clock::now();
+  State &s = st();
+  need_direct(a[0], "get_bedge_q", 0);
+  for (int k = 1; k < 3; k++)
+    if (!a[k].map || a[k].map->from != bedges || a[k].idx != k - 1)
+      die(std::string("get_bedge_q: argument ") + std::to_string(k) +
+          " must be the node coordinates through the boundary-edge-to-node map");
+  for (int k : {3, 4, 5, 6, 8}) need_mapped(a[k], bedges, 0, "get_bedge_q", k);
+  if (a[3].map != a[5].map || a[4].map != a[5].map || a[8].map != a[5].map)
+    die("get_bedge_q: every cell argument must use the same boundary-edge-to-cell map");
+  const double t = gbl_double(a[7], "get_bedge_q", 7);
+  op_set cells = a[5].map->to;
+  if (s.b2c != a[5].map || s.bnum != a[0].dat || s.btype) {
+    s.b2c = a[5].map;
+    s.bnum = a[0].dat;
+    s.btype = nullptr;
+    s.mesh = false;
+  }
+  ensure_mesh(cells);
+  set_vortex_exact_state();
+  const int np = Np(), nf = a[8].dat->dim / 4;
+  check(dg_get_bedge_q_exact(s.ctx, dptr(a[5], cells, np, name, 5), dptr(a[6], cells, np, name, 6), t,
+                             dptr(a[8], cells, 4 * nf, name, 8)),
+        name);
+  sync_and_time(name, t0);
+}
+
+// l2_error(x, y, q, t, four OP_INC globals, error) (vortex/src/vortex.cpp:
+// 314-324): the squared nodal errors against the exact vortex into error,
+// their sums added to the globals
+void vortex_l2_error(char const *name, op_set cells, const op_arg *a) {
+  auto t0 = std::chrono::steady_clock::now();
+  ensure_mesh(cells);
+  set_vortex_exact_state();
+  const int np = Np();
+  for (int k : {0, 1, 2, 8}) need_direct(a[k], "l2_error", k);
+  const double t = gbl_double(a[3], "l2_error", 3);
+  double err[4];
+  for (int k = 0; k < 4; k++) {
+    err[k] = gbl_double(a[4 + k], "l2_error", 4 + k);
+    if (a[4 + k].acc != OP_INC) die("l2_error: arguments 4-7 must be OP_INC globals");
+  }
+  check(dg_l2_error(st().ctx, dptr(a[0], cells, np, name, 0), dptr(a[1], cells, np, name, 1),
+                    dptr(a[2], cells, 4 * np, name, 2), t, dptr(a[8], cells, 4 * np, name, 8), err),
+        name);
+  for (int k = 0; k < 4; k++) *(double *)a[4 + k].gbl = err[k];
+  sync_and_time(name, t0);
+}
+
+}  // namespace
+
 // src/airfoil.cpp's op_decl_const(dim, type, &var): the name is the variable.
 void op_decl_const_expr(int dim, char const *type, int type_size, char *data, char const *expr) {
   std::string e;
@@ -451,21 +568,31 @@ void op_decl_const_expr(int dim, char const *type, int type_size, char *data, ch
   op_decl_const_char(dim, type, type_size, data, e.c_str());
 }
 
-// src/airfoil.cpp's op_par_loop(kernel, "name", set, args...): the loop by name.
+// src/airfoil.cpp's op_par_loop(kernel, "name", set, args...): the loop by name
+// (and vortex/src/vortex.cpp's, which differ in their argument lists).
 void op_par_loop_by_name(char const *name, op_set set, std::initializer_list<op_arg> args) {
   const std::string n = name ? name : "";
   const op_arg *a = args.begin();
   const size_t na = args.size();
-  auto need = [&](size_t k) {
+  auto need = [&](size_t k, size_t vortex_form = 0) {
     if (na != k)
-      die("op_par_loop(" + n + "): " + std::to_string(na) + " arguments, the loop takes " + std::to_string(k));
+      die("op_par_loop(" + n + "): " + std::to_string(na) + " arguments, the loop takes " + std::to_string(k) +
+          (vortex_form ? " (" + std::to_string(vortex_form) + " in the vortex case's form)" : std::string()));
   };
+  // the vortex case's forms, told apart by their argument counts
+  if (n == "set_ic" && na == 5) return vortex_set_ic(name, set, a);
+  if (n == "get_neighbour_q" && na == 11) return vortex_get_neighbour_q(name, set, a);
+  if (n == "get_bedge_q" && na == 9) return vortex_get_bedge_q(name, set, a);
+  if (n == "l2_error") {
+    need(9);
+    return vortex_l2_error(name, set, a);
+  }
   if (n == "init_grid") {
     need(18);
     op_par_loop_init_grid(name, set, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11],
                           a[12], a[13], a[14], a[15], a[16], a[17]);
   } else if (n == "set_ic") {
-    need(2);
+    need(2, 5);
     op_par_loop_set_ic(name, set, a[0], a[1]);
   } else if (n == "neighbour_zero") {
     need(1);
@@ -474,10 +601,10 @@ void op_par_loop_by_name(char const *name, op_set set, std::initializer_list<op_
     need(3);
     op_par_loop_calc_dt(name, set, a[0], a[1], a[2]);
   } else if (n == "get_neighbour_q") {
-    need(9);
+    need(9, 11);
     op_par_loop_get_neighbour_q(name, set, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8]);
   } else if (n == "get_bedge_q") {
-    need(6);
+    need(6, 9);
     op_par_loop_get_bedge_q(name, set, a[0], a[1], a[2], a[3], a[4], a[5]);
   } else if (n == "euler_rhs") {
     need(10);

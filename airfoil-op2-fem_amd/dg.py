@@ -265,6 +265,12 @@ class Context:
         n = self.state_numel()
         self._chk(lib.dg_set_ic(self._h, _dev(q, "q", n), _dev(workingQ, "workingQ", n)))
 
+    def set_ic_vortex(self, t, x, y, q, workingQ):
+        """vortex/src/set_ic.h over cells at time t (the vortex of set_exact_state)."""
+        K, Np, n = self.ncells, self.Np, self.state_numel()
+        self._chk(lib.dg_set_ic_vortex(self._h, float(t), _dev(x, "x", K * Np), _dev(y, "y", K * Np),
+                                       _dev(q, "q", n), _dev(workingQ, "workingQ", n)))
+
     def neighbour_zero(self, exteriorQ):
         self._chk(lib.dg_neighbour_zero(self._h, _dev(exteriorQ, "exteriorQ", self.ncells * self.NF * 4)))
 

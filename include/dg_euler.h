@@ -133,6 +133,12 @@ int dg_init_grid(dg_ctx *ctx, const double *d_cell_xy, double *d_nodeX,
                  double *d_ny, double *d_fscale);
 /* src/set_ic.h:1-12 over cells (uniform free stream). */
 int dg_set_ic(dg_ctx *ctx, double *d_q, double *d_workingQ);
+/* The vortex case's set_ic (vortex/src/set_ic.h:1-28, called at
+ * vortex/src/vortex.cpp:208-213) over cells: q = workingQ = the isentropic
+ * vortex of dg_set_exact_state at the nodes' coordinates (d_x, d_y:
+ * init_grid's [ncells][Np]) and time t. */
+int dg_set_ic_vortex(dg_ctx *ctx, double t, const double *d_x, const double *d_y, double *d_q,
+                     double *d_workingQ);
 /* src/neighbour_zero.h:1-5 over cells. */
 int dg_neighbour_zero(dg_ctx *ctx, double *d_exteriorQ);
 /* src/get_neighbour_q.h:1-89 over edges: exteriorQ += neighbour face traces

@@ -112,7 +112,16 @@ void op_par_loop_update_Q(char const *, op_set, op_arg, op_arg, op_arg, op_arg, 
  * ("&gam" -> gam, "Drw" -> Drw), as OP2's translator turns it into
  * op_decl_const2("gam", ...); an expression that is not a plain name throws.
  * op_par_loop(kernel, "name", set, args...): the loop named "name" (one of
- * the nine above, with the same arguments). */
+ * the nine above, with the same arguments), or one of the vortex case's
+ * forms (vortex/src/vortex.cpp:195-328), told apart by argument count:
+ * set_ic(t, x, y, q, workingQ); get_neighbour_q with the edge's two node
+ * coordinates first (11 arguments); get_bedge_q(bedgeNum, two node
+ * coordinates, nodeX, nodeY, x, y, t, exteriorQ), the exact vortex state on
+ * every boundary edge; l2_error(x, y, q, t, four OP_INC globals, error).
+ * The vortex is the one of the constants vortex_x0, vortex_y0, vortex_beta
+ * advected at the free-stream speed 1 of vortex/src/set_ic.h:8; a mesh
+ * whose boundary edges have no int dat named bedge_type gives every one of
+ * them the exact state. */
 void op_decl_const_expr(int dim, char const *type, int type_size, char *data, char const *expr);
 template <class T>
 void op_decl_const_named(int dim, char const *type, T *data, char const *expr) {

@@ -1,7 +1,8 @@
 // Host-only checks of include/op2_dg.h's source form (no device work): the
 // name op_decl_const takes from its pointer expression, and the errors of
-// op_par_loop for an unknown loop name or a wrong argument count, which
-// throw before any loop runs.  Prints "source form checks ok".
+// op_par_loop for an unknown loop name or a wrong argument count (either
+// form, the airfoil's or the vortex case's), which throw before any loop
+// runs.  Prints "source form checks ok".
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -48,6 +49,12 @@ int main(int argc, char **argv) {
   });
   expect_throw("euler_rhs with two arguments", "the loop takes 10", [&] {
     op_par_loop(euler_rhs, "euler_rhs", cells, g, g);
+  });
+  expect_throw("get_bedge_q with three arguments", "the loop takes 6 (9 in the vortex case's form)", [&] {
+    op_par_loop(get_bedge_q, "get_bedge_q", cells, g, g, g);
+  });
+  expect_throw("l2_error with one argument", "1 arguments, the loop takes 9", [&] {
+    op_par_loop(l2_error, "l2_error", cells, g);
   });
   op_exit();
   std::printf("%s\n", failures ? "source form checks FAILED" : "source form checks ok");

@@ -34,6 +34,37 @@ __device__ __forceinline__ void copy_to_lds(const double *src, double *dst, int 
   }
 }
 
+// The same copy issued from inline assembly, for a kernel whose own LDS
+// accesses follow the copy before its data is needed: the compiler does not
+// see these loads, so it neither waits for them before those LDS accesses
+// (it cannot tell that they touch other LDS memory, and drained every load
+// in flight there) nor counts them in its waits for later loads (the copy
+// is older than those, and loads complete in order, so the waits it
+// computes for them stay sufficient).  The caller waits for the copy
+// itself (s_waitcnt vmcnt(0)) before reading dst, and makes sure no LDS
+// access to dst is pending when it issues the copy.  M0 takes dst's LDS
+// address (M0 is reserved: the compiler writes it right before each of its
+// own uses, so it cannot be clobbered under it); the s_nop covers the wait
+// state between writing M0 and the direct-to-LDS load that reads it.
+template <int MAXCH>
+__device__ __forceinline__ void copy_to_lds_unseen(const double *src, double *dst, int nch, int lane) {
+#pragma unroll
+  for (int r = 0; r < (MAXCH + 63) / 64; r++) {
+    const int ch = min(64 * r + lane, nch - 1);
+    const unsigned lds = (unsigned)(size_t)(lds_void_t *)(dst + 128 * r);
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(lds), "v"(src + 2 * ch)
+                 : "memory");
+  }
+}
+
+// Waits for the wave's own LDS accesses in flight; with the memory clobber,
+// also a compiler barrier for LDS accesses.  Between phases of a kernel
+// whose waves each own their LDS, it replaces __syncthreads(), whose
+// workgroup release fence would also wait for the wave's global stores.
+__device__ __forceinline__ void lds_wait() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
 // Doubles of LDS that copy_to_lds<MAXCH> writes (whole KiB per instruction).
 template <int MAXCH>
 constexpr int copy_to_lds_doubles() {

@@ -108,9 +108,10 @@ constexpr int mfma_lds_doubles() {
 
 // A workgroup is 1, 2 or 4 waves (16, 32 or 64 cells per workgroup visit);
 // each wave owns one 16-cell tile of the workgroup's block of tiles and its
-// own LDS tiles.  The walk over blocks of tiles is uniform across the
-// workgroup, so every wave reaches every barrier; waves whose tile lies past
-// the end only skip the work.
+// own LDS tiles, so the waves of a workgroup never wait for each other (a
+// phase waits only for its own wave's LDS accesses, lds_wait); the walk
+// over blocks of tiles is uniform across the workgroup, and waves whose
+// tile lies past the end only skip the work.
 template <int N, bool GATHER, int EPI, bool CG>
 __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const double *__restrict__ ops,
                                                        unsigned long long fmask4, Phys ph0,
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
   const int nblk = (ntiles + wpb - 1) / wpb;  // blocks of wpb tiles
   // The it_t-th tile of this wave: 0 = past the walk (uniform across the
   // workgroup), 1 = a tile of cells [e0, e0 + ne) of case cs, 2 = past the
-  // last tile (no work, but every barrier is reached).
+  // last tile (no work).
   auto locate = [&](int it_t, int &e0_, int &ne_, int &cs_) -> int {
     const int blk = walk == 0 ? xcd_tile(it_t, nblk) : (int)blockIdx.x + it_t * (int)gridDim.x;
     e0_ = 0; ne_ = 0; cs_ = 0;
@@ -245,6 +246,9 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       if (EPI != EPI_RHS)
         dt = a.dt_fixed ? a.dt_fixed[cs] : a.cfl / __longlong_as_double((long long)a.dt1_cur[cs]);
       ph_case = cs;
+      // used here, so that no load of them is in flight past this branch
+      asm volatile("" : : "v"(ph.gam), "v"(ph.bc_r), "v"(ph.bc_u), "v"(ph.bc_v), "v"(ph.bc_e), "v"(ph.bc_p),
+                   "v"(dt));
     }
     if (EPI == EPI_STAGE2 && active && cs != cfl_case) {
       // flush the running CFL max of the previous case (wave-uniform branch)
@@ -256,7 +260,11 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
       cfl_case = cs;
       cfl_bits = 0ull;
     }
-    __syncthreads();  // the previous tile's volume products are done with the LDS tiles
+    // The previous tile's volume products are done with the LDS tiles.  A
+    // wave's LDS tiles are its own, so its phases need only its own LDS
+    // accesses complete, not a workgroup barrier (whose release fence would
+    // also wait for the previous tile's stores).
+    lds_wait();
 
     // the tile's state records go to LDS while the face phase runs; the
     // Euler-flux phase reads them there instead of waiting on its own loads.
@@ -268,7 +276,8 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     // 1.147 ms per step over four alternating runs on two boxes; the copy
     // alone or the epilogue alone measured no gain, priority 3 the same as 2)
     __builtin_amdgcn_s_setprio(2);
-    copy_to_lds<TE * Np * 2>(a.w + (size_t)e0 * Np * 4, sW, active ? ne * Np * 2 : 1, lane);
+    // (issued unseen by the compiler: its waits in the face phase stay exact)
+    copy_to_lds_unseen<TE * Np * 2>(a.w + (size_t)e0 * Np * 4, sW, active ? ne * Np * 2 : 1, lane);
 
     // ---- Roe fluxes per (cell, face node), written in the A-operand layout
     {
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         }
       }
     }
-    __syncthreads();
+    lds_wait();
 
     // ---- LIFT products: the accumulators start from the surface term
     dbl4 acc[G];
@@ -370,7 +379,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
 #pragma unroll
       for (int ks = 0; ks < KF; ks++) acc[g] = mfma16x16x4(sL[g * GL + CSL * (4 * ks + kk) + jj], bL[ks], acc[g]);
     }
-    __syncthreads();  // the Euler fluxes overwrite the face-flux tiles
+    lds_wait();  // the Euler fluxes overwrite the face-flux tiles
 
     // ---- Euler fluxes per (cell, node), written in the A-operand layout.
     // The padding columns are never read (opA supplies their zeros), so a
@@ -419,7 +428,7 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
         }
       }
     }
-    __syncthreads();
+    lds_wait();
 
     // the next tile's face-map codes load while this tile's volume products run
     int e0n, nen, csn;

@@ -512,18 +512,21 @@ __global__ __launch_bounds__(kPairThreads, 3) void adj_pair_kernel(AdjArgs a, Ph
       gfp = -(pp.x * R[0] + pp.y * R[1] + pp.z * R[2] + pp.w * R[3]);
     }
     sG[0][t] = gnx; sG[1][t] = gny; sG[2][t] = gfo; sG[3][t] = gfp;
-    __syncthreads();
+    lds_wait();
     if (act && i == 0) {
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
       for (int j = 0; j < Nfp; j++) {
         s0 += sG[0][t + j]; s1 += sG[1][t + j]; s2 += sG[2][t + j]; s3 += sG[3][t + j];
       }
-      a.g_nxf[fo] += a.scale * s0;
-      a.g_nyf[fo] += a.scale * s1;
-      a.g_fsf[fo] += a.scale * s2;
-      a.g_fsf[fp] += a.scale * s3;
+      // the four accumulators (two faces, three arrays: never the same
+      // element) are read together, then written: one round trip
+      const double v0 = a.g_nxf[fo], v1 = a.g_nyf[fo], v2 = a.g_fsf[fo], v3 = a.g_fsf[fp];
+      a.g_nxf[fo] = v0 + a.scale * s0;
+      a.g_nyf[fo] = v1 + a.scale * s1;
+      a.g_fsf[fo] = v2 + a.scale * s2;
+      a.g_fsf[fp] = v3 + a.scale * s3;
     }
-    __syncthreads();
+    lds_wait();  // one-wave workgroups: the wave's own LDS accesses
   }
 }
 

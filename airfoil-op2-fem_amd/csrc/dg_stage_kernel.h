@@ -440,8 +440,9 @@ __global__ __launch_bounds__(256, 3) void rhs_mfma_kernel(StageArgs a, const dou
     // here, so that only w and q are left to each group's epilogue (724x724
     // vortex: stage 1 401-405 -> 387-396 us; the same in stage 0, or rx and
     // sx alone in stage 2, measured 1.5% and 5% slower, all four in stage 2
-    // spill).
-    constexpr bool kMetricsAhead = !CG && EPI == EPI_STAGE1;
+    // spill).  At N=4 the held factors spill in stage 1 too (582 -> 566 us
+    // without them), so there the epilogue loads them.
+    constexpr bool kMetricsAhead = !CG && EPI == EPI_STAGE1 && N <= 3;
     double hrx[G], hsx[G], hry[G], hsy[G];
 #pragma unroll
     for (int g = 0; g < G; g++) {
